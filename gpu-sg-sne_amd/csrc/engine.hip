@@ -1,0 +1,903 @@
+// engine.hip -- per-context state, the gradient-descent loop and the C-ABI.
+//
+// One context = one embedding job on the current device with its own stream.
+// The loop follows kl_minimization (CU/gradient_descend.cu:102-245;
+// HY/gradient_descendHYB.cpp:160-244): eta = 200, momentum 0.5 -> 0.8 after
+// iteration 250, alpha -> 1 after iteration early_iter, mean-centring every
+// iteration (HYB semantics), h-driven grid size (HY/qq.cpp:159-166).
+//
+// Per iteration there is one device->host transfer of the (min, max) of Y,
+// needed to size the grid; the attractive SpMV is issued before the host
+// waits for it, so the GPU stays busy across the round trip.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/sgtsne.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace sgt {
+
+static thread_local char g_err[1024] = "no error";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// getBestGridSize (HY/qq.cpp:108-130, CU/utils.cu:194-215): the FFT-friendly
+// size table fixes the interpolation grid and must match the reference.
+static const int kGridSizes[] = {
+    8,   9,   10,  11,  12,  13,  14,  15,  16,  20,  25,  26,  28,  32,  33,  35,  36,
+    39,  40,  42,  44,  45,  48,  49,  50,  52,  54,  55,  56,  60,  63,  64,  65,  66,
+    70,  72,  75,  77,  78,  80,  84,  88,  90,  91,  96,  98,  99,  100, 104, 105, 108,
+    110, 112, 117, 120, 125, 126, 130, 132, 135, 140, 144, 147, 150, 154, 156, 160, 165,
+    168, 175, 176, 180, 182, 189, 192, 195, 196, 198, 200, 208, 210, 216, 220, 224, 225,
+    231, 234, 240, 245, 250, 252, 260, 264, 270, 273, 275, 280, 288, 294, 297, 300, 308,
+    312, 315, 320, 325, 330, 336, 343, 350, 351, 352, 360, 364, 375, 378, 385, 390, 392,
+    396, 400, 416, 420, 432, 440, 441, 448, 450, 455, 462, 468, 480, 490, 495, 500, 504,
+    512};
+
+int best_grid_size(int nGrid) {
+  const int cnt = (int)(sizeof(kGridSizes) / sizeof(kGridSizes[0]));
+  for (int i = 0; i < cnt; i++)
+    if (nGrid + 2 <= kGridSizes[i]) return kGridSizes[i] - 2;
+  return kGridSizes[cnt - 1] - 2;
+}
+
+// CUSP's compute_optimal_entries_per_row rule (the reference calls it at
+// CU/matrix_converter.cpp:237): the ELL width is the smallest w at which the
+// rows longer than w are fewer than max(4096, n / 3).
+static int optimal_ell_width(int n, const std::vector<int32_t> &rowptr) {
+  int maxlen = 0;
+  for (int i = 0; i < n; i++) maxlen = std::max(maxlen, rowptr[i + 1] - rowptr[i]);
+  std::vector<int64_t> hist((size_t)maxlen + 1, 0);
+  for (int i = 0; i < n; i++) hist[rowptr[i + 1] - rowptr[i]]++;
+  int64_t rows = n;
+  for (int w = 0; w < maxlen; w++) {
+    rows -= hist[w];  // rows with more than w entries
+    if (3.0 * (double)rows < (double)n || rows < 4096) return w;
+  }
+  return maxlen;
+}
+
+struct PhaseEvents {
+  hipEvent_t e[6];  // attr start, attr end, s2g end, conv end, g2s end, update end
+  bool valid = false;
+};
+
+}  // namespace sgt
+
+using namespace sgt;
+
+struct sgtsne_ctx {
+  sgtsne_params p{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int n = 0, d = 2, ds = 2;
+  int64_t nnz = 0;
+  int iter = 0;
+  bool have_P = false, have_Y = false;
+
+  // P (CSR always; COO rows / hybrid as requested)
+  int32_t *rowptr = nullptr, *col = nullptr;
+  float *val = nullptr;
+  int32_t *coo_row = nullptr;
+  int ell_w = 0;
+  int32_t *ell_col = nullptr;
+  float *ell_val = nullptr;
+  int64_t tail_nnz = 0;
+  int32_t *tail_row = nullptr, *tail_col = nullptr;
+  float *tail_val = nullptr;
+  int seg_group = 16;
+
+  // per-point state, AoS with stride ds
+  float *Y = nullptr, *uY = nullptr, *gains = nullptr, *Fattr = nullptr, *Frep = nullptr;
+
+  // reductions
+  RedWork rw{};
+  DevScalars *sc = nullptr;       // device
+  DevScalars *sc_host = nullptr;  // pinned
+  hipEvent_t ev_scal = nullptr;
+
+  // grid workspace
+  size_t grid_cap = 0, w_cap = 0, k_cap = 0;
+  float *V = nullptr, *Phi = nullptr;
+  ConvBuffers cb{};
+
+  // timing
+  bool timing = true;
+  PhaseEvents pe[2];
+  double acc_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  double *time_info = nullptr;  // optional per-iteration output (sgtsne_run)
+  int time_info_base = 0;
+  GridParams last_grid{};
+
+  // compute_error history
+  std::vector<double> errors;
+};
+
+namespace {
+
+int alloc_dev(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return SGTSNE_ERR_NOMEM;
+  }
+  return SGTSNE_OK;
+}
+
+template <class T>
+void free_dev(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void free_P(sgtsne_ctx *c) {
+  free_dev(c->rowptr);
+  free_dev(c->col);
+  free_dev(c->val);
+  free_dev(c->coo_row);
+  free_dev(c->ell_col);
+  free_dev(c->ell_val);
+  free_dev(c->tail_row);
+  free_dev(c->tail_col);
+  free_dev(c->tail_val);
+  c->have_P = false;
+  c->nnz = 0;
+  c->ell_w = 0;
+  c->tail_nnz = 0;
+}
+
+int check_params(const sgtsne_params *p) {
+  SGT_CHECK(p != nullptr, SGTSNE_ERR_ARG, "params is NULL");
+  SGT_CHECK(p->d >= 1 && p->d <= 3, SGTSNE_ERR_ARG, "d must be 1, 2 or 3 (got %d)", p->d);
+  SGT_CHECK(p->format == SGTSNE_FORMAT_CSR || p->format == SGTSNE_FORMAT_COO ||
+                p->format == SGTSNE_FORMAT_HYBRID || p->format == SGTSNE_FORMAT_CSR_SEG,
+            SGTSNE_ERR_ARG, "format %d unsupported (BSR is a stub in the reference)", p->format);
+  SGT_CHECK(p->bs == 0, SGTSNE_ERR_ARG, "block formats are not supported (bs must be 0)");
+  SGT_CHECK(p->max_iter >= 0 && p->early_iter >= -1, SGTSNE_ERR_ARG, "bad iteration counts");
+  SGT_CHECK(p->lambda > 0, SGTSNE_ERR_ARG, "lambda must be positive");
+  SGT_CHECK(p->grid_mode == SGTSNE_GRID_H || p->grid_mode == SGTSNE_GRID_FIXED, SGTSNE_ERR_ARG,
+            "bad grid_mode");
+  SGT_CHECK(p->layout == SGTSNE_LAYOUT_SOA || p->layout == SGTSNE_LAYOUT_AOS, SGTSNE_ERR_ARG,
+            "bad layout");
+  return SGTSNE_OK;
+}
+
+double default_h(int d) { return d == 1 ? 0.5 : d == 2 ? 0.7 : 1.2; }
+
+// Host side of one iteration's grid mapping (HY/qq.cpp:159-166, HY/nuconv.cpp:28-47).
+int make_grid(sgtsne_ctx *c, const DevScalars &s, GridParams &g) {
+  const int d = c->d;
+  double maxy = 0.0;
+  for (int a = 0; a < d; a++) {
+    SGT_CHECK(std::isfinite(s.minv[a]) && std::isfinite(s.maxv[a]), SGTSNE_ERR_NUMERIC,
+              "embedding is not finite (iteration %d)", c->iter);
+    maxy = std::max(maxy, s.maxv[a] - s.minv[a]);
+  }
+  SGT_CHECK(maxy > 0.0, SGTSNE_ERR_NUMERIC, "embedding collapsed to a point (iteration %d)", c->iter);
+  int nGrid;
+  if (c->p.grid_mode == SGTSNE_GRID_FIXED) {
+    nGrid = best_grid_size(c->p.ng);
+  } else {
+    const double h = c->p.h > 0 ? c->p.h : default_h(d);
+    nGrid = std::max((int)std::ceil(maxy / h), 14);
+    nGrid = best_grid_size(nGrid);
+  }
+  g.d = d;
+  g.nGrid = nGrid;
+  g.ng = nGrid + 2;
+  int L = 1, lg = 0;
+  while (L < 2 * g.ng - 1) { L <<= 1; ++lg; }
+  g.L = L;
+  g.logL = lg;
+  for (int a = 0; a < 3; a++) {
+    g.minv[a] = a < d ? (float)s.minv[a] : 0.0f;
+    g.center[a] = a < d ? (float)(0.5 * (s.minv[a] + s.maxv[a])) : 0.0f;
+  }
+  g.scale = (float)((double)(nGrid - 1) / maxy);
+  g.gmax = std::nextafter((float)(nGrid - 1), 0.0f);
+  g.hgrid = maxy / ((double)(nGrid - 1) - DBL_EPSILON);
+  return SGTSNE_OK;
+}
+
+int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
+  const int m = c->d + 1;
+  const size_t G = c->d == 1 ? (size_t)g.ng : c->d == 2 ? (size_t)g.ng * g.ng : (size_t)g.ng * g.ng * g.ng;
+  const size_t gb = G * m * sizeof(float);
+  size_t wb = 0, kb = 0;
+  conv_workspace_bytes(c->d, g.ng, g.L, &wb, &kb);
+  if (gb > c->grid_cap || wb > c->w_cap || kb > c->k_cap) {
+    SGT_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (gb > c->grid_cap) {
+    free_dev(c->V);
+    free_dev(c->Phi);
+    SGT_TRY(alloc_dev((void **)&c->V, gb));
+    SGT_TRY(alloc_dev((void **)&c->Phi, gb));
+    c->grid_cap = gb;
+  }
+  if (wb > c->w_cap) {
+    free_dev(c->cb.W);
+    SGT_TRY(alloc_dev((void **)&c->cb.W, wb));
+    c->w_cap = wb;
+  }
+  if (kb > c->k_cap) {
+    free_dev(c->cb.Kt1);
+    free_dev(c->cb.Kt2);
+    free_dev(c->cb.Kh);
+    SGT_TRY(alloc_dev((void **)&c->cb.Kt1, kb));
+    SGT_TRY(alloc_dev((void **)&c->cb.Kt2, kb));
+    SGT_TRY(alloc_dev((void **)&c->cb.Kh, kb));
+    c->k_cap = kb;
+  }
+  return SGTSNE_OK;
+}
+
+int attractive(sgtsne_ctx *c) {
+  switch (c->p.format) {
+    case SGTSNE_FORMAT_CSR_SEG:
+      return launch_attr_csr_seg(c->stream, c->d, c->seg_group, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
+    case SGTSNE_FORMAT_CSR:
+      return launch_attr_csr_row(c->stream, c->d, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
+    case SGTSNE_FORMAT_COO:
+      SGT_TRY(launch_fill(c->stream, c->Fattr, (int64_t)c->n * c->ds, 0.0f));
+      return launch_attr_coo(c->stream, c->d, c->nnz, c->coo_row, c->col, c->val, c->Y, c->Fattr);
+    case SGTSNE_FORMAT_HYBRID:
+      SGT_TRY(launch_attr_ell(c->stream, c->d, c->n, c->ell_w, c->ell_col, c->ell_val, c->Y, c->Fattr));
+      return launch_attr_coo(c->stream, c->d, c->tail_nnz, c->tail_row, c->tail_col, c->tail_val, c->Y, c->Fattr);
+    default:
+      SGT_CHECK(false, SGTSNE_ERR_ARG, "bad format");
+  }
+}
+
+// Repulsive term for the current (centred) Y: S2G -> convolution -> G2S.
+int repulsive(sgtsne_ctx *c, const GridParams &g, PhaseEvents *pe) {
+  const int m = c->d + 1;
+  const int64_t G = c->d == 1 ? g.ng : c->d == 2 ? (int64_t)g.ng * g.ng : (int64_t)g.ng * g.ng * g.ng;
+  SGT_TRY(ensure_grid(c, g));
+  SGT_TRY(launch_fill(c->stream, c->V, G * m, 0.0f));
+  SGT_TRY(launch_s2g_atomic(c->stream, g, c->n, c->Y, c->V));
+  if (pe) SGT_HIP(hipEventRecord(pe->e[2], c->stream));
+  SGT_TRY(launch_conv(c->stream, g, m, c->V, c->Phi, c->cb));
+  if (pe) SGT_HIP(hipEventRecord(pe->e[3], c->stream));
+  SGT_TRY(launch_g2s_force(c->stream, g, c->n, c->Y, c->Phi, c->Frep, c->rw, c->sc));
+  if (pe) SGT_HIP(hipEventRecord(pe->e[4], c->stream));
+  c->last_grid = g;
+  return SGTSNE_OK;
+}
+
+int center_and_fetch(sgtsne_ctx *c, int apply_mean, PhaseEvents *pe = nullptr) {
+  SGT_TRY(launch_center_minmax(c->stream, c->d, c->n, c->Y, apply_mean, c->rw, c->sc));
+  if (pe) {
+    SGT_HIP(hipEventRecord(pe->e[5], c->stream));
+    pe->valid = true;
+  }
+  SGT_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  SGT_HIP(hipEventRecord(c->ev_scal, c->stream));
+  return SGTSNE_OK;
+}
+
+void harvest_timing(sgtsne_ctx *c, PhaseEvents &pe, int it) {
+  if (!pe.valid) return;
+  float t[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < 5; k++) (void)hipEventElapsedTime(&t[k], pe.e[k], pe.e[k + 1]);
+  // t: attr, s2g(+fill), conv, g2s, update(+centre)
+  c->acc_ms[0] += t[0];
+  c->acc_ms[1] += t[1];
+  c->acc_ms[2] += t[2];
+  c->acc_ms[3] += t[3];
+  c->acc_ms[4] += t[4];
+  c->acc_ms[5] += t[0] + t[1] + t[2] + t[3] + t[4];
+  c->acc_ms[6] += 1;
+  if (c->time_info) {
+    double *ti = c->time_info + 7 * (size_t)(it - c->time_info_base);
+    ti[0] = t[0];
+    ti[1] = t[1];
+    ti[2] = t[2];
+    ti[3] = t[3];
+    ti[4] = 0.0;  // forces are fused into G2S
+    ti[5] = t[1] + t[2] + t[3];
+    ti[6] = 0.0;
+  }
+  pe.valid = false;
+}
+
+int iterate(sgtsne_ctx *c) {
+  const int it = c->iter;
+  const double alpha = it <= c->p.early_iter ? c->p.alpha : 1.0;
+  const double momentum = it <= 250 ? 0.5 : 0.8;
+  const double eta = 200.0;
+  PhaseEvents *pe = c->timing ? &c->pe[it & 1] : nullptr;
+  if (pe) SGT_HIP(hipEventRecord(pe->e[0], c->stream));
+  SGT_TRY(attractive(c));
+  if (pe) SGT_HIP(hipEventRecord(pe->e[1], c->stream));
+  // wait for the (min, max) of this iteration's Y (issued by the previous centring)
+  SGT_HIP(hipEventSynchronize(c->ev_scal));
+  if (c->timing) harvest_timing(c, c->pe[(it + 1) & 1], it - 1);
+  const DevScalars s = *c->sc_host;
+  GridParams g{};
+  SGT_TRY(make_grid(c, s, g));
+  SGT_TRY(repulsive(c, g, pe));
+  if (c->p.compute_error > 0 && it % 10 == 0) {
+    // relative L2 error of the interpolated repulsion (CU/gradient_descend.cu:206-215)
+    SGT_HIP(hipStreamSynchronize(c->stream));
+    const size_t nb = (size_t)c->n * c->ds;
+    std::vector<float> fi(nb), fe(nb);
+    DevScalars sh{};
+    SGT_HIP(hipMemcpy(fi.data(), c->Frep, nb * sizeof(float), hipMemcpyDeviceToHost));
+    SGT_HIP(hipMemcpy(&sh, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+    float *tmp = nullptr;
+    SGT_TRY(alloc_dev((void **)&tmp, nb * sizeof(float)));
+    int st = launch_rep_exact(c->stream, c->d, c->n, c->Y, tmp, c->rw, c->sc);
+    if (st == SGTSNE_OK) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipMemcpy(fe.data(), tmp, nb * sizeof(float), hipMemcpyDeviceToHost);
+      DevScalars se{};
+      (void)hipMemcpy(&se, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost);
+      double num = 0, den = 0;
+      for (int i = 0; i < c->n; i++)
+        for (int a = 0; a < c->d; a++) {
+          double x = fi[(size_t)i * c->ds + a] / sh.Z, y = fe[(size_t)i * c->ds + a] / se.zexact;
+          num += (x - y) * (x - y);
+          den += y * y;
+        }
+      c->errors.push_back(den > 0 ? std::sqrt(num / den) : 0.0);
+    }
+    free_dev(tmp);
+    SGT_TRY(st);
+  }
+  SGT_TRY(launch_update(c->stream, c->d, c->n, c->Y, c->uY, c->gains, c->Fattr, c->Frep,
+                        (float)alpha, (float)momentum, (float)eta, c->rw, c->sc));
+  SGT_TRY(center_and_fetch(c, 1, pe));
+  c->iter++;
+  return SGTSNE_OK;
+}
+
+void pack_y(const float *y, int layout, int n, int d, int ds, std::vector<float> &out) {
+  out.assign((size_t)n * ds, 0.0f);
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < d; a++)
+      out[(size_t)i * ds + a] = layout == SGTSNE_LAYOUT_AOS ? y[(size_t)i * d + a] : y[(size_t)a * n + i];
+}
+
+void unpack_y(const std::vector<float> &in, int layout, int n, int d, int ds, float *y, double scale = 1.0) {
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < d; a++) {
+      float v = (float)(in[(size_t)i * ds + a] * scale);
+      if (layout == SGTSNE_LAYOUT_AOS) y[(size_t)i * d + a] = v;
+      else y[(size_t)a * n + i] = v;
+    }
+}
+
+int build_formats(sgtsne_ctx *c) {
+  if (c->p.format == SGTSNE_FORMAT_COO) {
+    SGT_TRY(alloc_dev((void **)&c->coo_row, sizeof(int32_t) * (size_t)std::max<int64_t>(c->nnz, 1)));
+    SGT_TRY(launch_rowids(c->stream, c->n, c->rowptr, c->coo_row));
+  } else if (c->p.format == SGTSNE_FORMAT_HYBRID) {
+    std::vector<int32_t> rp((size_t)c->n + 1);
+    SGT_HIP(hipMemcpyAsync(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, c->stream));
+    SGT_HIP(hipStreamSynchronize(c->stream));
+    int w = c->p.ell_width > 0 ? c->p.ell_width : optimal_ell_width(c->n, rp);
+    c->ell_w = w;
+    int64_t tail = 0;
+    for (int i = 0; i < c->n; i++) tail += std::max(0, rp[i + 1] - rp[i] - w);
+    c->tail_nnz = tail;
+    SGT_TRY(alloc_dev((void **)&c->ell_col, sizeof(int32_t) * (size_t)w * c->n));
+    SGT_TRY(alloc_dev((void **)&c->ell_val, sizeof(float) * (size_t)w * c->n));
+    int32_t *cnt = nullptr, *off = nullptr;
+    SGT_TRY(alloc_dev((void **)&cnt, sizeof(int32_t) * (size_t)c->n));
+    SGT_TRY(alloc_dev((void **)&off, sizeof(int32_t) * (size_t)c->n));
+    SGT_TRY(alloc_dev((void **)&c->tail_row, sizeof(int32_t) * (size_t)std::max<int64_t>(tail, 1)));
+    SGT_TRY(alloc_dev((void **)&c->tail_col, sizeof(int32_t) * (size_t)std::max<int64_t>(tail, 1)));
+    SGT_TRY(alloc_dev((void **)&c->tail_val, sizeof(float) * (size_t)std::max<int64_t>(tail, 1)));
+    int st = launch_build_ell(c->stream, c->n, w, c->rowptr, c->col, c->val, c->ell_col, c->ell_val, cnt);
+    if (st == SGTSNE_OK) st = device_exclusive_scan(c->stream, cnt, off, c->n);
+    if (st == SGTSNE_OK)
+      st = launch_build_tail(c->stream, c->n, w, c->rowptr, c->col, c->val, off, c->tail_row, c->tail_col, c->tail_val);
+    (void)hipStreamSynchronize(c->stream);
+    free_dev(cnt);
+    free_dev(off);
+    SGT_TRY(st);
+  }
+  // segmented CSR: lanes per row from the mean row length (power of two, 4..64)
+  int g = 4;
+  const double mean = c->n > 0 ? (double)c->nnz / c->n : 0.0;
+  while (g < 64 && g * 6 < mean) g <<= 1;
+  c->seg_group = g;
+  return SGTSNE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgtsne_params_default(sgtsne_params *p) {
+  SGT_CHECK(p != nullptr, SGTSNE_ERR_ARG, "params is NULL");
+  std::memset(p, 0, sizeof(*p));
+  p->d = 2;
+  p->lambda = 1.0;
+  p->alpha = 12.0;
+  p->max_iter = 1000;
+  p->early_iter = 250;
+  p->n = 0;
+  p->h = -1.0;
+  p->drop_leaf = 0;
+  p->np = 0;
+  p->format = SGTSNE_FORMAT_CSR_SEG;
+  p->bs = 0;
+  p->compute_error = -1;
+  p->ng = 50;
+  p->grid_mode = SGTSNE_GRID_H;
+  p->layout = SGTSNE_LAYOUT_SOA;
+  p->seed = 0;
+  p->ell_width = 0;
+  return SGTSNE_OK;
+}
+
+const char *sgtsne_last_error(void) { return g_err; }
+
+const char *sgtsne_version(void) { return "sgtsne-mi355x 0.1 (gfx950, HIP)"; }
+
+int sgtsne_ctx_create(sgtsne_ctx **out, const sgtsne_params *params) {
+  SGT_CHECK(out != nullptr, SGTSNE_ERR_ARG, "ctx pointer is NULL");
+  *out = nullptr;
+  SGT_TRY(check_params(params));
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  SGT_CHECK(e == hipSuccess && ndev > 0, SGTSNE_ERR_NODEV, "no HIP device visible");
+  sgtsne_ctx *c = new sgtsne_ctx();
+  c->p = *params;
+  c->d = params->d;
+  c->ds = point_stride(c->d);
+  (void)hipGetDevice(&c->device);
+  int st = SGTSNE_OK;
+  auto fail = [&](int code) {
+    sgtsne_ctx_destroy(c);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(SGTSNE_ERR_HIP);
+  }
+  c->rw.max_blocks = 65536;
+  if ((st = alloc_dev((void **)&c->rw.part, sizeof(double) * 8 * c->rw.max_blocks))) return fail(st);
+  if ((st = alloc_dev((void **)&c->rw.ticket, sizeof(unsigned) * 4))) return fail(st);
+  if ((st = alloc_dev((void **)&c->sc, sizeof(DevScalars)))) return fail(st);
+  if (hipMemset(c->rw.ticket, 0, sizeof(unsigned) * 4) != hipSuccess ||
+      hipMemset(c->sc, 0, sizeof(DevScalars)) != hipSuccess) {
+    set_error("hipMemset failed");
+    return fail(SGTSNE_ERR_HIP);
+  }
+  if (hipHostMalloc((void **)&c->sc_host, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(SGTSNE_ERR_NOMEM);
+  }
+  std::memset(c->sc_host, 0, sizeof(DevScalars));
+  if (hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming) != hipSuccess) {
+    set_error("hipEventCreate failed");
+    return fail(SGTSNE_ERR_HIP);
+  }
+  for (auto &pe : c->pe)
+    for (auto &ev : pe.e)
+      if (hipEventCreate(&ev) != hipSuccess) {
+        set_error("hipEventCreate failed");
+        return fail(SGTSNE_ERR_HIP);
+      }
+  if ((st = alloc_dev((void **)&c->cb.tw, sizeof(float2) * 512))) return fail(st);
+  if ((st = init_twiddles(c->stream, c->cb.tw))) return fail(st);
+  *out = c;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_destroy(sgtsne_ctx *c) {
+  if (!c) return SGTSNE_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_P(c);
+  free_dev(c->Y);
+  free_dev(c->uY);
+  free_dev(c->gains);
+  free_dev(c->Fattr);
+  free_dev(c->Frep);
+  free_dev(c->rw.part);
+  free_dev(c->rw.ticket);
+  free_dev(c->sc);
+  free_dev(c->V);
+  free_dev(c->Phi);
+  free_dev(c->cb.W);
+  free_dev(c->cb.Kt1);
+  free_dev(c->cb.Kt2);
+  free_dev(c->cb.Kh);
+  free_dev(c->cb.tw);
+  if (c->sc_host) (void)hipHostFree(c->sc_host);
+  if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
+  for (auto &pe : c->pe)
+    for (auto &ev : pe.e)
+      if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_upload_P(sgtsne_ctx *c, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                        const float *val, int symmetrize) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  SGT_CHECK(n > 0 && nnz >= 0 && rowptr && (nnz == 0 || (col && val)), SGTSNE_ERR_ARG, "bad P arguments");
+  SGT_CHECK(nnz < (int64_t)INT32_MAX, SGTSNE_ERR_ARG, "nnz exceeds the int32 index range");
+  SGT_CHECK(rowptr[0] == 0 && rowptr[n] == nnz, SGTSNE_ERR_ARG, "rowptr inconsistent with nnz");
+  // every index is checked on the host: the kernels gather Y[col] unguarded
+  for (int i = 0; i < n; i++)
+    SGT_CHECK(rowptr[i] <= rowptr[i + 1], SGTSNE_ERR_ARG, "rowptr decreases at row %d", i);
+  {
+    int bad = 0;
+    for (int64_t t = 0; t < nnz; t++) bad += (col[t] < 0 || col[t] >= n);
+    SGT_CHECK(bad == 0, SGTSNE_ERR_ARG, "%d column indices out of range [0, %d)", bad, n);
+  }
+  if (c->have_Y && c->n != n) {
+    set_error("P has %d rows but Y has %d points", n, c->n);
+    return SGTSNE_ERR_ARG;
+  }
+  free_P(c);
+  c->n = n;
+  int32_t *rp = nullptr, *cc = nullptr;
+  float *vv = nullptr;
+  SGT_TRY(alloc_dev((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
+  SGT_TRY(alloc_dev((void **)&cc, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+  SGT_TRY(alloc_dev((void **)&vv, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)));
+  SGT_HIP(hipMemcpyAsync(rp, rowptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream));
+  if (nnz > 0) {
+    SGT_HIP(hipMemcpyAsync(cc, col, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, c->stream));
+    SGT_HIP(hipMemcpyAsync(vv, val, sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice, c->stream));
+  }
+  if (symmetrize) {
+    if (c->p.lambda != 1.0) SGT_TRY(launch_lambda_rescale(c->stream, n, rp, vv, c->p.lambda, c->p.drop_leaf));
+    int64_t m = 0;
+    int st = gpu_symmetrize(c->stream, n, nnz, rp, cc, vv, &c->rowptr, &c->col, &c->val, &m);
+    (void)hipStreamSynchronize(c->stream);
+    free_dev(rp);
+    free_dev(cc);
+    free_dev(vv);
+    SGT_TRY(st);
+    c->nnz = m;
+  } else {
+    c->rowptr = rp;
+    c->col = cc;
+    c->val = vv;
+    c->nnz = nnz;
+  }
+  SGT_TRY(build_formats(c));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  c->have_P = true;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_get_P(sgtsne_ctx *c, int64_t *nnz, int32_t *rowptr, int32_t *col, float *val) {
+  SGT_CHECK(c != nullptr && nnz != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  SGT_CHECK(c->have_P, SGTSNE_ERR_STATE, "no P uploaded");
+  *nnz = c->nnz;
+  if (rowptr) SGT_HIP(hipMemcpy(rowptr, c->rowptr, sizeof(int32_t) * ((size_t)c->n + 1), hipMemcpyDeviceToHost));
+  if (col) SGT_HIP(hipMemcpy(col, c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  if (val) SGT_HIP(hipMemcpy(val, c->val, sizeof(float) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_set_Y(sgtsne_ctx *c, const float *y, int layout, int reset_state) {
+  SGT_CHECK(c != nullptr && y != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  SGT_CHECK(layout == SGTSNE_LAYOUT_SOA || layout == SGTSNE_LAYOUT_AOS, SGTSNE_ERR_ARG, "bad layout");
+  int n = c->n;
+  SGT_CHECK(n > 0, SGTSNE_ERR_STATE, "upload P before Y (n unknown)");
+  const size_t nb = (size_t)n * c->ds;
+  if (!c->Y) {
+    SGT_TRY(alloc_dev((void **)&c->Y, nb * sizeof(float)));
+    SGT_TRY(alloc_dev((void **)&c->uY, nb * sizeof(float)));
+    SGT_TRY(alloc_dev((void **)&c->gains, nb * sizeof(float)));
+    SGT_TRY(alloc_dev((void **)&c->Fattr, nb * sizeof(float)));
+    SGT_TRY(alloc_dev((void **)&c->Frep, nb * sizeof(float)));
+    SGT_HIP(hipMemsetAsync(c->Fattr, 0, nb * sizeof(float), c->stream));
+    SGT_HIP(hipMemsetAsync(c->Frep, 0, nb * sizeof(float), c->stream));
+    reset_state = 1;
+  }
+  std::vector<float> h;
+  pack_y(y, layout, n, c->d, c->ds, h);
+  double mean[4] = {0, 0, 0, 0};
+  for (int a = 0; a < c->d; a++) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += h[(size_t)i * c->ds + a];
+    mean[a] = s / n;
+  }
+  SGT_HIP(hipMemcpyAsync(c->Y, h.data(), nb * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  DevScalars s{};
+  SGT_HIP(hipMemcpy(&s, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+  for (int a = 0; a < 4; a++) s.mean[a] = mean[a];
+  SGT_HIP(hipMemcpy(c->sc, &s, sizeof(DevScalars), hipMemcpyHostToDevice));
+  if (reset_state) {
+    SGT_TRY(launch_fill(c->stream, c->uY, (int64_t)nb, 0.0f));
+    SGT_TRY(launch_fill(c->stream, c->gains, (int64_t)nb, 1.0f));
+    c->iter = 0;
+  }
+  SGT_TRY(center_and_fetch(c, 1));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  c->have_Y = true;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_get_Y(sgtsne_ctx *c, float *y, int layout) {
+  SGT_CHECK(c != nullptr && y != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  SGT_CHECK(c->have_Y, SGTSNE_ERR_STATE, "Y not set");
+  std::vector<float> h((size_t)c->n * c->ds);
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  SGT_HIP(hipMemcpy(h.data(), c->Y, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  unpack_y(h, layout, c->n, c->d, c->ds, y);
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_step(sgtsne_ctx *c, int iters) {
+  SGT_CHECK(c != nullptr && iters >= 0, SGTSNE_ERR_ARG, "bad arguments");
+  SGT_CHECK(c->have_P && c->have_Y, SGTSNE_ERR_STATE, "upload P and set Y before stepping");
+  for (auto &v : c->acc_ms) v = 0.0;
+  for (auto &pe : c->pe) pe.valid = false;
+  int st = SGTSNE_OK;
+  for (int k = 0; k < iters && st == SGTSNE_OK; k++) st = iterate(c);
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  if (c->timing && c->iter > 0) harvest_timing(c, c->pe[(c->iter - 1) & 1], c->iter - 1);
+  return st;
+}
+
+int sgtsne_ctx_terms(sgtsne_ctx *c, float *Fattr, float *Frep, double *Z, int *ng, int layout) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  SGT_CHECK(c->have_P && c->have_Y, SGTSNE_ERR_STATE, "upload P and set Y first");
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  SGT_TRY(center_and_fetch(c, 0));
+  SGT_HIP(hipEventSynchronize(c->ev_scal));
+  SGT_TRY(attractive(c));
+  GridParams g{};
+  SGT_TRY(make_grid(c, *c->sc_host, g));
+  SGT_TRY(repulsive(c, g, nullptr));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  DevScalars s{};
+  SGT_HIP(hipMemcpy(&s, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+  const size_t nb = (size_t)c->n * c->ds;
+  std::vector<float> h(nb);
+  if (Fattr) {
+    SGT_HIP(hipMemcpy(h.data(), c->Fattr, nb * sizeof(float), hipMemcpyDeviceToHost));
+    unpack_y(h, layout, c->n, c->d, c->ds, Fattr);
+  }
+  if (Frep) {
+    SGT_HIP(hipMemcpy(h.data(), c->Frep, nb * sizeof(float), hipMemcpyDeviceToHost));
+    unpack_y(h, layout, c->n, c->d, c->ds, Frep, 1.0 / s.Z);
+  }
+  if (Z) *Z = s.Z;
+  if (ng) *ng = g.ng;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_kl(sgtsne_ctx *c, double alpha, double *kl) {
+  SGT_CHECK(c != nullptr && kl != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  SGT_CHECK(c->have_P && c->have_Y, SGTSNE_ERR_STATE, "upload P and set Y first");
+  // Z of the current Y
+  SGT_TRY(sgtsne_ctx_terms(c, nullptr, nullptr, nullptr, nullptr, SGTSNE_LAYOUT_AOS));
+  SGT_TRY(launch_kl(c->stream, c->d, c->n, c->rowptr, c->col, c->val, c->Y, alpha, c->sc, c->rw, c->sc));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  DevScalars s{};
+  SGT_HIP(hipMemcpy(&s, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+  *kl = s.kl;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_repulsive_exact(sgtsne_ctx *c, float *Frep, double *Z, int layout) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  SGT_CHECK(c->have_Y, SGTSNE_ERR_STATE, "Y not set");
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  const size_t nb = (size_t)c->n * c->ds;
+  float *tmp = nullptr;
+  SGT_TRY(alloc_dev((void **)&tmp, nb * sizeof(float)));
+  int st = launch_rep_exact(c->stream, c->d, c->n, c->Y, tmp, c->rw, c->sc);
+  std::vector<float> h(nb);
+  DevScalars s{};
+  if (st == SGTSNE_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipMemcpy(h.data(), tmp, nb * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(&s, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost);
+  }
+  free_dev(tmp);
+  SGT_TRY(st);
+  if (Frep) unpack_y(h, layout, c->n, c->d, c->ds, Frep, 1.0 / s.zexact);
+  if (Z) *Z = s.zexact;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_timing(sgtsne_ctx *c, double *out7) {
+  SGT_CHECK(c != nullptr && out7 != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  for (int k = 0; k < 7; k++) out7[k] = c->acc_ms[k];
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_info(sgtsne_ctx *c, int *iter, int *n, int *d, int64_t *nnz) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  if (iter) *iter = c->iter;
+  if (n) *n = c->n;
+  if (d) *d = c->d;
+  if (nnz) *nnz = c->nnz;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_set_timing(sgtsne_ctx *c, int on) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  c->timing = on != 0;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_errors(sgtsne_ctx *c, double *out, int max, int *count) {
+  SGT_CHECK(c != nullptr && count != nullptr, SGTSNE_ERR_ARG, "bad arguments");
+  *count = (int)c->errors.size();
+  if (out)
+    for (int k = 0; k < std::min(max, *count); k++) out[k] = c->errors[k];
+  return SGTSNE_OK;
+}
+
+int sgtsne_run(int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, const float *val,
+               const sgtsne_params *params, const float *y_in, float *y_out, double *time_info) {
+  SGT_TRY(check_params(params));
+  SGT_CHECK(y_out != nullptr, SGTSNE_ERR_ARG, "y_out is NULL");
+  sgtsne_ctx *c = nullptr;
+  SGT_TRY(sgtsne_ctx_create(&c, params));
+  int st = sgtsne_ctx_upload_P(c, n, nnz, rowptr, col, val, 1);
+  std::vector<float> y0;
+  const float *yi = y_in;
+  if (st == SGTSNE_OK && !y_in) {
+    // N(0, 1e-4^2) start (CU/utils.cu:185-192), seeded
+    std::mt19937_64 gen(params->seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    y0.resize((size_t)n * params->d);
+    for (auto &v : y0) v = (float)(1e-4 * nd(gen));
+    yi = y0.data();
+  }
+  if (st == SGTSNE_OK) st = sgtsne_ctx_set_Y(c, yi, params->layout, 1);
+  if (st == SGTSNE_OK) {
+    c->time_info = time_info;
+    c->time_info_base = 0;
+    c->timing = true;
+    auto t0 = std::chrono::steady_clock::now();
+    st = sgtsne_ctx_step(c, params->max_iter);
+    auto t1 = std::chrono::steady_clock::now();
+    if (time_info) time_info[7 * (size_t)params->max_iter] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  if (st == SGTSNE_OK) st = sgtsne_ctx_get_Y(c, y_out, params->layout);
+  sgtsne_ctx_destroy(c);
+  return st;
+}
+
+int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D) {
+  SGT_CHECK(X && I && D, SGTSNE_ERR_ARG, "NULL argument");
+  SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "need n > k >= 1 and dim >= 1");
+  hipStream_t s = nullptr;
+  SGT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const size_t nx = (size_t)n * dim;
+  std::vector<double> xd(nx);
+  for (size_t t = 0; t < nx; t++) xd[t] = (double)X[t];
+  double *Xd = nullptr, *Dd = nullptr;
+  int32_t *Id = nullptr;
+  int st = alloc_dev((void **)&Xd, nx * sizeof(double));
+  if (!st) st = alloc_dev((void **)&Dd, (size_t)n * (k + 1) * sizeof(double));
+  if (!st) st = alloc_dev((void **)&Id, (size_t)n * (k + 1) * sizeof(int32_t));
+  if (!st && hipMemcpyAsync(Xd, xd.data(), nx * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy of X failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (!st) st = launch_knn_exact(s, Xd, n, dim, k, Id, Dd);
+  if (!st && (hipMemcpyAsync(I, Id, (size_t)n * (k + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipMemcpyAsync(D, Dd, (size_t)n * (k + 1) * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)) {
+    set_error("copy of kNN result failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && !st) {
+    set_error("kNN kernel failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  free_dev(Xd);
+  free_dev(Dd);
+  free_dev(Id);
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi) {
+  SGT_CHECK(V && Phi, SGTSNE_ERR_ARG, "NULL argument");
+  SGT_CHECK(d >= 1 && d <= 3 && m >= 1 && m <= 4 && ng >= 4 && ng <= 512 && h > 0, SGTSNE_ERR_ARG,
+            "bad grid_conv sizes (d=%d ng=%d m=%d)", d, ng, m);
+  GridParams g{};
+  g.d = d;
+  g.ng = ng;
+  g.nGrid = ng - 2;
+  int L = 1, lg = 0;
+  while (L < 2 * ng - 1) { L <<= 1; ++lg; }
+  g.L = L;
+  g.logL = lg;
+  g.hgrid = h;
+  const size_t G = d == 1 ? (size_t)ng : d == 2 ? (size_t)ng * ng : (size_t)ng * ng * ng;
+  size_t wb = 0, kb = 0;
+  conv_workspace_bytes(d, ng, L, &wb, &kb);
+  hipStream_t s = nullptr;
+  SGT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  ConvBuffers cb{};
+  float *Vd = nullptr, *Pd = nullptr;
+  int st = alloc_dev((void **)&Vd, G * m * sizeof(float));
+  if (!st) st = alloc_dev((void **)&Pd, G * m * sizeof(float));
+  if (!st) st = alloc_dev((void **)&cb.W, wb);
+  if (!st) st = alloc_dev((void **)&cb.Kt1, kb);
+  if (!st) st = alloc_dev((void **)&cb.Kt2, kb);
+  if (!st) st = alloc_dev((void **)&cb.Kh, kb);
+  if (!st) st = alloc_dev((void **)&cb.tw, 512 * sizeof(float2));
+  if (!st) st = init_twiddles(s, cb.tw);
+  if (!st && hipMemcpyAsync(Vd, V, G * m * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy of V failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (!st) st = launch_conv(s, g, m, Vd, Pd, cb);
+  if (!st && hipMemcpyAsync(Phi, Pd, G * m * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) {
+    set_error("copy of Phi failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && !st) {
+    set_error("convolution failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  free_dev(Vd);
+  free_dev(Pd);
+  free_dev(cb.W);
+  free_dev(cb.Kt1);
+  free_dev(cb.Kt2);
+  free_dev(cb.Kh);
+  free_dev(cb.tw);
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+int sgtsne_perplexity_equalize(const int32_t *I, const double *D, int n, int k, double u,
+                               int32_t *rowptr, int32_t *col, float *val) {
+  SGT_CHECK(I && D && rowptr && col && val, SGTSNE_ERR_ARG, "NULL argument");
+  SGT_CHECK(n > 0 && k >= 1 && u > 0, SGTSNE_ERR_ARG, "bad sizes");
+  hipStream_t s = nullptr;
+  SGT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  double *Dd = nullptr;
+  float *vd = nullptr;
+  int st = alloc_dev((void **)&Dd, (size_t)n * (k + 1) * sizeof(double));
+  if (!st) st = alloc_dev((void **)&vd, (size_t)n * k * sizeof(float));
+  if (!st && hipMemcpyAsync(Dd, D, (size_t)n * (k + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy of D failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (!st) st = launch_perplexity(s, Dd, n, k, u, vd, nullptr);
+  if (!st && hipMemcpyAsync(val, vd, (size_t)n * k * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) {
+    set_error("copy of P failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && !st) {
+    set_error("perplexity kernel failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  free_dev(Dd);
+  free_dev(vd);
+  (void)hipStreamDestroy(s);
+  SGT_TRY(st);
+  // kNN ids -> CSR without the self column (CU/perplexity_search.cu:100-112)
+  for (int i = 0; i < n; i++) {
+    rowptr[i] = i * k;
+    for (int e = 0; e < k; e++) col[(size_t)i * k + e] = I[(size_t)i * (k + 1) + e + 1];
+  }
+  rowptr[n] = n * k;
+  return SGTSNE_OK;
+}
+
+}  // extern "C"

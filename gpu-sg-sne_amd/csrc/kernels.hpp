@@ -1,0 +1,99 @@
+// kernels.hpp -- launchers of the gfx950 kernels (host-callable).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sgt {
+
+// Scalars the iteration keeps on the device (reductions and Z).
+struct DevScalars {
+  double sum[4];    // sum of Y per dimension (after the update)
+  double minv[4];   // min of the centred Y per dimension
+  double maxv[4];   // max of the centred Y per dimension
+  double mean[4];   // mean subtracted by the centring kernel
+  double Z;         // normalisation term (sum z_i - N)
+  double kl;        // KL divergence (tsneCost)
+  double zexact;    // exact-repulsion Z (validation)
+  double pad;
+};
+
+// Partial-reduction workspace: one slot per block plus a ticket counter.
+struct RedWork {
+  double *part;       // [max_blocks * 8]
+  unsigned *ticket;   // single counter, reset by the last block
+  int max_blocks;
+};
+
+// Grid mapping of one iteration (computed on the host from min/max).
+struct GridParams {
+  int d;
+  int nGrid;        // interpolation intervals + 1 (reference nGridDim)
+  int ng;           // grid side = nGrid + 2
+  int L;            // FFT length (power of two, >= 2 ng - 1)
+  int logL;
+  float minv[3];    // min of Y per dimension (grid origin)
+  float scale;      // (nGrid - 1) / maxy
+  float gmax;       // largest float below nGrid - 1 (clamp)
+  float center[3];  // charge shift (bounding-box centre)
+  double hgrid;     // grid spacing in embedding units
+};
+
+// ---- attractive term (kernels_spmv.hip) ----
+int launch_attr_csr_seg(hipStream_t s, int d, int group, int n, const int32_t *rowptr,
+                        const int32_t *col, const float *val, const float *Y, float *F);
+int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
+                        const float *val, const float *Y, float *F);
+int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,
+                    const float *val, const float *Y, float *F);
+int launch_attr_ell(hipStream_t s, int d, int n, int width, const int32_t *ecol, const float *eval,
+                    const float *Y, float *F);
+int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
+              const float *val, const float *Y, double alpha, const DevScalars *sc, RedWork rw,
+              DevScalars *out);
+
+// ---- repulsion (kernels_grid.hip) ----
+int launch_fill(hipStream_t s, float *p, int64_t count, float v);
+int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, RedWork rw,
+                         DevScalars *sc);
+int launch_s2g_atomic(hipStream_t s, const GridParams &g, int n, const float *Y, float *V);
+int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
+                     float *Frep, RedWork rw, DevScalars *sc);
+int launch_update(hipStream_t s, int d, int n, float *Y, float *uY, float *gains,
+                  const float *Fattr, const float *Frep, float alpha, float momentum, float eta,
+                  RedWork rw, DevScalars *sc);
+int launch_rep_exact(hipStream_t s, int d, int n, const float *Y, float *Frep, RedWork rw,
+                     DevScalars *sc);
+
+// ---- FFT convolution (kernels_fft.hip) ----
+struct ConvBuffers {
+  float2 *W;      // complex work [pairs][...]
+  float *Kt1;     // real half-spectrum scratch
+  float *Kt2;
+  float *Kh;      // real half-spectrum of the kernel [(L/2+1)^d]
+  float2 *tw;     // twiddles exp(-2 pi i t / 1024), t < 512
+};
+int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes);
+int launch_conv(hipStream_t s, const GridParams &g, int m, const float *V, float *Phi,
+                const ConvBuffers &b);
+int init_twiddles(hipStream_t s, float2 *tw);
+
+// ---- P construction (kernels_prep.hip) ----
+int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D);
+int launch_perplexity(hipStream_t s, const double *D, int n, int k, double u, float *val,
+                      double *beta_out);
+int launch_lambda_rescale(hipStream_t s, int n, const int32_t *rowptr, float *val, double lambda,
+                          int drop_leaf);
+int launch_rowids(hipStream_t s, int n, const int32_t *rowptr, int32_t *row);
+int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d,
+                   const int32_t *col_d, const float *val_d, int32_t **rowptr_out,
+                   int32_t **col_out, float **val_out, int64_t *nnz_out);
+int launch_build_ell(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
+                     const float *val, int32_t *ecol, float *eval, int32_t *tail_cnt);
+int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
+                      const float *val, const int32_t *tail_off, int32_t *trow, int32_t *tcol,
+                      float *tval);
+int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n);
+
+}  // namespace sgt

@@ -1,0 +1,470 @@
+// kernels_prep.hip -- construction of P on the GPU: exact kNN, perplexity
+// bisection, lambda rescaling, P + P^T, and the COO / hybrid format builds.
+//
+// Reference (CU = code/cuda/SG-tSNE-CUDA/src, HY = code/cuda/SG-tSNE-HYB/sgtsnepi/src):
+//   kNN            CU/allknn.cuh:75-92 (FLANN, approximate; exact here)
+//   perplexity     HY/sgtsne.cpp:201-254 (FP64 twin of CU/perplexity_search.cu:18-208)
+//   lambda         HY/graph_rescaling.cpp:19-104 (CU/graph_rescaling.cu:11-107)
+//   P + P^T        CU/graph_rescaling.cu:266-398, HY/sparsematrix.cpp:69-164,
+//                  normalised to sum 1 as HY/sgtsne.cpp:78-85
+//   COO rows       CU/prepareMatrix.cu:2-27
+//   hybrid         CU/prepareMatrix.cu:56-97, CU/matrix_converter.cpp:106-248
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/sgtsne.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <climits>
+#include <vector>
+
+namespace sgt {
+
+namespace {
+
+// ---- exact kNN -----------------------------------------------------------
+// Work-group tile: 64 queries x 32 references; 256 threads each own a 4x2
+// micro-tile of (query, reference) distances accumulated over the dimensions
+// in order with fma(diff, diff, acc).  Candidates below a query's current
+// worst key (distance, index) are queued in LDS and merged by one thread per
+// query into a max-heap of k entries kept in global memory.
+constexpr int kKnnQ = 64;
+constexpr int kKnnR = 32;
+constexpr int kKnnDC = 16;  // dimensions staged per round
+constexpr int kKnnThreads = 256;
+
+__device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+__global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restrict__ X, int n,
+                                                          int dim, int k, double *__restrict__ hd,
+                                                          int *__restrict__ hi) {
+  __shared__ double qs[kKnnDC][kKnnQ + 1];
+  __shared__ double rs[kKnnDC][kKnnR + 1];
+  __shared__ double qd[kKnnQ][kKnnR];
+  __shared__ int qi[kKnnQ][kKnnR];
+  __shared__ int qcnt[kKnnQ];
+  __shared__ double thr_d[kKnnQ];
+  __shared__ int thr_i[kKnnQ];
+
+  const int q0 = blockIdx.x * kKnnQ;
+  const int tid = threadIdx.x;
+  const int tq = tid >> 4;   // 0..15 -> queries tq*4 .. tq*4+3
+  const int tr = tid & 15;   // 0..15 -> references tr + 16*c
+  if (tid < kKnnQ) {
+    qcnt[tid] = 0;
+    thr_d[tid] = INFINITY;
+    thr_i[tid] = INT_MAX;
+    const int q = q0 + tid;
+    if (q < n)
+      for (int e = 0; e < k; ++e) {
+        hd[(size_t)q * k + e] = INFINITY;
+        hi[(size_t)q * k + e] = INT_MAX;
+      }
+  }
+  __syncthreads();
+
+  for (int r0 = 0; r0 < n; r0 += kKnnR) {
+    double acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = 0.0;
+    for (int l0 = 0; l0 < dim; l0 += kKnnDC) {
+      for (int idx = tid; idx < kKnnDC * kKnnQ; idx += kKnnThreads) {
+        const int qq = idx / kKnnDC, ll = idx % kKnnDC;
+        const int q = q0 + qq, l = l0 + ll;
+        qs[ll][qq] = (q < n && l < dim) ? X[(size_t)q * dim + l] : 0.0;
+        if (qq < kKnnR) {
+          const int r = r0 + qq;
+          rs[ll][qq] = (r < n && l < dim) ? X[(size_t)r * dim + l] : 0.0;
+        }
+      }
+      __syncthreads();
+      const int lc = min(kKnnDC, dim - l0);
+      for (int ll = 0; ll < lc; ++ll) {
+        double qv[4], rv[2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) qv[a] = qs[ll][tq * 4 + a];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) rv[b] = rs[ll][tr + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const double df = qv[a] - rv[b];
+            acc[a][b] = fma(df, df, acc[a][b]);
+          }
+      }
+      __syncthreads();
+    }
+    // queue candidates
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int ql = tq * 4 + a;
+      const int q = q0 + ql;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int r = r0 + tr + 16 * b;
+        if (q < n && r < n && r != q && key_less(acc[a][b], r, thr_d[ql], thr_i[ql])) {
+          const int pos = atomicAdd(&qcnt[ql], 1);
+          qd[ql][pos] = acc[a][b];
+          qi[ql][pos] = r;
+        }
+      }
+    }
+    __syncthreads();
+    // merge queued candidates into the heaps (one thread per query)
+    if (tid < kKnnQ) {
+      const int q = q0 + tid;
+      const int cnt = qcnt[tid];
+      if (q < n && cnt > 0) {
+        double *H = hd + (size_t)q * k;
+        int *HI = hi + (size_t)q * k;
+        double rd = H[0];
+        int ri = HI[0];
+        for (int c = 0; c < cnt; ++c) {
+          const double cd = qd[tid][c];
+          const int ci = qi[tid][c];
+          if (!key_less(cd, ci, rd, ri)) continue;
+          int pos = 0;
+          for (;;) {
+            const int l = 2 * pos + 1, rr = l + 1;
+            int m = -1;
+            double md = cd;
+            int mi = ci;
+            if (l < k && key_less(md, mi, H[l], HI[l])) { m = l; md = H[l]; mi = HI[l]; }
+            if (rr < k && key_less(md, mi, H[rr], HI[rr])) { m = rr; md = H[rr]; mi = HI[rr]; }
+            if (m < 0) break;
+            H[pos] = md;
+            HI[pos] = mi;
+            pos = m;
+          }
+          H[pos] = cd;
+          HI[pos] = ci;
+          rd = H[0];
+          ri = HI[0];
+        }
+        thr_d[tid] = rd;
+        thr_i[tid] = ri;
+      }
+      qcnt[tid] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Heap -> ascending rows with the point itself first.
+__global__ void knn_finalize_kernel(int n, int k, double *__restrict__ hd, int *__restrict__ hi,
+                                    int32_t *__restrict__ I, double *__restrict__ D) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double *H = hd + (size_t)q * k;
+  int *HI = hi + (size_t)q * k;
+  int32_t *Iq = I + (size_t)q * (k + 1);
+  double *Dq = D + (size_t)q * (k + 1);
+  Iq[0] = q;
+  Dq[0] = 0.0;
+  for (int e = k - 1; e >= 0; --e) {
+    Iq[1 + e] = HI[0];
+    Dq[1 + e] = H[0];
+    const double cd = H[e];
+    const int ci = HI[e];
+    int pos = 0;
+    for (;;) {
+      const int l = 2 * pos + 1, rr = l + 1;
+      int m = -1;
+      double md = cd;
+      int mi = ci;
+      if (l < e && key_less(md, mi, H[l], HI[l])) { m = l; md = H[l]; mi = HI[l]; }
+      if (rr < e && key_less(md, mi, H[rr], HI[rr])) { m = rr; md = H[rr]; mi = HI[rr]; }
+      if (m < 0) break;
+      H[pos] = md;
+      HI[pos] = mi;
+      pos = m;
+    }
+    H[pos] = cd;
+    HI[pos] = ci;
+  }
+}
+
+// ---- perplexity bisection (one thread per row, FP64) ---------------------
+// Exact reference semantics: the values normalised at the end are those of
+// the last evaluated beta; this version recomputes them from that beta.
+__global__ void perplexity_kernel2(const double *__restrict__ D, int n, int k, double u,
+                                   float *__restrict__ val, double *__restrict__ beta_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *dist = D + (size_t)i * (k + 1) + 1;
+  const double tol = 1e-5;
+  const double logu = log(u);
+  double beta = 1.0, min_beta = -DBL_MAX, max_beta = DBL_MAX;
+  double sum_P = 0.0, beta_eval = 1.0;
+  bool found = false;
+  int iter = 0;
+  while (!found && iter < 200) {
+    beta_eval = beta;
+    sum_P = DBL_MIN;
+    for (int m = 0; m < k; ++m) sum_P += exp(-beta * dist[m]);
+    double H = 0.0;
+    for (int m = 0; m < k; ++m) H += beta * (dist[m] * exp(-beta * dist[m]));
+    H = (H / sum_P) + log(sum_P);
+    const double Hdiff = H - logu;
+    if (Hdiff < tol && -Hdiff < tol) {
+      found = true;
+    } else if (Hdiff > 0) {
+      min_beta = beta;
+      beta = (max_beta == DBL_MAX || max_beta == -DBL_MAX) ? beta * 2.0 : (beta + max_beta) / 2.0;
+    } else {
+      max_beta = beta;
+      beta = (min_beta == -DBL_MAX || min_beta == DBL_MAX) ? beta / 2.0 : (beta + min_beta) / 2.0;
+    }
+    ++iter;
+  }
+  for (int m = 0; m < k; ++m)
+    val[(size_t)i * k + m] = (float)(exp(-beta_eval * dist[m]) / sum_P);
+  if (beta_out) beta_out[i] = beta;
+}
+
+// ---- lambda rescaling (one thread per row, FP64) --------------------------
+__global__ void lambda_kernel(int n, const int32_t *__restrict__ rowptr, float *__restrict__ val,
+                              double lambda, int drop_leaf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tolBinary = 1e-5;
+  const int maxIter = 100;
+  double fval = 1.0 - lambda;
+  double sig2 = 1.0, a = -1e3, c = INFINITY;
+  int iter = 0;
+  const int s = rowptr[i], e = rowptr[i + 1];
+  while (fabs(fval) > tolBinary && iter < maxIter) {
+    ++iter;
+    if (fval > 0) {
+      a = sig2;
+      sig2 = isinf(c) ? 2 * a : 0.5 * (a + c);
+    } else {
+      c = sig2;
+      sig2 = 0.5 * (a + c);
+    }
+    double sum_i = 0.0;
+    for (int j = s; j < e; ++j) sum_i += exp(log((double)val[j]) * sig2);
+    fval = sum_i - lambda;
+  }
+  double sum_i = 0.0;
+  for (int j = s; j < e; ++j) sum_i += exp(log((double)val[j]) * sig2);
+  for (int j = s; j < e; ++j) val[j] = (float)(exp(log((double)val[j]) * sig2) / sum_i);
+  if (drop_leaf && e - s == 1) val[s] = 0.0f;
+}
+
+__global__ void rowids_kernel(int n, const int32_t *__restrict__ rowptr, int32_t *__restrict__ row) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int t = rowptr[i]; t < rowptr[i + 1]; ++t) row[t] = i;
+}
+
+// ---- P + P^T ----------------------------------------------------------------
+__global__ void sym_keys_kernel(int n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                const float *__restrict__ val, int64_t nnz,
+                                unsigned long long *__restrict__ keys, double *__restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int t = rowptr[i]; t < rowptr[i + 1]; ++t) {
+    const unsigned long long r = (unsigned)i, c = (unsigned)col[t];
+    keys[t] = (r << 32) | c;
+    keys[nnz + t] = (c << 32) | r;
+    vals[t] = (double)val[t];
+    vals[nnz + t] = (double)val[t];
+  }
+}
+
+__global__ void sym_finish_kernel(int64_t m, const unsigned long long *__restrict__ ukeys,
+                                  const double *__restrict__ uvals, const double *__restrict__ total,
+                                  int32_t *__restrict__ col, float *__restrict__ val) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  col[t] = (int32_t)(ukeys[t] & 0xffffffffull);
+  val[t] = (float)(uvals[t] / *total);
+}
+
+__global__ void sym_rowptr_kernel(int n, int64_t m, const unsigned long long *__restrict__ ukeys,
+                                  int32_t *__restrict__ rowptr) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  // first position whose row >= r
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)(ukeys[mid] >> 32) < r) lo = mid + 1;
+    else hi = mid;
+  }
+  rowptr[r] = (int32_t)lo;
+}
+
+// ---- hybrid ELL + COO tail ----------------------------------------------------
+__global__ void build_ell_kernel(int n, int width, const int32_t *__restrict__ rowptr,
+                                 const int32_t *__restrict__ col, const float *__restrict__ val,
+                                 int32_t *__restrict__ ecol, float *__restrict__ eval,
+                                 int32_t *__restrict__ tail_cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = rowptr[r], len = rowptr[r + 1] - s;
+  for (int e = 0; e < width; ++e) {
+    const int64_t o = (int64_t)e * n + r;
+    if (e < len) { ecol[o] = col[s + e]; eval[o] = val[s + e]; }
+    else { ecol[o] = r; eval[o] = 0.0f; }
+  }
+  tail_cnt[r] = len > width ? len - width : 0;
+}
+
+__global__ void build_tail_kernel(int n, int width, const int32_t *__restrict__ rowptr,
+                                  const int32_t *__restrict__ col, const float *__restrict__ val,
+                                  const int32_t *__restrict__ tail_off, int32_t *__restrict__ trow,
+                                  int32_t *__restrict__ tcol, float *__restrict__ tval) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = rowptr[r], len = rowptr[r + 1] - s;
+  int o = tail_off[r];
+  for (int e = width; e < len; ++e, ++o) {
+    trow[o] = r;
+    tcol[o] = col[s + e];
+    tval[o] = val[s + e];
+  }
+}
+
+__global__ void to_double_kernel(const float *__restrict__ in, double *__restrict__ out, int64_t m) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = (double)in[t];
+}
+
+}  // namespace
+
+int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D) {
+  SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "kNN needs n > k >= 1 (n=%d k=%d)", n, k);
+  SGT_CHECK(k <= kKnnR * 64, SGTSNE_ERR_ARG, "k too large");
+  double *hd = nullptr;
+  int *hi = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&hd, sizeof(double) * (size_t)n * k, s));
+  SGT_HIP(hipMallocAsync((void **)&hi, sizeof(int) * (size_t)n * k, s));
+  knn_kernel<<<div_up(n, kKnnQ), kKnnThreads, 0, s>>>(X, n, dim, k, hd, hi);
+  SGT_LAUNCH_CHECK();
+  knn_finalize_kernel<<<div_up(n, 256), 256, 0, s>>>(n, k, hd, hi, I, D);
+  SGT_LAUNCH_CHECK();
+  SGT_HIP(hipFreeAsync(hd, s));
+  SGT_HIP(hipFreeAsync(hi, s));
+  return SGTSNE_OK;
+}
+
+int launch_perplexity(hipStream_t s, const double *D, int n, int k, double u, float *val,
+                      double *beta_out) {
+  perplexity_kernel2<<<div_up(n, 128), 128, 0, s>>>(D, n, k, u, val, beta_out);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_lambda_rescale(hipStream_t s, int n, const int32_t *rowptr, float *val, double lambda,
+                          int drop_leaf) {
+  lambda_kernel<<<div_up(n, 128), 128, 0, s>>>(n, rowptr, val, lambda, drop_leaf);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_rowids(hipStream_t s, int n, const int32_t *rowptr, int32_t *row) {
+  rowids_kernel<<<div_up(n, 256), 256, 0, s>>>(n, rowptr, row);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n) {
+  size_t tb = 0;
+  SGT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, s));
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  return SGTSNE_OK;
+}
+
+int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
+                   const float *val_d, int32_t **rowptr_out, int32_t **col_out, float **val_out,
+                   int64_t *nnz_out) {
+  const int64_t m2 = 2 * nnz;
+  SGT_CHECK(m2 < (int64_t)INT_MAX, SGTSNE_ERR_ARG, "2*nnz exceeds the int32 range");
+  unsigned long long *k0 = nullptr, *k1 = nullptr, *uk = nullptr;
+  double *v0 = nullptr, *v1 = nullptr, *uv = nullptr, *total = nullptr;
+  int *nruns = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&k0, sizeof(unsigned long long) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&k1, sizeof(unsigned long long) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&v0, sizeof(double) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&v1, sizeof(double) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&nruns, sizeof(int), s));
+  SGT_HIP(hipMallocAsync((void **)&total, sizeof(double), s));
+  sym_keys_kernel<<<div_up(n, 256), 256, 0, s>>>(n, rowptr_d, col_d, val_d, nnz, k0, v0);
+  SGT_LAUNCH_CHECK();
+  int nbits = 1;
+  while ((1ll << nbits) < (long long)n) ++nbits;
+  const int end_bit = 32 + nbits;
+  size_t tb = 0;
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)m2, 0, end_bit, s));
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)m2, 0, end_bit, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  // unique keys into k0 / v0 (reuse), summing duplicates (P_ij + P_ji)
+  uk = k0;
+  uv = v0;
+  tb = 0;
+  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, tb, k1, uk, v1, uv, nruns, hipcub::Sum(), (int)m2, s));
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(tmp, tb, k1, uk, v1, uv, nruns, hipcub::Sum(), (int)m2, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  int runs = 0;
+  SGT_HIP(hipMemcpyAsync(&runs, nruns, sizeof(int), hipMemcpyDeviceToHost, s));
+  SGT_HIP(hipStreamSynchronize(s));
+  tb = 0;
+  SGT_HIP(hipcub::DeviceReduce::Sum(nullptr, tb, uv, total, runs, s));
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceReduce::Sum(tmp, tb, uv, total, runs, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  int32_t *rp = nullptr, *cc = nullptr;
+  float *vv = nullptr;
+  SGT_HIP(hipMalloc((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
+  SGT_HIP(hipMalloc((void **)&cc, sizeof(int32_t) * (size_t)(runs > 0 ? runs : 1)));
+  SGT_HIP(hipMalloc((void **)&vv, sizeof(float) * (size_t)(runs > 0 ? runs : 1)));
+  if (runs > 0) {
+    sym_finish_kernel<<<div_up(runs, 256), 256, 0, s>>>(runs, uk, uv, total, cc, vv);
+    SGT_LAUNCH_CHECK();
+  }
+  sym_rowptr_kernel<<<div_up((int64_t)n + 1, 256), 256, 0, s>>>(n, runs, uk, rp);
+  SGT_LAUNCH_CHECK();
+  SGT_HIP(hipFreeAsync(k0, s));
+  SGT_HIP(hipFreeAsync(k1, s));
+  SGT_HIP(hipFreeAsync(v0, s));
+  SGT_HIP(hipFreeAsync(v1, s));
+  SGT_HIP(hipFreeAsync(nruns, s));
+  SGT_HIP(hipFreeAsync(total, s));
+  SGT_HIP(hipStreamSynchronize(s));
+  *rowptr_out = rp;
+  *col_out = cc;
+  *val_out = vv;
+  *nnz_out = runs;
+  return SGTSNE_OK;
+}
+
+int launch_build_ell(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
+                     const float *val, int32_t *ecol, float *eval, int32_t *tail_cnt) {
+  build_ell_kernel<<<div_up(n, 256), 256, 0, s>>>(n, width, rowptr, col, val, ecol, eval, tail_cnt);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
+                      const float *val, const int32_t *tail_off, int32_t *trow, int32_t *tcol,
+                      float *tval) {
+  build_tail_kernel<<<div_up(n, 256), 256, 0, s>>>(n, width, rowptr, col, val, tail_off, trow, tcol, tval);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+}  // namespace sgt

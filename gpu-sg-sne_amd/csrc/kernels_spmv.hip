@@ -1,0 +1,279 @@
+// kernels_spmv.hip -- attractive term Fattr_i = sum_j P_ij (y_i - y_j) / (1 + |y_i - y_j|^2)
+// over the symmetric P in four storage formats, and the KL cost.
+//
+// Reference: AttractiveEstimation CU/pq.cu:25-57 with pq_csr_naive
+// (CU/pq.cuh:16-75, format 0), coo_pq_kernel (CU/pq.cuh:77-118, format 2),
+// ell_spmv_kernel + coo_spmv_kernel (CU/hybridpq.cu:4-113, format 3); CPU
+// HY/pq.cpp:15-57.  KL: ComputeKLkernel CU/pq.cu:61-102.
+//
+// Y is AoS with one aligned vector per point (float / float2 / float4), so
+// every neighbour gather is a single load.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "../../include/sgtsne.h"
+
+#include <cfloat>
+
+namespace sgt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int D>
+__device__ __forceinline__ void accum_pair(const float (&yi)[D], const float (&yj)[D], float p,
+                                           float (&acc)[D]) {
+  float df[D], dist = 0.0f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    df[a] = yi[a] - yj[a];
+    dist += df[a] * df[a];
+  }
+  const float pq = p / (1.0f + dist);
+#pragma unroll
+  for (int a = 0; a < D; ++a) acc[a] += pq * df[a];
+}
+
+// Format 4 (default): a group of G lanes per row walks the row's contiguous
+// (col, val) stream; the group sum is a cross-lane butterfly.
+template <int D, int G>
+__global__ __launch_bounds__(kBlock) void attr_csr_seg_kernel(int n, const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ col,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ Y,
+                                                              float *__restrict__ F) {
+  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int ngroups = gridDim.x * blockDim.x / G;
+  for (int row = gid; row < n; row += ngroups) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    const int e = rowptr[row + 1];
+    for (int t = rowptr[row] + lane; t < e; t += G) {
+      float yj[D];
+      load_pt<D>(Y, col[t], yj);
+      accum_pair<D>(yi, yj, val[t], acc);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
+// Format 0: one thread per row (CU/pq.cuh:16-75).
+template <int D>
+__global__ __launch_bounds__(kBlock) void attr_csr_row_kernel(int n, const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ col,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ Y,
+                                                              float *__restrict__ F) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) {
+      float yj[D];
+      load_pt<D>(Y, col[t], yj);
+      accum_pair<D>(yi, yj, val[t], acc);
+    }
+    store_pt<D>(F, row, acc);
+  }
+}
+
+// Format 2: one thread per nonzero with atomic accumulation (CU/pq.cuh:77-118).
+// Also the COO tail of format 3.  F must be zeroed (or hold the ELL part).
+template <int D>
+__global__ __launch_bounds__(kBlock) void attr_coo_kernel(int64_t nnz, const int32_t *__restrict__ row,
+                                                          const int32_t *__restrict__ col,
+                                                          const float *__restrict__ val,
+                                                          const float *__restrict__ Y,
+                                                          float *__restrict__ F) {
+  const int ds = point_stride(D);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nnz;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int r = row[t];
+    float yi[D], yj[D], acc[D];
+    load_pt<D>(Y, r, yi);
+    load_pt<D>(Y, col[t], yj);
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    accum_pair<D>(yi, yj, val[t], acc);
+#pragma unroll
+    for (int a = 0; a < D; ++a) unsafeAtomicAdd(&F[(int64_t)r * ds + a], acc[a]);
+  }
+}
+
+// Format 3, ELL part: one thread per row over a column-major [width][n]
+// slab (CU/hybridpq.cu:4-69).  Padding entries point at the row itself.
+template <int D>
+__global__ __launch_bounds__(kBlock) void attr_ell_kernel(int n, int width, const int32_t *__restrict__ ecol,
+                                                          const float *__restrict__ eval,
+                                                          const float *__restrict__ Y,
+                                                          float *__restrict__ F) {
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    for (int e = 0; e < width; ++e) {
+      const int64_t o = (int64_t)e * n + row;
+      float yj[D];
+      load_pt<D>(Y, ecol[o], yj);
+      accum_pair<D>(yi, yj, eval[o], acc);
+    }
+    store_pt<D>(F, row, acc);
+  }
+}
+
+// KL = sum_nnz p log((p + FLT_MIN) / (q + FLT_MIN)), p = alpha P_ij,
+// q = 1 / (Z (1 + d_ij^2)); FP64 accumulation, deterministic block partials.
+__device__ __forceinline__ void store_partial(double *slot, double v) {
+  __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_partial(const double *slot) {
+  return __hip_atomic_load(const_cast<double *>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kl_kernel(int n, const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const float *__restrict__ val,
+                                                    const float *__restrict__ Y, double alpha,
+                                                    const DevScalars *sc, RedWork rw,
+                                                    DevScalars *out) {
+  __shared__ double lds[4];
+  __shared__ int am_last;
+  const double Z = sc->Z;
+  double c = 0.0;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    for (int t = rowptr[row]; t < rowptr[row + 1]; ++t) {
+      float yj[D];
+      load_pt<D>(Y, col[t], yj);
+      double dist = 0.0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double df = (double)yi[a] - (double)yj[a];
+        dist += df * df;
+      }
+      const double p = (double)val[t] * alpha;
+      const double q = 1.0 / (Z * (1.0 + dist));
+      c += p * log((p + (double)FLT_MIN) / (q + (double)FLT_MIN));
+    }
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) store_partial(&rw.part[blockIdx.x * 8], lds[0] + lds[1] + lds[2] + lds[3]);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = __hip_atomic_fetch_add(rw.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (am_last && threadIdx.x == 0) {
+    __threadfence();
+    double s = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b) s += load_partial(&rw.part[b * 8]);
+    out->kl = s;
+    __hip_atomic_store(rw.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <template <int> class K>
+struct Dispatch;
+
+}  // namespace
+
+int launch_attr_csr_seg(hipStream_t s, int d, int group, int n, const int32_t *rowptr,
+                        const int32_t *col, const float *val, const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  int blocks = div_up((int64_t)n * group, kBlock);
+#define SEG_CASE(DD, GG)                                                                      \
+  case GG:                                                                                    \
+    attr_csr_seg_kernel<DD, GG><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, F);         \
+    break;
+#define SEG_D(DD)                                                                             \
+  switch (group) {                                                                            \
+    SEG_CASE(DD, 4) SEG_CASE(DD, 8) SEG_CASE(DD, 16) SEG_CASE(DD, 32) SEG_CASE(DD, 64)        \
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "row group must be 4..64");                     \
+  }
+  switch (d) {
+    case 1: SEG_D(1) break;
+    case 2: SEG_D(2) break;
+    case 3: SEG_D(3) break;
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+  }
+#undef SEG_D
+#undef SEG_CASE
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
+                        const float *val, const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  int blocks = div_up(n, kBlock);
+  switch (d) {
+    case 1: attr_csr_row_kernel<1><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, F); break;
+    case 2: attr_csr_row_kernel<2><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, F); break;
+    case 3: attr_csr_row_kernel<3><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, F); break;
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+  }
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,
+                    const float *val, const float *Y, float *F) {
+  if (nnz <= 0) return SGTSNE_OK;
+  int blocks = (int)std::min<int64_t>((nnz + kBlock - 1) / kBlock, 1 << 20);
+  switch (d) {
+    case 1: attr_coo_kernel<1><<<blocks, kBlock, 0, s>>>(nnz, row, col, val, Y, F); break;
+    case 2: attr_coo_kernel<2><<<blocks, kBlock, 0, s>>>(nnz, row, col, val, Y, F); break;
+    case 3: attr_coo_kernel<3><<<blocks, kBlock, 0, s>>>(nnz, row, col, val, Y, F); break;
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+  }
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_ell(hipStream_t s, int d, int n, int width, const int32_t *ecol, const float *eval,
+                    const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  int blocks = div_up(n, kBlock);
+  switch (d) {
+    case 1: attr_ell_kernel<1><<<blocks, kBlock, 0, s>>>(n, width, ecol, eval, Y, F); break;
+    case 2: attr_ell_kernel<2><<<blocks, kBlock, 0, s>>>(n, width, ecol, eval, Y, F); break;
+    case 3: attr_ell_kernel<3><<<blocks, kBlock, 0, s>>>(n, width, ecol, eval, Y, F); break;
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+  }
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
+              const float *val, const float *Y, double alpha, const DevScalars *sc, RedWork rw,
+              DevScalars *out) {
+  int blocks = std::min(div_up(n > 0 ? n : 1, kBlock), std::min(rw.max_blocks, 2048));
+  switch (d) {
+    case 1: kl_kernel<1><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
+    case 2: kl_kernel<2><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
+    case 3: kl_kernel<3><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+  }
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+}  // namespace sgt

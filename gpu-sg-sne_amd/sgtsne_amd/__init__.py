@@ -1,0 +1,276 @@
+"""Python binding of the gfx950 SG-t-SNE engine (C-ABI in include/sgtsne.h).
+
+The names mirror the reference's interface:
+  tsneparams               CU/types.hpp:55-72 (HY/sgtsne.hpp:25-37)
+  sgtsne(P, params, y_in)  sgtsneCUDA CU/sgtsne.cu:26-128 / sgtsne HY/sgtsne.cpp:37-197
+  perplexityEqualization   CU/perplexity_search.cu:114-141 / HY/sgtsne.cpp:257-297
+plus the stepwise Context used for parity tests, batching and serving.
+
+Every call goes to libsgtsne_hip.so; there is no CPU fallback: importing this
+module without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsgtsne_hip.so")
+
+FORMAT_CSR = 0
+FORMAT_COO = 2
+FORMAT_HYBRID = 3
+FORMAT_CSR_SEG = 4
+GRID_H = 0
+GRID_FIXED = 1
+LAYOUT_SOA = 0
+LAYOUT_AOS = 1
+
+EXPORTED = [
+    "sgtsne_params_default", "sgtsne_last_error", "sgtsne_version", "sgtsne_knn_exact",
+    "sgtsne_perplexity_equalize", "sgtsne_grid_conv", "sgtsne_run", "sgtsne_ctx_create", "sgtsne_ctx_upload_P",
+    "sgtsne_ctx_get_P", "sgtsne_ctx_set_Y", "sgtsne_ctx_get_Y", "sgtsne_ctx_step",
+    "sgtsne_ctx_terms", "sgtsne_ctx_kl", "sgtsne_ctx_repulsive_exact", "sgtsne_ctx_timing",
+    "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
+]
+
+
+class SgtsneError(RuntimeError):
+    pass
+
+
+class tsneparams(C.Structure):
+    """Mirror of sgtsne_params (every tsneparams field, CU/types.hpp:55-72)."""
+    _fields_ = [
+        ("d", C.c_int), ("lambda_", C.c_double), ("alpha", C.c_double), ("maxIter", C.c_int),
+        ("earlyIter", C.c_int), ("n", C.c_int), ("h", C.c_double), ("dropLeaf", C.c_int),
+        ("np", C.c_int), ("format", C.c_int), ("bs", C.c_int), ("ComputeError", C.c_int),
+        ("ng", C.c_int), ("grid_mode", C.c_int), ("layout", C.c_int), ("seed", C.c_uint64),
+        ("ell_width", C.c_int),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        _check(lib().sgtsne_params_default(C.byref(self)))
+        for k, v in kw.items():
+            if k == "lambda":
+                k = "lambda_"
+            if not hasattr(self, k):
+                raise AttributeError(f"tsneparams has no field {k}")
+            setattr(self, k, v)
+
+
+_lib = None
+
+
+def lib():
+    """Loads libsgtsne_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SgtsneError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    P = C.POINTER(tsneparams)
+    sig = {
+        "sgtsne_params_default": (C.c_int, [P]),
+        "sgtsne_last_error": (C.c_char_p, []),
+        "sgtsne_version": (C.c_char_p, []),
+        "sgtsne_knn_exact": (C.c_int, [f32p, C.c_int, C.c_int, C.c_int, i32p, f64p]),
+        "sgtsne_perplexity_equalize": (C.c_int, [i32p, f64p, C.c_int, C.c_int, C.c_double, i32p, i32p, f32p]),
+        "sgtsne_grid_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, f32p, f32p]),
+        "sgtsne_run": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, f32p, P, vp, f32p, vp]),
+        "sgtsne_ctx_create": (C.c_int, [C.POINTER(vp), P]),
+        "sgtsne_ctx_upload_P": (C.c_int, [vp, C.c_int, C.c_int64, i32p, i32p, f32p, C.c_int]),
+        "sgtsne_ctx_get_P": (C.c_int, [vp, C.POINTER(C.c_int64), vp, vp, vp]),
+        "sgtsne_ctx_set_Y": (C.c_int, [vp, f32p, C.c_int, C.c_int]),
+        "sgtsne_ctx_get_Y": (C.c_int, [vp, f32p, C.c_int]),
+        "sgtsne_ctx_step": (C.c_int, [vp, C.c_int]),
+        "sgtsne_ctx_terms": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
+        "sgtsne_ctx_kl": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
+        "sgtsne_ctx_repulsive_exact": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
+        "sgtsne_ctx_timing": (C.c_int, [vp, f64p]),
+        "sgtsne_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+        "sgtsne_ctx_errors": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
+        "sgtsne_ctx_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int64)]),
+        "sgtsne_ctx_destroy": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().sgtsne_last_error().decode(errors="replace")
+        raise SgtsneError(f"sgtsne error {status}: {msg}")
+
+
+def version():
+    return lib().sgtsne_version().decode()
+
+
+class CSR:
+    """Row-compressed P: int32 rowptr [n+1], int32 col [nnz], float32 val [nnz]."""
+
+    def __init__(self, n, rowptr, col, val):
+        self.n = int(n)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float32)
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+
+def knn_exact(X, k):
+    """Exact kNN on the GPU: I [n][k+1] int32 (self first), D [n][k+1] float64."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, dim = X.shape
+    I = np.empty((n, k + 1), np.int32)
+    D = np.empty((n, k + 1), np.float64)
+    _check(lib().sgtsne_knn_exact(X, n, dim, int(k), I, D))
+    return I, D
+
+
+def grid_conv(V, h):
+    """GPU non-periodic convolution of V [m][ng]^d by (1 + h^2 r^2)^-2."""
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    m, ng, d = V.shape[0], V.shape[1], V.ndim - 1
+    Phi = np.empty_like(V)
+    _check(lib().sgtsne_grid_conv(d, ng, m, float(h), V.reshape(-1), Phi.reshape(-1)))
+    return Phi
+
+
+def perplexityEqualization(I, D, n, nn, u):
+    """Row-stochastic kNN CSR after perplexity equalisation (reference name)."""
+    I = np.ascontiguousarray(I, dtype=np.int32).reshape(n, nn + 1)
+    D = np.ascontiguousarray(D, dtype=np.float64).reshape(n, nn + 1)
+    rowptr = np.empty(n + 1, np.int32)
+    col = np.empty(n * nn, np.int32)
+    val = np.empty(n * nn, np.float32)
+    _check(lib().sgtsne_perplexity_equalize(I, D, int(n), int(nn), float(u), rowptr, col, val))
+    return CSR(n, rowptr, col, val)
+
+
+def sgtsne(P, params, y_in=None, time_info=None):
+    """Drop-in for sgtsneCUDA / sgtsne: returns Y in params.layout ([d][N] for
+    SoA, [N][d] for AoS).  time_info: optional float64 array of 7*maxIter+1."""
+    n = P.n
+    d = params.d
+    shape = (d, n) if params.layout == LAYOUT_SOA else (n, d)
+    y_out = np.empty(shape, np.float32)
+    yp = None
+    if y_in is not None:
+        y_in = np.ascontiguousarray(y_in, dtype=np.float32).reshape(shape)
+        yp = y_in.ctypes.data
+    tp = None
+    if time_info is not None:
+        assert time_info.dtype == np.float64 and time_info.size >= 7 * params.maxIter + 1
+        tp = time_info.ctypes.data
+    _check(lib().sgtsne_run(n, P.nnz, P.rowptr, P.col, P.val, C.byref(params), yp, y_out, tp))
+    return y_out
+
+
+class Context:
+    """One embedding job on the device (stepwise API)."""
+
+    def __init__(self, params):
+        self.params = params
+        self._h = C.c_void_p()
+        _check(lib().sgtsne_ctx_create(C.byref(self._h), C.byref(params)))
+        self.d = params.d
+
+    def close(self):
+        if self._h:
+            lib().sgtsne_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_P(self, P, symmetrize=True):
+        _check(lib().sgtsne_ctx_upload_P(self._h, P.n, P.nnz, P.rowptr, P.col, P.val, int(bool(symmetrize))))
+        self.n = P.n
+
+    def get_P(self):
+        nnz = C.c_int64()
+        _check(lib().sgtsne_ctx_get_P(self._h, C.byref(nnz), None, None, None))
+        rp = np.empty(self.n + 1, np.int32)
+        col = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.float32)
+        _check(lib().sgtsne_ctx_get_P(self._h, C.byref(nnz), rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return CSR(self.n, rp, col, val)
+
+    def set_Y(self, y, layout=LAYOUT_AOS, reset_state=True):
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        _check(lib().sgtsne_ctx_set_Y(self._h, y.reshape(-1), int(layout), int(bool(reset_state))))
+
+    def get_Y(self, layout=LAYOUT_AOS):
+        shape = (self.n, self.d) if layout == LAYOUT_AOS else (self.d, self.n)
+        y = np.empty(shape, np.float32)
+        _check(lib().sgtsne_ctx_get_Y(self._h, y.reshape(-1), int(layout)))
+        return y
+
+    def step(self, iters=1):
+        _check(lib().sgtsne_ctx_step(self._h, int(iters)))
+
+    def terms(self, layout=LAYOUT_AOS):
+        """(Fattr, Frep, Z, ng) for the current Y (Frep normalised by Z)."""
+        shape = (self.n, self.d) if layout == LAYOUT_AOS else (self.d, self.n)
+        Fa = np.empty(shape, np.float32)
+        Fr = np.empty(shape, np.float32)
+        Z = C.c_double()
+        ng = C.c_int()
+        _check(lib().sgtsne_ctx_terms(self._h, Fa.ctypes.data, Fr.ctypes.data, C.byref(Z), C.byref(ng), int(layout)))
+        return Fa, Fr, Z.value, ng.value
+
+    def kl(self, alpha=1.0):
+        out = C.c_double()
+        _check(lib().sgtsne_ctx_kl(self._h, float(alpha), C.byref(out)))
+        return out.value
+
+    def repulsive_exact(self, layout=LAYOUT_AOS):
+        shape = (self.n, self.d) if layout == LAYOUT_AOS else (self.d, self.n)
+        Fr = np.empty(shape, np.float32)
+        Z = C.c_double()
+        _check(lib().sgtsne_ctx_repulsive_exact(self._h, Fr.ctypes.data, C.byref(Z), int(layout)))
+        return Fr, Z.value
+
+    def timing(self):
+        out = np.zeros(7, np.float64)
+        _check(lib().sgtsne_ctx_timing(self._h, out))
+        return out
+
+    def set_timing(self, on):
+        _check(lib().sgtsne_ctx_set_timing(self._h, int(bool(on))))
+
+    def errors(self):
+        cnt = C.c_int()
+        _check(lib().sgtsne_ctx_errors(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), np.float64)
+        _check(lib().sgtsne_ctx_errors(self._h, out.ctypes.data, cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def info(self):
+        it, n, d, nnz = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        _check(lib().sgtsne_ctx_info(self._h, C.byref(it), C.byref(n), C.byref(d), C.byref(nnz)))
+        return {"iter": it.value, "n": n.value, "d": d.value, "nnz": nnz.value}

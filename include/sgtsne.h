@@ -1,0 +1,187 @@
+/*
+ * sgtsne.h -- C-ABI of the MI355X (gfx950) SG-t-SNE engine.
+ *
+ * Plain C: int status codes, plain pointers and sizes, per-context state, one
+ * HIP stream per context, no globals.  Host pointers unless a name says
+ * "_dev".  Every entry point names the reference interface it replaces
+ * (paths relative to the reference root: CU = code/cuda/SG-tSNE-CUDA/src,
+ * HY = code/cuda/SG-tSNE-HYB/sgtsnepi/src).
+ *
+ * Errors: the reference aborts the process (gpuErrchk -> exit, CU/common.cuh:15-22;
+ * exit(1) on an out-of-grid point, HY/nuconv.cpp:43-44).  Here every call
+ * returns SGTSNE_OK or a negative code and sgtsne_last_error() describes it.
+ */
+#ifndef SGTSNE_H
+#define SGTSNE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define SGTSNE_OK 0
+#define SGTSNE_ERR_ARG (-1)       /* invalid argument / size              */
+#define SGTSNE_ERR_HIP (-2)       /* HIP runtime error                     */
+#define SGTSNE_ERR_NOMEM (-3)     /* device or host allocation failed      */
+#define SGTSNE_ERR_NUMERIC (-4)   /* non-finite embedding (out of grid)    */
+#define SGTSNE_ERR_STATE (-5)     /* call order (e.g. step before upload)  */
+#define SGTSNE_ERR_NODEV (-6)     /* no gfx950 device visible              */
+
+/* ---- sparse formats: the reference's numbering (CU/types.hpp:66,
+ *      CU/pq.cu:25-57): 0 CSR, 1 BSR (stub in the reference, rejected
+ *      here), 2 COO, 3 hybrid ELL+COO.  4 is this engine's own
+ *      wavefront-segmented CSR (the default).                             */
+#define SGTSNE_FORMAT_CSR 0
+#define SGTSNE_FORMAT_COO 2
+#define SGTSNE_FORMAT_HYBRID 3
+#define SGTSNE_FORMAT_CSR_SEG 4
+
+/* grid-size policy: h-driven (HY/qq.cpp:165-166, the varGrid semantics) or
+ * the fixed getBestGridSize(ng) of the plain CUDA path (CU/gradient_descend.cu:139) */
+#define SGTSNE_GRID_H 0
+#define SGTSNE_GRID_FIXED 1
+
+/* Y layouts: SoA [d][N] (CU path) or AoS [N][d] (HYB path) */
+#define SGTSNE_LAYOUT_SOA 0
+#define SGTSNE_LAYOUT_AOS 1
+
+/* Parameter set: every field of tsneparams (CU/types.hpp:55-72; HYB subset
+ * HY/sgtsne.hpp:25-37) plus the engine's layout and grid-policy flags. */
+typedef struct sgtsne_params {
+  int d;              /* embedding dimensions 1..3                      (2)   */
+  double lambda;      /* lambda rescaling; 1 skips it                   (1)   */
+  double alpha;       /* early exaggeration multiplier                  (12)  */
+  int max_iter;       /* gradient iterations                            (1000)*/
+  int early_iter;     /* iterations 0..early_iter use alpha             (250) */
+  int n;              /* number of points (filled from P when 0)              */
+  double h;           /* grid side length; <= 0: 0.5/0.7/1.2 for d=1/2/3      */
+  int drop_leaf;      /* drop edges of leaf rows during lambda rescaling      */
+  int np;             /* CPU workers in the reference; unused on the GPU      */
+  int format;         /* SGTSNE_FORMAT_*                        (CSR_SEG)     */
+  int bs;             /* BSR block size in the reference; must be 0           */
+  int compute_error;  /* > 0: record the relative L2 error of the interpolated
+                         repulsion against the exact O(N^2) one every 10
+                         iterations (CU/gradient_descend.cu:206-215)          */
+  int ng;             /* grid request for SGTSNE_GRID_FIXED             (50)  */
+  int grid_mode;      /* SGTSNE_GRID_*                          (H)           */
+  int layout;         /* SGTSNE_LAYOUT_* of y_in / y_out        (SOA)         */
+  uint64_t seed;      /* seed of the N(0, 1e-4^2) start when y_in is NULL     */
+  int ell_width;      /* hybrid ELL width; <= 0 = automatic                   */
+} sgtsne_params;
+
+/* Fills the defaults listed above.  Replaces the in-class defaults of
+ * tsneparams (CU/types.hpp:55-72). */
+int sgtsne_params_default(sgtsne_params *p);
+
+/* Description of the last error raised on this thread (never NULL). */
+const char *sgtsne_last_error(void);
+
+/* Library build string (version, target). */
+const char *sgtsne_version(void);
+
+/* ---- P construction front end --------------------------------------- */
+
+/* Exact all-kNN on the GPU: k+1 neighbours per point, self first, squared
+ * FP64 distances summed in dimension order with fma, ties broken by the lower
+ * index (bit-exact against a CPU computation of the same order).
+ * X row-major [n][dim] float; I [n][k+1] int32; D [n][k+1] double.
+ * Replaces allKNNsearchflann (CU/allknn.cuh:75-92), the approximate
+ * FLANN search the reference calls from CU/testing_modules/demo_perplexity.cu:131. */
+int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D);
+
+/* Perplexity equalisation on the GPU (FP64 bisection, |H - log u| < 1e-5,
+ * <= 200 steps, self column dropped, rows normalised).  I, D as above with
+ * k = nn.  Output CSR: rowptr [n+1] (= k*i), col [n*k], val [n*k] float.
+ * Replaces perplexityEqualization (CU/perplexity_search.cuh:18-20,
+ * CU/perplexity_search.cu:114-141; FP64 twin HY/sgtsne.cpp:257-297). */
+int sgtsne_perplexity_equalize(const int32_t *I, const double *D, int n, int k, double u,
+                               int32_t *rowptr, int32_t *col, float *val);
+
+/* Non-periodic convolution of an ng^d grid with m <= 4 channels by the
+ * kernel (1 + h^2 r^2)^-2, r in grid steps, on the GPU: the G2G step of the
+ * repulsion (HY/non_periodic_conv.cpp:26-708, CU/non_periodic_convF.cu:162-517).
+ * V and Phi are host arrays [m][ng^d], x fastest; 4 <= ng <= 512. */
+int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi);
+
+/* ---- drop-in entry -------------------------------------------------- */
+
+/* Runs the whole method: lambda rescaling (if lambda != 1), P + P^T and
+ * normalisation to sum 1, the sparse format build, initialisation (y_in or a
+ * seeded N(0, 1e-4^2) start), params.max_iter iterations, mean-centring.
+ * P is the row-stochastic kNN CSR (rowptr [n+1], col/val [nnz]).
+ * y_in / y_out use params.layout; y_in may be NULL; time_info may be NULL,
+ * else receives 7 doubles per iteration (ms: attractive, S2G, G2G, G2S,
+ * forces, repulsion total, preparation) followed by the total loop time
+ * (ms) at index 7*max_iter.
+ * Replaces sgtsneCUDA (CU/sgtsne.cuh:13-18, CU/sgtsne.cu:26-128) and
+ * sgtsne (HY/sgtsne.hpp:68-72, HY/sgtsne.cpp:37-197). */
+int sgtsne_run(int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, const float *val,
+               const sgtsne_params *params, const float *y_in, float *y_out, double *time_info);
+
+/* ---- stepwise context API (parity, batching, serving) ----------------- */
+
+typedef struct sgtsne_ctx sgtsne_ctx;
+
+/* Creates a context on the current device (own stream, own buffers).
+ * Replaces the allocation block of kl_minimization (CU/gradient_descend.cu:125-188). */
+int sgtsne_ctx_create(sgtsne_ctx **ctx, const sgtsne_params *params);
+
+/* Uploads P (CSR, host memory).  symmetrize != 0: lambda rescaling (if
+ * params.lambda != 1), P_sym = (P + P^T) / sum and the format build run on the
+ * GPU (CU/sgtsne.cu:50-93, CU/graph_rescaling.cu:266-398).  symmetrize == 0:
+ * P is taken as the final symmetric, normalised matrix. */
+int sgtsne_ctx_upload_P(sgtsne_ctx *ctx, int n, int64_t nnz, const int32_t *rowptr,
+                        const int32_t *col, const float *val, int symmetrize);
+
+/* Downloads the device P in CSR (after symmetrisation).  Call with col ==
+ * NULL to query nnz into *nnz. */
+int sgtsne_ctx_get_P(sgtsne_ctx *ctx, int64_t *nnz, int32_t *rowptr, int32_t *col, float *val);
+
+/* Sets / reads Y (layout per argument), and resets the optimiser state
+ * (uy = 0, gains = 1, iteration counter = 0) when reset_state != 0. */
+int sgtsne_ctx_set_Y(sgtsne_ctx *ctx, const float *y, int layout, int reset_state);
+int sgtsne_ctx_get_Y(sgtsne_ctx *ctx, float *y, int layout);
+
+/* Runs `iters` iterations of the schedule from the context's iteration
+ * counter (CU/gradient_descend.cu:193-234; HY/gradient_descendHYB.cpp:203-244). */
+int sgtsne_ctx_step(sgtsne_ctx *ctx, int iters);
+
+/* Evaluates, for the current Y and without updating it, the per-iteration
+ * terms the oracle compares: Fattr (attractive, without alpha), Frep (repulsive,
+ * normalised by Z), Z, and the grid side ng (= nGrid + 2).  Fattr/Frep in
+ * `layout`; any output pointer may be NULL.
+ * Replaces compute_gradient (CU/gradient_descend.cu:53-101). */
+int sgtsne_ctx_terms(sgtsne_ctx *ctx, float *Fattr, float *Frep, double *Z, int *ng, int layout);
+
+/* KL divergence of the current Y with exaggeration `alpha`
+ * (CU/pq.cu:61-125, tsneCost). */
+int sgtsne_ctx_kl(sgtsne_ctx *ctx, double alpha, double *kl);
+
+/* Exact O(N^2) repulsion on the GPU (validation; CU/compute_error.cu:8-53). */
+int sgtsne_ctx_repulsive_exact(sgtsne_ctx *ctx, float *Frep, double *Z, int layout);
+
+/* Per-phase device time of the last sgtsne_ctx_step call, ms summed over
+ * its iterations: [0] attractive, [1] S2G (incl. binning), [2] G2G (FFT
+ * convolution), [3] G2S + forces, [4] update, [5] total, [6] iterations. */
+int sgtsne_ctx_timing(sgtsne_ctx *ctx, double *out7);
+
+/* Enables (default) or disables the per-phase event timing of ctx_step. */
+int sgtsne_ctx_set_timing(sgtsne_ctx *ctx, int on);
+
+/* Relative L2 errors recorded by params.compute_error > 0 (one every 10
+ * iterations); *count receives how many exist, up to max are copied. */
+int sgtsne_ctx_errors(sgtsne_ctx *ctx, double *out, int max, int *count);
+
+/* Current iteration counter, n, d, nnz. */
+int sgtsne_ctx_info(sgtsne_ctx *ctx, int *iter, int *n, int *d, int64_t *nnz);
+
+int sgtsne_ctx_destroy(sgtsne_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGTSNE_H */

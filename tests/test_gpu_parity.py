@@ -1,0 +1,299 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C-ABI.
+
+Tolerances: integer / index outputs bit-exact; FP32 terms within 1e-4
+relative (L2 norm) of the FP64 oracle from identical P and Y; Z within 1e-4
+relative; final KL within 1%."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-4
+
+
+def _mixture(n, dim, clusters, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, 4, size=(clusters, dim))
+    return (c[rng.integers(0, clusters, n)] + rng.standard_normal((n, dim))).astype(np.float32)
+
+
+def _oracle_csr(O, P):
+    return O.CSR(P.n, P.rowptr.astype(np.int64), P.col, P.val.astype(np.float64))
+
+
+@pytest.fixture(scope="module")
+def mnist():
+    return np.load(os.path.join(GOLD, "mnist500.npz")), np.load(os.path.join(GOLD, "mnist500_oracle.npz"))
+
+
+# ---------------------------------------------------------------- kNN
+def test_knn_bitexact_golden(sg, mnist):
+    f, g = mnist
+    I, D = sg.knn_exact(f["X"], 90)
+    assert np.array_equal(I, g["I"])
+    assert np.array_equal(D, g["D"])
+
+
+@pytest.mark.parametrize("n,dim,k", [(1000, 50, 90), (777, 7, 30), (130, 3, 129)])
+def test_knn_bitexact_random(sg, O, n, dim, k):
+    X = _mixture(n, dim, 5, n)
+    I, D = sg.knn_exact(X, k)
+    Io, Do = O.knn_exact(X, k)
+    assert np.array_equal(I, Io) and np.array_equal(D, Do)
+
+
+def test_knn_ties(sg, O):
+    X = np.repeat(np.arange(40, dtype=np.float32)[:, None] % 4, 3, axis=1)  # many duplicates
+    I, D = sg.knn_exact(X, 15)
+    Io, Do = O.knn_exact(X, 15)
+    assert np.array_equal(I, Io) and np.array_equal(D, Do)
+
+
+# ---------------------------------------------------------------- P construction
+def test_perplexity_golden(sg, mnist):
+    f, g = mnist
+    P = sg.perplexityEqualization(g["I"], g["D"], 500, 90, 30.0)
+    assert np.array_equal(P.rowptr, g["P_rowptr"]) and np.array_equal(P.col, g["P_col"])
+    assert np.abs(P.val - g["P_val"]).max() <= 2e-7 * np.abs(g["P_val"]).max()
+
+
+def test_perplexity_random(sg, O):
+    X = _mixture(3000, 20, 8, 1)
+    I, D = O.knn_exact(X, 45)
+    P = sg.perplexityEqualization(I, D, 3000, 45, 15.0)
+    Po = O.perplexity_equalize(I, D, 15.0)
+    assert np.array_equal(P.col, Po.col)
+    assert np.allclose(P.val, Po.val, rtol=1e-6, atol=1e-12)
+    assert np.allclose(np.add.reduceat(P.val.astype(np.float64), P.rowptr[:-1]), 1.0, atol=1e-5)
+
+
+def test_symmetrize_golden(sg, mnist):
+    f, g = mnist
+    P = sg.CSR(500, g["P_rowptr"], g["P_col"], g["P_val"].astype(np.float32))
+    with sg.Context(sg.tsneparams()) as ctx:
+        ctx.upload_P(P, symmetrize=True)
+        S = ctx.get_P()
+    assert np.array_equal(S.rowptr, g["S_rowptr"]) and np.array_equal(S.col, g["S_col"])
+    assert np.allclose(S.val, g["S_val"], rtol=1e-6, atol=0)
+
+
+def test_lambda_rescale(sg, O):
+    X = _mixture(800, 10, 4, 2)
+    I, D = O.knn_exact(X, 20)
+    Po = O.perplexity_equalize(I, D, 7.0)
+    P = sg.CSR(800, Po.rowptr, Po.col, Po.val.astype(np.float32))
+    with sg.Context(sg.tsneparams(**{"lambda": 3.0})) as ctx:
+        ctx.upload_P(P, symmetrize=True)
+        S = ctx.get_P()
+    Pl = _oracle_csr(O, P)
+    O.lambda_rescale(Pl, 3.0)
+    So = O.symmetrize(Pl)
+    assert np.array_equal(S.col, So.col)
+    assert np.allclose(S.val, So.val, rtol=2e-5, atol=0)
+
+
+# ---------------------------------------------------------------- FFT convolution
+@pytest.mark.parametrize("d,ng", [(1, 16), (1, 117), (1, 512), (2, 16), (2, 52), (2, 66), (2, 130),
+                                  (3, 16), (3, 28)])
+def test_grid_conv_vs_oracle(sg, O, d, ng):
+    rng = np.random.default_rng(ng + d)
+    V = rng.uniform(0, 1, size=(d + 1,) + (ng,) * d)
+    for h in (0.05, 0.7):
+        Phi = sg.grid_conv(V.astype(np.float32), h)
+        if ng ** d <= 4096:
+            ref = O.conv_nopad_direct(V, h)
+        else:
+            ref = O.conv_nopad_fft(V, h)
+        assert rel_l2(Phi, ref) < 1e-5
+
+
+@pytest.mark.parametrize("d,ng", [(1, 200), (2, 64), (2, 90), (3, 20)])
+def test_grid_conv_vs_rocfft(sg, d, ng):
+    # a zero-padded convolution through torch.fft (rocFFT) in FP32
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU for torch")
+    rng = np.random.default_rng(3 * ng + d)
+    V = rng.uniform(0, 1, size=(d + 1,) + (ng,) * d).astype(np.float32)
+    h = 0.3
+    L = 2 * ng
+    idx = torch.arange(L, device="cuda", dtype=torch.float32)
+    m = torch.minimum(idx, L - idx)
+    r2 = sum(g ** 2 for g in torch.meshgrid(*([m] * d), indexing="ij"))
+    K = (1.0 + h * h * r2) ** -2
+    Vt = torch.zeros((d + 1,) + (L,) * d, device="cuda")
+    Vt[(slice(None),) + (slice(0, ng),) * d] = torch.from_numpy(V).cuda()
+    dims = tuple(range(1, d + 1))
+    out = torch.fft.ifftn(torch.fft.fftn(Vt, dim=dims) * torch.fft.fftn(K), dim=dims).real
+    ref = out[(slice(None),) + (slice(0, ng),) * d].cpu().numpy()
+    Phi = sg.grid_conv(V, h)
+    assert rel_l2(Phi, ref) < 1e-5
+
+
+# ---------------------------------------------------------------- per-iteration terms
+def _terms_oracle(O, S, Y, h):
+    Yd = Y.astype(np.float64)
+    Fa = O.attractive(S, Yd)
+    Fr, Z, ng = O.repulsive_interp(Yd.copy(), h)
+    return Fa, Fr, Z, ng
+
+
+@pytest.mark.parametrize("fmt", ["CSR_SEG", "CSR", "COO", "HYBRID"])
+def test_terms_golden(sg, mnist, fmt):
+    f, g = mnist
+    S = sg.CSR(500, g["S_rowptr"], g["S_col"], g["S_val"].astype(np.float32))
+    with sg.Context(sg.tsneparams(format=getattr(sg, "FORMAT_" + fmt))) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        for tag in ("y0", "ys"):
+            ctx.set_Y(g[f"{tag}_Y"])
+            Fa, Fr, Z, ng = ctx.terms()
+            assert ng == g[f"{tag}_ng"]
+            assert rel_l2(Fa, g[f"{tag}_Fattr"]) < TOL
+            assert rel_l2(Fr, g[f"{tag}_Frep"]) < TOL
+            assert abs(Z - g[f"{tag}_Z"]) / g[f"{tag}_Z"] < TOL
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("fmt", ["CSR_SEG", "COO", "HYBRID"])
+def test_terms_random(sg, O, d, fmt):
+    n = 4000
+    X = _mixture(n, 30, 6, 10 + d)
+    I, D = O.knn_exact(X, 60)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 20.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(d)
+    with sg.Context(sg.tsneparams(d=d, format=getattr(sg, "FORMAT_" + fmt))) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        for scale in (1e-3, 3.0, 40.0):
+            Y = (rng.standard_normal((n, d)) * scale).astype(np.float32)
+            ctx.set_Y(Y)
+            Yc = ctx.get_Y()  # centred copy: the oracle sees the same Y
+            Fa, Fr, Z, ng = ctx.terms()
+            Fao, Fro, Zo, ngo = _terms_oracle(O, So, Yc, {1: 0.5, 2: 0.7, 3: 1.2}[d])
+            assert ng == ngo
+            assert rel_l2(Fa, Fao) < TOL
+            assert rel_l2(Fr, Fro) < TOL
+            assert abs(Z - Zo) / Zo < TOL
+
+
+def test_terms_ragged_rows(sg, O):
+    # rows from 0 to 300 entries, including empty rows and a hub
+    rng = np.random.default_rng(9)
+    n = 2000
+    lens = rng.integers(0, 40, n)
+    lens[:5] = 0
+    lens[7] = 300
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, n, rows.size)
+    A = {}
+    for r, c in zip(rows, cols):
+        if r != c:
+            A[(r, c)] = rng.uniform(0.1, 1)
+            A[(c, r)] = A[(r, c)]
+    keys = sorted(A)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int32)
+    val = np.array([A[k] for k in keys])
+    val /= val.sum()
+    So = O.CSR(n, rp, col, val)
+    for fmt in ("CSR_SEG", "CSR", "COO", "HYBRID"):
+        with sg.Context(sg.tsneparams(format=getattr(sg, "FORMAT_" + fmt))) as ctx:
+            ctx.upload_P(sg.CSR(n, rp, col, val.astype(np.float32)), symmetrize=False)
+            Y = (rng.standard_normal((n, 2)) * 5).astype(np.float32)
+            ctx.set_Y(Y)
+            Yc = ctx.get_Y()
+            Fa, Fr, Z, ng = ctx.terms()
+            Fao, Fro, Zo, _ = _terms_oracle(O, So, Yc, 0.7)
+            assert rel_l2(Fa, Fao) < TOL, fmt
+            assert np.all(Fa[:5] == 0)
+
+
+def test_exact_repulsion(sg, O):
+    n = 3000
+    rng = np.random.default_rng(11)
+    Y = (rng.standard_normal((n, 2)) * 10).astype(np.float32)
+    S = sg.CSR(n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    with sg.Context(sg.tsneparams()) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        ctx.set_Y(Y)
+        Yc = ctx.get_Y()
+        Fe, Ze = ctx.repulsive_exact()
+    Fo, Zo = O.repulsive_exact(Yc.astype(np.float64))
+    assert rel_l2(Fe, Fo) < 1e-5 and abs(Ze - Zo) / Zo < 1e-6
+
+
+def test_kl_golden(sg, mnist):
+    f, g = mnist
+    S = sg.CSR(500, g["S_rowptr"], g["S_col"], g["S_val"].astype(np.float32))
+    with sg.Context(sg.tsneparams()) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        ctx.set_Y(g["ys_Y"])
+        kl = ctx.kl(1.0)
+    assert abs(kl - g["ys_kl"]) / abs(g["ys_kl"]) < TOL
+
+
+# ---------------------------------------------------------------- iterations
+def test_iterations_golden(sg, O, mnist):
+    # 100 iterations from the reference's own start (initial2.mat)
+    f, g = mnist
+    S = sg.CSR(500, g["S_rowptr"], g["S_col"], g["S_val"].astype(np.float32))
+    with sg.Context(sg.tsneparams()) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        ctx.set_Y(f["Y0"])
+        ctx.step(100)
+        Y = ctx.get_Y()
+        kl = ctx.kl(1.0)
+    Yo = g["run100_Y"]
+    So = O.CSR(500, g["S_rowptr"].astype(np.int64), g["S_col"], g["S_val"])
+    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7)
+    klo = O.kl(So, Yo, Zo, 1.0)
+    assert abs(kl - klo) / klo < 0.01
+    assert rel_l2(Y, Yo) < 0.05
+
+
+def test_full_run_final_kl(sg, O):
+    # the drop-in entry, 1000 iterations, final KL within 1% of the FP64 loop
+    n = 2500
+    X = _mixture(n, 50, 10, 12)
+    I, D = sg.knn_exact(X, 90)
+    P = sg.perplexityEqualization(I, D, n, 90, 30.0)
+    y0 = (1e-4 * np.random.default_rng(1).standard_normal((n, 2))).astype(np.float32)
+    prm = sg.tsneparams(layout=sg.LAYOUT_AOS)
+    ti = np.zeros(7 * prm.maxIter + 1)
+    Y = sg.sgtsne(P, prm, y_in=y0, time_info=ti)
+    assert np.all(np.isfinite(Y)) and ti[7 * prm.maxIter] > 0 and ti[0] > 0
+    So = O.symmetrize(_oracle_csr(O, P))
+    Yo = y0.astype(np.float64).copy()
+    O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, 1000)
+    _, Zg, _ = O.repulsive_interp(Y.astype(np.float64).copy(), 0.7)
+    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7)
+    klg = O.kl(So, Y.astype(np.float64), Zg)
+    klo = O.kl(So, Yo, Zo)
+    assert abs(klg - klo) / klo < 0.01, (klg, klo)
+
+
+def test_soa_layout_and_errors(sg):
+    n = 600
+    X = _mixture(n, 10, 3, 13)
+    I, D = sg.knn_exact(X, 30)
+    P = sg.perplexityEqualization(I, D, n, 30, 10.0)
+    prm = sg.tsneparams(maxIter=20, layout=sg.LAYOUT_SOA, d=3)
+    Y = sg.sgtsne(P, prm)
+    assert Y.shape == (3, n) and np.all(np.isfinite(Y))
+    # a non-finite embedding is reported, not propagated
+    with sg.Context(sg.tsneparams()) as ctx:
+        ctx.upload_P(P, symmetrize=True)
+        Yb = np.zeros((n, 2), np.float32)
+        Yb[5, 0] = np.nan
+        ctx.set_Y(Yb)
+        with pytest.raises(sg.SgtsneError, match="-4"):
+            ctx.step(1)
