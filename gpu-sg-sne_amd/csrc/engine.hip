@@ -14,6 +14,7 @@
 #include "../../include/sgtsne.h"
 
 #include <algorithm>
+#include <climits>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -123,6 +124,10 @@ struct sgtsne_ctx {
 
   // compute_error history
   std::vector<double> errors;
+
+  // locality order: device row r holds the caller's point perm[r]
+  bool permuted = false;
+  std::vector<int32_t> perm, iperm;
 };
 
 namespace {
@@ -171,6 +176,8 @@ int check_params(const sgtsne_params *p) {
             "bad grid_mode");
   SGT_CHECK(p->layout == SGTSNE_LAYOUT_SOA || p->layout == SGTSNE_LAYOUT_AOS, SGTSNE_ERR_ARG,
             "bad layout");
+  SGT_CHECK(p->zeta_mode == SGTSNE_ZETA_DIRECT || p->zeta_mode == SGTSNE_ZETA_ALGEBRAIC, SGTSNE_ERR_ARG,
+            "bad zeta_mode");
   return SGTSNE_OK;
 }
 
@@ -212,7 +219,7 @@ int make_grid(sgtsne_ctx *c, const DevScalars &s, GridParams &g) {
 }
 
 int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
-  const int m = c->d + 1;
+  const int m = c->d + 2;  // charges 0..d, plus the direct-Z channel of Phi
   const size_t G = c->d == 1 ? (size_t)g.ng : c->d == 2 ? (size_t)g.ng * g.ng : (size_t)g.ng * g.ng * g.ng;
   const size_t gb = G * m * sizeof(float);
   size_t wb = 0, kb = 0;
@@ -265,13 +272,14 @@ int attractive(sgtsne_ctx *c) {
 int repulsive(sgtsne_ctx *c, const GridParams &g, PhaseEvents *pe) {
   const int m = c->d + 1;
   const int64_t G = c->d == 1 ? g.ng : c->d == 2 ? (int64_t)g.ng * g.ng : (int64_t)g.ng * g.ng * g.ng;
+  const int zdirect = c->p.zeta_mode == SGTSNE_ZETA_DIRECT;
   SGT_TRY(ensure_grid(c, g));
   SGT_TRY(launch_fill(c->stream, c->V, G * m, 0.0f));
-  SGT_TRY(launch_s2g_atomic(c->stream, g, c->n, c->Y, c->V));
+  SGT_TRY(launch_s2g(c->stream, g, c->n, c->Y, c->V));
   if (pe) SGT_HIP(hipEventRecord(pe->e[2], c->stream));
-  SGT_TRY(launch_conv(c->stream, g, m, c->V, c->Phi, c->cb));
+  SGT_TRY(launch_conv(c->stream, g, zdirect, c->V, c->Phi, c->cb));
   if (pe) SGT_HIP(hipEventRecord(pe->e[3], c->stream));
-  SGT_TRY(launch_g2s_force(c->stream, g, c->n, c->Y, c->Phi, c->Frep, c->rw, c->sc));
+  SGT_TRY(launch_g2s_force(c->stream, g, c->n, c->Y, c->Phi, c->Frep, zdirect, c->rw, c->sc));
   if (pe) SGT_HIP(hipEventRecord(pe->e[4], c->stream));
   c->last_grid = g;
   return SGTSNE_OK;
@@ -364,20 +372,85 @@ int iterate(sgtsne_ctx *c) {
   return SGTSNE_OK;
 }
 
-void pack_y(const float *y, int layout, int n, int d, int ds, std::vector<float> &out) {
+// caller order <-> device order (perm empty: identity)
+void pack_y(const float *y, int layout, int n, int d, int ds, const std::vector<int32_t> &perm,
+            std::vector<float> &out) {
   out.assign((size_t)n * ds, 0.0f);
-  for (int i = 0; i < n; i++)
+  for (int r = 0; r < n; r++) {
+    const int i = perm.empty() ? r : perm[r];
     for (int a = 0; a < d; a++)
-      out[(size_t)i * ds + a] = layout == SGTSNE_LAYOUT_AOS ? y[(size_t)i * d + a] : y[(size_t)a * n + i];
+      out[(size_t)r * ds + a] = layout == SGTSNE_LAYOUT_AOS ? y[(size_t)i * d + a] : y[(size_t)a * n + i];
+  }
 }
 
-void unpack_y(const std::vector<float> &in, int layout, int n, int d, int ds, float *y, double scale = 1.0) {
-  for (int i = 0; i < n; i++)
+void unpack_y(const std::vector<float> &in, int layout, int n, int d, int ds, const std::vector<int32_t> &perm,
+              float *y, double scale = 1.0) {
+  for (int r = 0; r < n; r++) {
+    const int i = perm.empty() ? r : perm[r];
     for (int a = 0; a < d; a++) {
-      float v = (float)(in[(size_t)i * ds + a] * scale);
+      float v = (float)(in[(size_t)r * ds + a] * scale);
       if (layout == SGTSNE_LAYOUT_AOS) y[(size_t)i * d + a] = v;
       else y[(size_t)a * n + i] = v;
     }
+  }
+}
+
+// Breadth-first order of the graph of P (Cuthill-McKee without the degree
+// sort): each component from its lowest-degree vertex, neighbours in column
+// order.  Rows that are neighbours in P end up close in memory, so the
+// gathers of the attractive term stay in cache.
+void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t> &perm) {
+  perm.assign(n, 0);
+  std::vector<uint8_t> seen(n, 0);
+  int start = 0, mind = INT32_MAX;
+  for (int i = 0; i < n; i++)
+    if (rp[i + 1] - rp[i] < mind) { mind = rp[i + 1] - rp[i]; start = i; }
+  int head = 0, tail = 0, seed = 0;
+  seen[start] = 1;
+  perm[tail++] = start;
+  while (tail < n) {
+    if (head == tail) {
+      while (seen[seed]) ++seed;
+      seen[seed] = 1;
+      perm[tail++] = seed;
+    }
+    const int v = perm[head++];
+    for (int t = rp[v]; t < rp[v + 1]; t++) {
+      const int u = col[t];
+      if (!seen[u]) { seen[u] = 1; perm[tail++] = u; }
+    }
+  }
+}
+
+int apply_locality_order(sgtsne_ctx *c) {
+  const int n = c->n;
+  std::vector<int32_t> rp((size_t)n + 1), col((size_t)std::max<int64_t>(c->nnz, 1));
+  SGT_HIP(hipMemcpy(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost));
+  if (c->nnz > 0) SGT_HIP(hipMemcpy(col.data(), c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  bfs_order(n, rp.data(), col.data(), c->perm);
+  c->iperm.assign(n, 0);
+  for (int r = 0; r < n; r++) c->iperm[c->perm[r]] = r;
+  int32_t *dperm = nullptr, *diperm = nullptr, *rp2 = nullptr, *col2 = nullptr;
+  float *val2 = nullptr;
+  SGT_TRY(alloc_dev((void **)&dperm, sizeof(int32_t) * (size_t)n));
+  SGT_TRY(alloc_dev((void **)&diperm, sizeof(int32_t) * (size_t)n));
+  SGT_TRY(alloc_dev((void **)&rp2, sizeof(int32_t) * ((size_t)n + 1)));
+  SGT_TRY(alloc_dev((void **)&col2, sizeof(int32_t) * (size_t)std::max<int64_t>(c->nnz, 1)));
+  SGT_TRY(alloc_dev((void **)&val2, sizeof(float) * (size_t)std::max<int64_t>(c->nnz, 1)));
+  SGT_HIP(hipMemcpyAsync(dperm, c->perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  SGT_HIP(hipMemcpyAsync(diperm, c->iperm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  int st = launch_permute_csr(c->stream, n, c->nnz, dperm, diperm, c->rowptr, c->col, c->val, rp2, col2, val2);
+  (void)hipStreamSynchronize(c->stream);
+  free_dev(dperm);
+  free_dev(diperm);
+  free_dev(c->rowptr);
+  free_dev(c->col);
+  free_dev(c->val);
+  c->rowptr = rp2;
+  c->col = col2;
+  c->val = val2;
+  c->permuted = true;
+  return st;
 }
 
 int build_formats(sgtsne_ctx *c) {
@@ -442,6 +515,8 @@ int sgtsne_params_default(sgtsne_params *p) {
   p->layout = SGTSNE_LAYOUT_SOA;
   p->seed = 0;
   p->ell_width = 0;
+  p->zeta_mode = SGTSNE_ZETA_DIRECT;
+  p->reorder = 1;
   return SGTSNE_OK;
 }
 
@@ -549,6 +624,10 @@ int sgtsne_ctx_upload_P(sgtsne_ctx *c, int n, int64_t nnz, const int32_t *rowptr
   }
   free_P(c);
   c->n = n;
+  c->have_Y = false;  // the device order of Y follows P: set Y again
+  c->permuted = false;
+  c->perm.clear();
+  c->iperm.clear();
   int32_t *rp = nullptr, *cc = nullptr;
   float *vv = nullptr;
   SGT_TRY(alloc_dev((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
@@ -575,6 +654,7 @@ int sgtsne_ctx_upload_P(sgtsne_ctx *c, int n, int64_t nnz, const int32_t *rowptr
     c->val = vv;
     c->nnz = nnz;
   }
+  if (c->p.reorder) SGT_TRY(apply_locality_order(c));
   SGT_TRY(build_formats(c));
   SGT_HIP(hipStreamSynchronize(c->stream));
   c->have_P = true;
@@ -585,9 +665,41 @@ int sgtsne_ctx_get_P(sgtsne_ctx *c, int64_t *nnz, int32_t *rowptr, int32_t *col,
   SGT_CHECK(c != nullptr && nnz != nullptr, SGTSNE_ERR_ARG, "bad arguments");
   SGT_CHECK(c->have_P, SGTSNE_ERR_STATE, "no P uploaded");
   *nnz = c->nnz;
-  if (rowptr) SGT_HIP(hipMemcpy(rowptr, c->rowptr, sizeof(int32_t) * ((size_t)c->n + 1), hipMemcpyDeviceToHost));
-  if (col) SGT_HIP(hipMemcpy(col, c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
-  if (val) SGT_HIP(hipMemcpy(val, c->val, sizeof(float) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  if (!rowptr && !col && !val) return SGTSNE_OK;
+  const int n = c->n;
+  std::vector<int32_t> rp((size_t)n + 1), cc((size_t)std::max<int64_t>(c->nnz, 1));
+  std::vector<float> vv((size_t)std::max<int64_t>(c->nnz, 1));
+  SGT_HIP(hipMemcpy(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost));
+  if (c->nnz > 0) {
+    SGT_HIP(hipMemcpy(cc.data(), c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+    SGT_HIP(hipMemcpy(vv.data(), c->val, sizeof(float) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  }
+  if (!c->permuted) {
+    if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
+    if (col) std::copy(cc.begin(), cc.begin() + c->nnz, col);
+    if (val) std::copy(vv.begin(), vv.begin() + c->nnz, val);
+    return SGTSNE_OK;
+  }
+  // back to the caller's order, columns ascending within each row
+  std::vector<int32_t> orp((size_t)n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    const int r = c->iperm[i];
+    orp[i + 1] = orp[i] + (rp[r + 1] - rp[r]);
+  }
+  std::vector<std::pair<int32_t, float>> row;
+  for (int i = 0; i < n; i++) {
+    const int r = c->iperm[i];
+    row.clear();
+    for (int t = rp[r]; t < rp[r + 1]; t++) row.emplace_back(c->perm[cc[t]], vv[t]);
+    std::sort(row.begin(), row.end(), [](const std::pair<int32_t, float> &a, const std::pair<int32_t, float> &b) {
+      return a.first < b.first;
+    });
+    for (size_t e = 0; e < row.size(); e++) {
+      if (col) col[orp[i] + e] = row[e].first;
+      if (val) val[orp[i] + e] = row[e].second;
+    }
+  }
+  if (rowptr) std::copy(orp.begin(), orp.end(), rowptr);
   return SGTSNE_OK;
 }
 
@@ -608,7 +720,7 @@ int sgtsne_ctx_set_Y(sgtsne_ctx *c, const float *y, int layout, int reset_state)
     reset_state = 1;
   }
   std::vector<float> h;
-  pack_y(y, layout, n, c->d, c->ds, h);
+  pack_y(y, layout, n, c->d, c->ds, c->perm, h);
   double mean[4] = {0, 0, 0, 0};
   for (int a = 0; a < c->d; a++) {
     double s = 0;
@@ -638,7 +750,7 @@ int sgtsne_ctx_get_Y(sgtsne_ctx *c, float *y, int layout) {
   std::vector<float> h((size_t)c->n * c->ds);
   SGT_HIP(hipStreamSynchronize(c->stream));
   SGT_HIP(hipMemcpy(h.data(), c->Y, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  unpack_y(h, layout, c->n, c->d, c->ds, y);
+  unpack_y(h, layout, c->n, c->d, c->ds, c->perm, y);
   return SGTSNE_OK;
 }
 
@@ -671,11 +783,11 @@ int sgtsne_ctx_terms(sgtsne_ctx *c, float *Fattr, float *Frep, double *Z, int *n
   std::vector<float> h(nb);
   if (Fattr) {
     SGT_HIP(hipMemcpy(h.data(), c->Fattr, nb * sizeof(float), hipMemcpyDeviceToHost));
-    unpack_y(h, layout, c->n, c->d, c->ds, Fattr);
+    unpack_y(h, layout, c->n, c->d, c->ds, c->perm, Fattr);
   }
   if (Frep) {
     SGT_HIP(hipMemcpy(h.data(), c->Frep, nb * sizeof(float), hipMemcpyDeviceToHost));
-    unpack_y(h, layout, c->n, c->d, c->ds, Frep, 1.0 / s.Z);
+    unpack_y(h, layout, c->n, c->d, c->ds, c->perm, Frep, 1.0 / s.Z);
   }
   if (Z) *Z = s.Z;
   if (ng) *ng = g.ng;
@@ -712,7 +824,7 @@ int sgtsne_ctx_repulsive_exact(sgtsne_ctx *c, float *Frep, double *Z, int layout
   }
   free_dev(tmp);
   SGT_TRY(st);
-  if (Frep) unpack_y(h, layout, c->n, c->d, c->ds, Frep, 1.0 / s.zexact);
+  if (Frep) unpack_y(h, layout, c->n, c->d, c->ds, c->perm, Frep, 1.0 / s.zexact);
   if (Z) *Z = s.zexact;
   return SGTSNE_OK;
 }
@@ -812,10 +924,11 @@ int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *
   return st;
 }
 
-int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi) {
+int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float *Phi) {
   SGT_CHECK(V && Phi, SGTSNE_ERR_ARG, "NULL argument");
-  SGT_CHECK(d >= 1 && d <= 3 && m >= 1 && m <= 4 && ng >= 4 && ng <= 512 && h > 0, SGTSNE_ERR_ARG,
-            "bad grid_conv sizes (d=%d ng=%d m=%d)", d, ng, m);
+  SGT_CHECK(d >= 1 && d <= 3 && ng >= 4 && ng <= 512 && h > 0, SGTSNE_ERR_ARG,
+            "bad grid_conv sizes (d=%d ng=%d)", d, ng);
+  const int m = d + 1, mo = zdirect ? d + 2 : d + 1;
   GridParams g{};
   g.d = d;
   g.ng = ng;
@@ -833,7 +946,7 @@ int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi)
   ConvBuffers cb{};
   float *Vd = nullptr, *Pd = nullptr;
   int st = alloc_dev((void **)&Vd, G * m * sizeof(float));
-  if (!st) st = alloc_dev((void **)&Pd, G * m * sizeof(float));
+  if (!st) st = alloc_dev((void **)&Pd, G * mo * sizeof(float));
   if (!st) st = alloc_dev((void **)&cb.W, wb);
   if (!st) st = alloc_dev((void **)&cb.Kt1, kb);
   if (!st) st = alloc_dev((void **)&cb.Kt2, kb);
@@ -844,8 +957,8 @@ int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi)
     set_error("copy of V failed");
     st = SGTSNE_ERR_HIP;
   }
-  if (!st) st = launch_conv(s, g, m, Vd, Pd, cb);
-  if (!st && hipMemcpyAsync(Phi, Pd, G * m * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) {
+  if (!st) st = launch_conv(s, g, zdirect, Vd, Pd, cb);
+  if (!st && hipMemcpyAsync(Phi, Pd, G * mo * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) {
     set_error("copy of Phi failed");
     st = SGTSNE_ERR_HIP;
   }

@@ -57,9 +57,10 @@ int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t 
 int launch_fill(hipStream_t s, float *p, int64_t count, float v);
 int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, RedWork rw,
                          DevScalars *sc);
-int launch_s2g_atomic(hipStream_t s, const GridParams &g, int n, const float *Y, float *V);
+int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V);
 int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
-                     float *Frep, RedWork rw, DevScalars *sc);
+                     float *Frep, int zdirect, RedWork rw, DevScalars *sc);
+int launch_finalize_kl(hipStream_t s, int nblocks, RedWork rw, DevScalars *sc);
 int launch_update(hipStream_t s, int d, int n, float *Y, float *uY, float *gains,
                   const float *Fattr, const float *Frep, float alpha, float momentum, float eta,
                   RedWork rw, DevScalars *sc);
@@ -69,13 +70,14 @@ int launch_rep_exact(hipStream_t s, int d, int n, const float *Y, float *Frep, R
 // ---- FFT convolution (kernels_fft.hip) ----
 struct ConvBuffers {
   float2 *W;      // complex work [pairs][...]
-  float *Kt1;     // real half-spectrum scratch
-  float *Kt2;
-  float *Kh;      // real half-spectrum of the kernel [(L/2+1)^d]
+  float2 *Kt1;    // (K2, K1) half-spectrum scratch
+  float2 *Kt2;
+  float2 *Kh;     // (K2^, K1^) half-spectra of the kernels [(L/2+1)^d]
   float2 *tw;     // twiddles exp(-2 pi i t / 1024), t < 512
 };
 int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes);
-int launch_conv(hipStream_t s, const GridParams &g, int m, const float *V, float *Phi,
+// V [d+1][ng^d] -> Phi [d+1 (+1 with zdirect: K1 * V0 in channel d+1)][ng^d]
+int launch_conv(hipStream_t s, const GridParams &g, int zdirect, const float *V, float *Phi,
                 const ConvBuffers &b);
 int init_twiddles(hipStream_t s, float2 *tw);
 
@@ -95,5 +97,8 @@ int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, co
                       const float *val, const int32_t *tail_off, int32_t *trow, int32_t *tcol,
                       float *tval);
 int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n);
+int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
+                       const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
+                       int32_t *col2, float *val2);
 
 }  // namespace sgt

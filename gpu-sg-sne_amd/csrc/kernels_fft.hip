@@ -8,9 +8,13 @@
 // embedding K_circ[i] = K(min(i, L - i)) gives exactly the same product on the
 // ng outputs, so here L is the next power of two and every line transform is a
 // radix-2 Stockham FFT held in LDS.  K_circ is even in every axis, so its
-// spectrum is real and even: only its (L/2+1)^d half-spectrum is kept, and two
-// real grid channels are packed into one complex line (their results are the
-// real and imaginary parts of the inverse transform).
+// spectrum is real and even: only its (L/2+1)^d half-spectrum is kept.  Two
+// real grid channels share one complex line (their results are the real and
+// imaginary parts of the inverse transform).  The two kernels K2 = (1+h^2r^2)^-2
+// and K1 = (1+h^2r^2)^-1 are transformed together as K2 + i K1 (both spectra
+// are real), and the unit-charge channel V0 is transformed alone and multiplied
+// by K2^ + i K1^, which returns K2*V0 + i K1*V0: the potential and the direct
+// normalisation term from one line.
 //
 // Layouts: grid V/Phi [m][ng^d] (x fastest, HY/gridding.cpp:22-24);
 // complex work W per channel pair: 2-D [ng][L] (rows y < ng, all kx),
@@ -40,14 +44,15 @@ struct PassArgs {
   int n_in;
   int64_t o_os, o_is, o_es, pair_out;
   int n_out;
-  int m;        // channel count of V / Phi
   float scale;  // scale of OUT_PHIPAIR
   float hsq;    // h^2 for IN_KGEN
   int H;        // L / 2 + 1
   int kdim;     // spectrum index form for MID_CONV
+  int kpow;     // IN_KGEN: 0 -> (K2, K1) pair, 2 -> K2 only
+  int in_c[4][2], out_c[4][2], kcplx[4];  // per-pair channels (-1: none) and spectrum kind
   const void *in;
   void *out;
-  const float *Kh;
+  const float2 *Kh;
   const float2 *tw;
 };
 
@@ -107,21 +112,21 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
       if (IN == IN_VPAIR) {
         const float *V = static_cast<const float *>(a.in);
         const int64_t off = outer * a.i_os + inner * a.i_is + e * a.i_es;
-        const int c0 = 2 * p;
-        v.x = V[(int64_t)c0 * a.pair_in + off];
-        if (c0 + 1 < a.m) v.y = V[(int64_t)(c0 + 1) * a.pair_in + off];
+        v.x = V[(int64_t)a.in_c[p][0] * a.pair_in + off];
+        if (a.in_c[p][1] >= 0) v.y = V[(int64_t)a.in_c[p][1] * a.pair_in + off];
       } else if (IN == IN_W) {
         const float2 *W = static_cast<const float2 *>(a.in);
         v = W[(int64_t)p * a.pair_in + outer * a.i_os + inner * a.i_is + e * a.i_es];
       } else if (IN == IN_KGEN) {
         const int me = mirror(e, L);
         const float r2 = (float)me * me + (float)inner * inner + (float)outer * outer;
-        const float t = 1.0f + a.hsq * r2;
-        v.x = 1.0f / (t * t);
-      } else {  // IN_KT: real half-spectrum, mirrored along the pass axis
-        const float *K = static_cast<const float *>(a.in);
+        const float k1 = 1.0f / (1.0f + a.hsq * r2);
+        v.x = k1 * k1;
+        if (a.kpow == 0) v.y = k1;
+      } else {  // IN_KT: (K2, K1) half-spectra, mirrored along the pass axis
+        const float2 *K = static_cast<const float2 *>(a.in);
         const int me = mirror(e, L);
-        v.x = K[outer * a.i_os + inner * a.i_is + me * a.i_es];
+        v = K[outer * a.i_os + inner * a.i_is + me * a.i_es];
       }
     }
     b0[l * L + e] = v;
@@ -139,7 +144,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
       const int l = idx >> logL, e = idx & (L - 1);
       const int line = line0 + l;
-      float kv = 0.f;
+      float kv = 0.f, kvi = 0.f;
       if (line < a.nlines) {
         const int outer = line / a.ninner;
         const int inner = line - outer * a.ninner;
@@ -148,10 +153,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
         if (a.kdim == 1) ki = me;
         else if (a.kdim == 2) ki = (int64_t)me * a.H + mirror(inner, L);
         else ki = ((int64_t)me * a.H + mirror(outer, L)) * a.H + mirror(inner, L);
-        kv = a.Kh[ki];
+        const float2 k = a.Kh[ki];
+        kv = k.x;
+        kvi = a.kcplx[p] ? k.y : 0.0f;
       }
-      float2 v = r[idx];
-      r[idx] = make_float2(v.x * kv, v.y * kv);
+      const float2 v = r[idx];
+      r[idx] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
     }
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
@@ -174,12 +181,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
       W[(int64_t)p * a.pair_out + off] = v;
     } else if (OUT == OUT_PHIPAIR) {
       float *Phi = static_cast<float *>(a.out);
-      const int c0 = 2 * p;
-      Phi[(int64_t)c0 * a.pair_out + off] = v.x * a.scale;
-      if (c0 + 1 < a.m) Phi[(int64_t)(c0 + 1) * a.pair_out + off] = v.y * a.scale;
-    } else {
-      float *K = static_cast<float *>(a.out);
-      K[off] = v.x;
+      Phi[(int64_t)a.out_c[p][0] * a.pair_out + off] = v.x * a.scale;
+      if (a.out_c[p][1] >= 0) Phi[(int64_t)a.out_c[p][1] * a.pair_out + off] = v.y * a.scale;
+    } else {  // the spectra of real even kernels are real: keep (Re, Im) = (K2^, K1^)
+      float2 *K = static_cast<float2 *>(a.out);
+      K[off] = v;
     }
   }
 }
@@ -204,7 +210,37 @@ int run_pass(hipStream_t s, PassArgs a, int pairs) {
   return SGTSNE_OK;
 }
 
-PassArgs base_args(const GridParams &g, int m, const ConvBuffers &b) {
+// Channel plan of one convolution.  zdirect: pair 0 = (V0, -) with the
+// spectrum K2^ + i K1^ -> (Phi0, Psi0 in channel d+1); then the charge
+// channels 1..d in pairs with K2^.  Otherwise: channels 0..d in pairs with K2^.
+struct Plan {
+  int pairs;
+  int in_c[4][2], out_c[4][2], kcplx[4];
+};
+
+Plan make_plan(int d, bool zdirect) {
+  Plan pl{};
+  int p = 0;
+  int c = 0;
+  if (zdirect) {
+    pl.in_c[0][0] = 0; pl.in_c[0][1] = -1;
+    pl.out_c[0][0] = 0; pl.out_c[0][1] = d + 1;
+    pl.kcplx[0] = 1;
+    p = 1;
+    c = 1;
+  }
+  for (; c <= d; c += 2, ++p) {
+    pl.in_c[p][0] = c;
+    pl.in_c[p][1] = c + 1 <= d ? c + 1 : -1;
+    pl.out_c[p][0] = pl.in_c[p][0];
+    pl.out_c[p][1] = pl.in_c[p][1];
+    pl.kcplx[p] = 0;
+  }
+  pl.pairs = p;
+  return pl;
+}
+
+PassArgs base_args(const GridParams &g, const Plan &pl, const ConvBuffers &b) {
   PassArgs a{};
   a.L = g.L;
   a.logL = g.logL;
@@ -214,13 +250,20 @@ PassArgs base_args(const GridParams &g, int m, const ConvBuffers &b) {
   int lg = 0;
   while ((1 << (lg + 1)) <= lpw) ++lg;
   a.lpw_log = lg;
-  a.m = m;
   a.H = g.L / 2 + 1;
   a.hsq = (float)(g.hgrid * g.hgrid);
   a.tw = b.tw;
   a.Kh = b.Kh;
   a.scale = 1.0f;
   a.ninner = 1;
+  a.kpow = 0;
+  for (int p = 0; p < 4; ++p) {
+    a.in_c[p][0] = pl.in_c[p][0];
+    a.in_c[p][1] = pl.in_c[p][1];
+    a.out_c[p][0] = pl.out_c[p][0];
+    a.out_c[p][1] = pl.out_c[p][1];
+    a.kcplx[p] = pl.kcplx[p];
+  }
   return a;
 }
 
@@ -233,28 +276,30 @@ int init_twiddles(hipStream_t s, float2 *tw) {
 }
 
 int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes) {
-  const int pairs = 2;  // (d + 1 + 1) / 2 <= 2 for d <= 3
+  const int pairs = 3;  // at most (1 + ceil(d / 2)) for d <= 3
   const size_t H = (size_t)L / 2 + 1;
   size_t wl = d == 1 ? 0 : d == 2 ? (size_t)ng * L : (size_t)ng * L * L;
   *w_bytes = pairs * wl * sizeof(float2);
   size_t hd = d == 1 ? H : d == 2 ? H * H : H * H * H;
-  *k_bytes = hd * sizeof(float);
+  *k_bytes = hd * sizeof(float2);
   return SGTSNE_OK;
 }
 
-int launch_conv(hipStream_t s, const GridParams &g, int m, const float *V, float *Phi,
+int launch_conv(hipStream_t s, const GridParams &g, int zdirect, const float *V, float *Phi,
                 const ConvBuffers &b) {
   const int d = g.d, ng = g.ng, L = g.L, H = L / 2 + 1;
-  const int pairs = (m + 1) / 2;
+  const Plan pl = make_plan(d, zdirect != 0);
+  const int pairs = pl.pairs;
   SGT_CHECK(L <= (1 << kTwLog) && L >= 2 * ng - 1, SGTSNE_ERR_ARG, "FFT length %d invalid for ng %d", L, ng);
   const int64_t G = d == 1 ? ng : d == 2 ? (int64_t)ng * ng : (int64_t)ng * ng * ng;
+  const int kpow = zdirect ? 0 : 2;
 
   if (d == 1) {
-    PassArgs k = base_args(g, m, b);  // kernel spectrum
-    k.nlines = 1; k.ninner = 1; k.n_in = L; k.n_out = H; k.o_es = 1;
+    PassArgs k = base_args(g, pl, b);  // kernel spectra
+    k.nlines = 1; k.ninner = 1; k.n_in = L; k.n_out = H; k.o_es = 1; k.kpow = kpow;
     k.out = b.Kh;
     SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, k, 1)));
-    PassArgs c = base_args(g, m, b);
+    PassArgs c = base_args(g, pl, b);
     c.nlines = 1; c.ninner = 1;
     c.in = V; c.i_es = 1; c.n_in = ng; c.pair_in = G;
     c.out = Phi; c.o_es = 1; c.n_out = ng; c.pair_out = G;
@@ -264,28 +309,28 @@ int launch_conv(hipStream_t s, const GridParams &g, int m, const float *V, float
   }
 
   if (d == 2) {
-    PassArgs ka = base_args(g, m, b);
-    ka.nlines = H; ka.ninner = H; ka.n_in = L; ka.o_is = H; ka.o_es = 1; ka.n_out = H;
+    PassArgs ka = base_args(g, pl, b);
+    ka.nlines = H; ka.ninner = H; ka.n_in = L; ka.o_is = H; ka.o_es = 1; ka.n_out = H; ka.kpow = kpow;
     ka.out = b.Kt1;
     SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, ka, 1)));
-    PassArgs kb = base_args(g, m, b);
+    PassArgs kb = base_args(g, pl, b);
     kb.nlines = H; kb.ninner = H; kb.in = b.Kt1; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
     kb.out = b.Kh; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
     SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kb, 1)));
 
     const int64_t wp = (int64_t)ng * L;
-    PassArgs pa = base_args(g, m, b);  // rows: V -> W
+    PassArgs pa = base_args(g, pl, b);  // rows: V -> W
     pa.nlines = ng; pa.ninner = ng;
     pa.in = V; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
     pa.out = b.W; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
     SGT_TRY((run_pass<IN_VPAIR, OUT_W, MID_FWD, true>(s, pa, pairs)));
-    PassArgs pb = base_args(g, m, b);  // columns: fwd, x Kh, inv
+    PassArgs pb = base_args(g, pl, b);  // columns: fwd, x spectrum, inv
     pb.nlines = L; pb.ninner = L;
     pb.in = b.W; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
     pb.out = b.W; pb.o_is = 1; pb.o_es = L; pb.n_out = ng; pb.pair_out = wp;
     pb.kdim = 2;
     SGT_TRY((run_pass<IN_W, OUT_W, MID_CONV, false>(s, pb, pairs)));
-    PassArgs pc = base_args(g, m, b);  // rows: inverse -> Phi
+    PassArgs pc = base_args(g, pl, b);  // rows: inverse -> Phi
     pc.nlines = ng; pc.ninner = ng;
     pc.in = b.W; pc.i_is = L; pc.i_es = 1; pc.n_in = L; pc.pair_in = wp;
     pc.out = Phi; pc.o_is = ng; pc.o_es = 1; pc.n_out = ng; pc.pair_out = G;
@@ -296,42 +341,42 @@ int launch_conv(hipStream_t s, const GridParams &g, int m, const float *V, float
 
   // d == 3
   const int64_t H2 = (int64_t)H * H, L2 = (int64_t)L * L;
-  PassArgs ka = base_args(g, m, b);
-  ka.nlines = (int)H2; ka.ninner = H; ka.n_in = L;
+  PassArgs ka = base_args(g, pl, b);
+  ka.nlines = (int)H2; ka.ninner = H; ka.n_in = L; ka.kpow = kpow;
   ka.out = b.Kt1; ka.o_os = H2; ka.o_is = H; ka.o_es = 1; ka.n_out = H;
   SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, ka, 1)));
-  PassArgs kb = base_args(g, m, b);
+  PassArgs kb = base_args(g, pl, b);
   kb.nlines = (int)H2; kb.ninner = H; kb.in = b.Kt1; kb.i_os = H2; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
   kb.out = b.Kt2; kb.o_os = H2; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
   SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kb, 1)));
-  PassArgs kc = base_args(g, m, b);
+  PassArgs kc = base_args(g, pl, b);
   kc.nlines = (int)H2; kc.ninner = H; kc.in = b.Kt2; kc.i_os = H; kc.i_is = 1; kc.i_es = H2; kc.n_in = L;
   kc.out = b.Kh; kc.o_os = H; kc.o_is = 1; kc.o_es = H2; kc.n_out = H;
   SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kc, 1)));
 
   const int64_t wp = (int64_t)ng * L2;
-  PassArgs pa = base_args(g, m, b);  // x: V -> W
+  PassArgs pa = base_args(g, pl, b);  // x: V -> W
   pa.nlines = ng * ng; pa.ninner = ng;
   pa.in = V; pa.i_os = (int64_t)ng * ng; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
   pa.out = b.W; pa.o_os = L2; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
   SGT_TRY((run_pass<IN_VPAIR, OUT_W, MID_FWD, true>(s, pa, pairs)));
-  PassArgs pb = base_args(g, m, b);  // y forward (all ky kept)
+  PassArgs pb = base_args(g, pl, b);  // y forward (all ky kept)
   pb.nlines = ng * L; pb.ninner = L;
   pb.in = b.W; pb.i_os = L2; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
   pb.out = b.W; pb.o_os = L2; pb.o_is = 1; pb.o_es = L; pb.n_out = L; pb.pair_out = wp;
   SGT_TRY((run_pass<IN_W, OUT_W, MID_FWD, false>(s, pb, pairs)));
-  PassArgs pc = base_args(g, m, b);  // z: fwd, x Kh, inv (z < ng kept)
+  PassArgs pc = base_args(g, pl, b);  // z: fwd, x spectrum, inv (z < ng kept)
   pc.nlines = (int)L2; pc.ninner = L;
   pc.in = b.W; pc.i_os = L; pc.i_is = 1; pc.i_es = L2; pc.n_in = ng; pc.pair_in = wp;
   pc.out = b.W; pc.o_os = L; pc.o_is = 1; pc.o_es = L2; pc.n_out = ng; pc.pair_out = wp;
   pc.kdim = 3;
   SGT_TRY((run_pass<IN_W, OUT_W, MID_CONV, false>(s, pc, pairs)));
-  PassArgs pd = base_args(g, m, b);  // y inverse (y < ng kept)
+  PassArgs pd = base_args(g, pl, b);  // y inverse (y < ng kept)
   pd.nlines = ng * L; pd.ninner = L;
   pd.in = b.W; pd.i_os = L2; pd.i_is = 1; pd.i_es = L; pd.n_in = L; pd.pair_in = wp;
   pd.out = b.W; pd.o_os = L2; pd.o_is = 1; pd.o_es = L; pd.n_out = ng; pd.pair_out = wp;
   SGT_TRY((run_pass<IN_W, OUT_W, MID_INV, false>(s, pd, pairs)));
-  PassArgs pe = base_args(g, m, b);  // x inverse -> Phi
+  PassArgs pe = base_args(g, pl, b);  // x inverse -> Phi
   pe.nlines = ng * ng; pe.ninner = ng;
   pe.in = b.W; pe.i_os = L2; pe.i_is = L; pe.i_es = 1; pe.n_in = L; pe.pair_in = wp;
   pe.out = Phi; pe.o_os = (int64_t)ng * ng; pe.o_is = ng; pe.o_es = 1; pe.n_out = ng; pe.pair_out = G;

@@ -9,6 +9,10 @@
 //   gather (G2S)      HY/gridding.cpp:508-560 / CU/gridding.cu:187-230
 //   forces and Z      HY/qq.cpp:75-106 / CU/Frep.cu:43-80
 //   update            CU/gradient_descend.cu:12-44, HY/gradient_descendHYB.cpp:29-78
+//
+// Reductions: every block writes its partials to rw.part[block * 8 + k]; a
+// one-block finalize kernel combines them in a fixed order (deterministic),
+// so no kernel needs a device-wide fence.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "../../include/sgtsne.h"
@@ -19,48 +23,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// ---- partial reduction helpers -------------------------------------------
-// Each block writes `cnt` doubles to rw.part[block*8 ...]; the block that
-// takes the last ticket sees all partials (release/acquire through the
-// ticket counter at agent scope) and finalises.
-__device__ __forceinline__ void store_partial(double *slot, double v) {
-  __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_partial(const double *slot) {
-  return __hip_atomic_load(const_cast<double *>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ bool take_last_ticket(unsigned *ticket) {
-  __shared__ int am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last != 0;
-}
-
-__device__ __forceinline__ void reset_ticket(unsigned *ticket) {
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int K>
-__device__ void block_reduce_sum(double (&v)[K], double *lds /* [K*4] */) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[k * 4 + wid] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = lds[k * 4 + 0] + lds[k * 4 + 1] + lds[k * 4 + 2] + lds[k * 4 + 3];
-  __syncthreads();
-}
+enum FinMode { FIN_MINMAX = 0, FIN_ZNEU = 1, FIN_MEAN = 2, FIN_ZEXACT = 3, FIN_KL = 4 };
 
 // Lagrange cubic weights for the 4 stencil nodes f..f+3 of a point at
 // f + 1 + delta (HY/gridding.cpp:28-40: g1 at distance < 1, g2 at 1..2).
@@ -86,6 +49,34 @@ __device__ __forceinline__ void stencil(const GridParams &g, const float (&y)[D]
   }
 }
 
+// Visits the 4^D stencil nodes of one point: fn(linear node index, weight).
+template <int D, class Fn>
+__device__ __forceinline__ void for_stencil(int ng, const int (&f)[D], const float (&w)[D][4], Fn fn) {
+  if constexpr (D == 1) {
+#pragma unroll
+    for (int i0 = 0; i0 < 4; ++i0) fn(f[0] + i0, w[0][i0]);
+  } else if constexpr (D == 2) {
+#pragma unroll
+    for (int i1 = 0; i1 < 4; ++i1)
+#pragma unroll
+      for (int i0 = 0; i0 < 4; ++i0) fn((f[1] + i1) * ng + f[0] + i0, w[1][i1] * w[0][i0]);
+  } else {
+#pragma unroll
+    for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+      for (int i1 = 0; i1 < 4; ++i1) {
+        const float w21 = w[2][i2] * w[1][i1];
+#pragma unroll
+        for (int i0 = 0; i0 < 4; ++i0) fn(((f[2] + i2) * ng + f[1] + i1) * ng + f[0] + i0, w21 * w[0][i0]);
+      }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ int grid_cells(int ng) {
+  return D == 1 ? ng : D == 2 ? ng * ng : ng * ng * ng;
+}
+
 // ---- fill ------------------------------------------------------------------
 __global__ void fill_kernel(float *__restrict__ p, int64_t count, float v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,17 +86,18 @@ __global__ void fill_kernel(float *__restrict__ p, int64_t count, float v) {
 
 // ---- centring + min/max ------------------------------------------------------
 // Y -= mean (the mean of the last update, HY/gradient_descendHYB.cpp:64-76),
-// then min/max per dimension of the centred Y (HY/qq.cpp:147-163).
+// then per-block min/max of the centred Y (HY/qq.cpp:147-163) and a count of
+// non-finite coordinates.
 template <int D>
 __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict__ Y, int n,
-                                                               int apply_mean, RedWork rw,
-                                                               DevScalars *sc) {
-  __shared__ double lds[2 * D * 4 + 4];
+                                                               int apply_mean, double *part,
+                                                               const DevScalars *sc) {
+  __shared__ double lds[(2 * D + 1) * 4];
   float m[D];
 #pragma unroll
   for (int a = 0; a < D; ++a) m[a] = apply_mean ? (float)sc->mean[a] : 0.0f;
   float mn[D], mx[D];
-  float bad = 0.0f;  // count of non-finite coordinates
+  float bad = 0.0f;
 #pragma unroll
   for (int a = 0; a < D; ++a) { mn[a] = INFINITY; mx[a] = -INFINITY; }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -123,7 +115,6 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
       bad += isfinite(y[a]) ? 0.0f : 1.0f;
     }
   }
-  const double vbad = wave_sum((double)bad);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double vmin[D], vmax[D];
 #pragma unroll
@@ -131,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
     vmin[a] = wave_min((double)mn[a]);
     vmax[a] = wave_max((double)mx[a]);
   }
+  const double vbad = wave_sum((double)bad);
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -141,101 +133,97 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    store_partial(&rw.part[blockIdx.x * 8 + 3],
-                  lds[2 * D * 4] + lds[2 * D * 4 + 1] + lds[2 * D * 4 + 2] + lds[2 * D * 4 + 3]);
+    double *pb = part + (size_t)blockIdx.x * 8;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      double lo = fmin(fmin(lds[a * 4], lds[a * 4 + 1]), fmin(lds[a * 4 + 2], lds[a * 4 + 3]));
-      double hi = fmax(fmax(lds[(D + a) * 4], lds[(D + a) * 4 + 1]),
+      pb[a] = fmin(fmin(lds[a * 4], lds[a * 4 + 1]), fmin(lds[a * 4 + 2], lds[a * 4 + 3]));
+      pb[4 + a] = fmax(fmax(lds[(D + a) * 4], lds[(D + a) * 4 + 1]),
                        fmax(lds[(D + a) * 4 + 2], lds[(D + a) * 4 + 3]));
-      store_partial(&rw.part[blockIdx.x * 8 + a], lo);
-      store_partial(&rw.part[blockIdx.x * 8 + 4 + a], hi);
     }
-  }
-  if (take_last_ticket(rw.ticket)) {
-    if (threadIdx.x == 0) {
-      double nbad = 0.0;
-      for (int b = 0; b < (int)gridDim.x; ++b) nbad += load_partial(&rw.part[b * 8 + 3]);
-      for (int a = 0; a < D; ++a) {
-        double lo = INFINITY, hi = -INFINITY;
-        for (int b = 0; b < (int)gridDim.x; ++b) {
-          lo = fmin(lo, load_partial(&rw.part[b * 8 + a]));
-          hi = fmax(hi, load_partial(&rw.part[b * 8 + 4 + a]));
-        }
-        // a non-finite coordinate poisons the extent: the host rejects it
-        sc->minv[a] = nbad > 0.0 ? NAN : lo;
-        sc->maxv[a] = hi;
-        if (apply_mean) sc->mean[a] = 0.0;
-      }
-    }
-    reset_ticket(rw.ticket);
+    pb[3] = lds[2 * D * 4] + lds[2 * D * 4 + 1] + lds[2 * D * 4 + 2] + lds[2 * D * 4 + 3];
   }
 }
 
-// ---- S2G with global float atomics (baseline form) ----------------------------
+// ---- S2G, LDS-privatised whole grid -------------------------------------------
+// For grids that fit in LDS: each block spreads a contiguous chunk of points
+// into a private LDS copy of the grid (LDS float atomics), then adds its
+// non-zero cells to the global grid.  This removes the global-atomic
+// contention of clustered embeddings on small grids.
+template <int D>
+__global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
+                                                         float *__restrict__ V, int chunk) {
+  extern __shared__ float sgrid[];
+  const int ng = g.ng;
+  const int G = grid_cells<D>(ng);
+  constexpr int M = D + 1;
+  for (int e = threadIdx.x; e < M * G; e += blockDim.x) sgrid[e] = 0.0f;
+  __syncthreads();
+  const int i0 = blockIdx.x * chunk;
+  const int i1 = min(n, i0 + chunk);
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    float y[D];
+    load_pt<D>(Y, i, y);
+    int f[D];
+    float w[D][4];
+    stencil<D>(g, y, f, w);
+    float q[M];
+    q[0] = 1.0f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) q[a + 1] = y[a] - g.center[a];
+    for_stencil<D>(ng, f, w, [&](int idx, float ww) {
+#pragma unroll
+      for (int c = 0; c < M; ++c) atomicAdd(&sgrid[c * G + idx], q[c] * ww);
+    });
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < M * G; e += blockDim.x) {
+    const float v = sgrid[e];
+    if (v != 0.0f) unsafeAtomicAdd(&V[e], v);
+  }
+}
+
+// ---- S2G with global float atomics (large grids) -------------------------------
 template <int D>
 __global__ __launch_bounds__(kBlock) void s2g_atomic_kernel(GridParams g, int n,
                                                             const float *__restrict__ Y,
                                                             float *__restrict__ V) {
   const int ng = g.ng;
-  const int G = D == 1 ? ng : D == 2 ? ng * ng : ng * ng * ng;
+  const int G = grid_cells<D>(ng);
+  constexpr int M = D + 1;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float y[D];
     load_pt<D>(Y, i, y);
     int f[D];
     float w[D][4];
     stencil<D>(g, y, f, w);
-    float q[D + 1];
+    float q[M];
     q[0] = 1.0f;
 #pragma unroll
     for (int a = 0; a < D; ++a) q[a + 1] = y[a] - g.center[a];
-    if constexpr (D == 1) {
+    for_stencil<D>(ng, f, w, [&](int idx, float ww) {
 #pragma unroll
-      for (int i0 = 0; i0 < 4; ++i0) {
-        int idx = f[0] + i0;
-#pragma unroll
-        for (int c = 0; c <= D; ++c) unsafeAtomicAdd(&V[c * G + idx], q[c] * w[0][i0]);
-      }
-    } else if constexpr (D == 2) {
-#pragma unroll
-      for (int i1 = 0; i1 < 4; ++i1)
-#pragma unroll
-        for (int i0 = 0; i0 < 4; ++i0) {
-          float ww = w[1][i1] * w[0][i0];
-          int idx = (f[1] + i1) * ng + f[0] + i0;
-#pragma unroll
-          for (int c = 0; c <= D; ++c) unsafeAtomicAdd(&V[c * G + idx], q[c] * ww);
-        }
-    } else {
-#pragma unroll
-      for (int i2 = 0; i2 < 4; ++i2)
-#pragma unroll
-        for (int i1 = 0; i1 < 4; ++i1)
-#pragma unroll
-          for (int i0 = 0; i0 < 4; ++i0) {
-            float ww = w[2][i2] * w[1][i1] * w[0][i0];
-            int idx = ((f[2] + i2) * ng + f[1] + i1) * ng + f[0] + i0;
-#pragma unroll
-            for (int c = 0; c <= D; ++c) unsafeAtomicAdd(&V[c * G + idx], q[c] * ww);
-          }
-    }
+      for (int c = 0; c < M; ++c) unsafeAtomicAdd(&V[c * G + idx], q[c] * ww);
+    });
   }
 }
 
 // ---- G2S + forces + Z ------------------------------------------------------------
-// Phi_c(y_i) by the same stencil; Frep_i (unnormalised) = ~y_i Phi0 - Phi_{1..d};
-// z_i = (1 + 2|~y_i|^2) Phi0 - 2 ~y_i . Phi_{1..d} in FP64 and a compensated
-// sum; the last block writes Z = sum z_i - N (HY/qq.cpp:75-106).  ~y is y minus
-// the charge shift used by S2G (any shift leaves Frep and sum z_i unchanged).
-template <int D>
+// Phi_c(y_i) by the same stencil; Frep_i (unnormalised) = ~y_i Phi0 - Phi_{1..d}.
+// Z per point, compensated sum over points, FP64:
+//   ZDIRECT: z_i = psi_i, the gathered convolution of the unit charges with
+//            1/(1 + r^2) (channel D+1), so Z = sum_i psi_i - N;
+//   else:    z_i = (1 + 2|~y_i|^2) Phi0 - 2 ~y_i . Phi_{1..d} (HY/qq.cpp:75-106).
+// ~y is y minus the charge shift used by S2G (any shift leaves Frep and
+// sum z_i unchanged).
+template <int D, bool ZDIRECT>
 __global__ __launch_bounds__(kBlock) void g2s_force_kernel(GridParams g, int n,
                                                            const float *__restrict__ Y,
                                                            const float *__restrict__ Phi,
-                                                           float *__restrict__ Frep, RedWork rw,
-                                                           DevScalars *sc) {
+                                                           float *__restrict__ Frep, double *part) {
   __shared__ double lds[8];
   const int ng = g.ng;
-  const int G = D == 1 ? ng : D == 2 ? ng * ng : ng * ng * ng;
+  const int G = grid_cells<D>(ng);
+  constexpr int C = ZDIRECT ? D + 2 : D + 1;
   Neu z{0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float y[D];
@@ -243,52 +231,25 @@ __global__ __launch_bounds__(kBlock) void g2s_force_kernel(GridParams g, int n,
     int f[D];
     float w[D][4];
     stencil<D>(g, y, f, w);
-    float ph[D + 1];
+    float ph[C];
 #pragma unroll
-    for (int c = 0; c <= D; ++c) ph[c] = 0.0f;
-    if constexpr (D == 1) {
+    for (int c = 0; c < C; ++c) ph[c] = 0.0f;
+    for_stencil<D>(ng, f, w, [&](int idx, float ww) {
 #pragma unroll
-      for (int i0 = 0; i0 < 4; ++i0) {
-        int idx = f[0] + i0;
-#pragma unroll
-        for (int c = 0; c <= D; ++c) ph[c] += Phi[c * G + idx] * w[0][i0];
-      }
-    } else if constexpr (D == 2) {
-#pragma unroll
-      for (int i1 = 0; i1 < 4; ++i1)
-#pragma unroll
-        for (int i0 = 0; i0 < 4; ++i0) {
-          float ww = w[1][i1] * w[0][i0];
-          int idx = (f[1] + i1) * ng + f[0] + i0;
-#pragma unroll
-          for (int c = 0; c <= D; ++c) ph[c] += Phi[c * G + idx] * ww;
-        }
-    } else {
-#pragma unroll
-      for (int i2 = 0; i2 < 4; ++i2)
-#pragma unroll
-        for (int i1 = 0; i1 < 4; ++i1) {
-          float w21 = w[2][i2] * w[1][i1];
-#pragma unroll
-          for (int i0 = 0; i0 < 4; ++i0) {
-            float ww = w21 * w[0][i0];
-            int idx = ((f[2] + i2) * ng + f[1] + i1) * ng + f[0] + i0;
-#pragma unroll
-            for (int c = 0; c <= D; ++c) ph[c] += Phi[c * G + idx] * ww;
-          }
-        }
-    }
+      for (int c = 0; c < C; ++c) ph[c] += Phi[c * G + idx] * ww;
+    });
     float fr[D];
     double ysq = 0.0, ydot = 0.0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float yt = y[a] - g.center[a];
+      const float yt = y[a] - g.center[a];
       fr[a] = yt * ph[0] - ph[a + 1];
       ysq += (double)yt * (double)yt;
       ydot += (double)yt * (double)ph[a + 1];
     }
     store_pt<D>(Frep, i, fr);
-    neu_add(z, (1.0 + 2.0 * ysq) * (double)ph[0] - 2.0 * ydot);
+    if constexpr (ZDIRECT) neu_add(z, (double)ph[D + 1]);
+    else neu_add(z, (1.0 + 2.0 * ysq) * (double)ph[0] - 2.0 * ydot);
   }
   z = wave_neu(z);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -297,23 +258,14 @@ __global__ __launch_bounds__(kBlock) void g2s_force_kernel(GridParams g, int n,
   if (threadIdx.x == 0) {
     Neu t{lds[0], lds[1]};
     for (int k = 1; k < (int)(blockDim.x >> 6); ++k) neu_merge(t, Neu{lds[2 * k], lds[2 * k + 1]});
-    store_partial(&rw.part[blockIdx.x * 8 + 0], t.s);
-    store_partial(&rw.part[blockIdx.x * 8 + 1], t.c);
-  }
-  if (take_last_ticket(rw.ticket)) {
-    if (threadIdx.x == 0) {
-      Neu t{0.0, 0.0};
-      for (int b = 0; b < (int)gridDim.x; ++b)
-        neu_merge(t, Neu{load_partial(&rw.part[b * 8]), load_partial(&rw.part[b * 8 + 1])});
-      sc->Z = (t.s + t.c) - (double)n;
-    }
-    reset_ticket(rw.ticket);
+    part[(size_t)blockIdx.x * 8 + 0] = t.s;
+    part[(size_t)blockIdx.x * 8 + 1] = t.c;
   }
 }
 
 // ---- fused gradient + gains + momentum update ----------------------------------
 // dy = alpha Fattr - Frep / Z; gains, uy, y exactly as CU/gradient_descend.cu:29-44;
-// partial sums of the new Y feed the mean of the next centring.
+// per-block sums of the new Y feed the mean of the next centring.
 __device__ __forceinline__ float sgnf(float x) { return x == 0.0f ? 0.0f : (x < 0.0f ? -1.0f : 1.0f); }
 
 template <int D>
@@ -323,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int n, float *__restrict
                                                         const float *__restrict__ Fattr,
                                                         const float *__restrict__ Frep,
                                                         float alpha, float momentum, float eta,
-                                                        RedWork rw, DevScalars *sc) {
+                                                        const DevScalars *sc, double *part) {
   __shared__ double lds[D * 4];
   const float zinv = (float)(1.0 / sc->Z);
   double s[D];
@@ -338,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int n, float *__restrict
     load_pt<D>(Frep, i, fr);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      float dy = alpha * fa[a] - fr[a] * zinv;
+      const float dy = alpha * fa[a] - fr[a] * zinv;
       float gg = (sgnf(dy) != sgnf(u[a])) ? (gn[a] + 0.2f) : (gn[a] * 0.8f);
       if (gg < 0.01f) gg = 0.01f;
       gn[a] = gg;
@@ -350,28 +302,23 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int n, float *__restrict
     store_pt<D>(uY, i, u);
     store_pt<D>(gains, i, gn);
   }
-  block_reduce_sum<D>(s, lds);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < D; ++a) s[a] = wave_sum(s[a]);
+  if (lane == 0)
+#pragma unroll
+    for (int a = 0; a < D; ++a) lds[a * 4 + wid] = s[a];
+  __syncthreads();
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int a = 0; a < D; ++a) store_partial(&rw.part[blockIdx.x * 8 + a], s[a]);
-  if (take_last_ticket(rw.ticket)) {
-    if (threadIdx.x == 0) {
-      for (int a = 0; a < D; ++a) {
-        double t = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) t += load_partial(&rw.part[b * 8 + a]);
-        sc->sum[a] = t;
-        sc->mean[a] = t / (double)n;
-      }
-    }
-    reset_ticket(rw.ticket);
-  }
+    for (int a = 0; a < D; ++a)
+      part[(size_t)blockIdx.x * 8 + a] = lds[a * 4] + lds[a * 4 + 1] + lds[a * 4 + 2] + lds[a * 4 + 3];
 }
 
 // ---- exact O(N^2) repulsion (validation, CU/compute_error.cu:8-53) -------------
 template <int D>
 __global__ __launch_bounds__(kBlock) void rep_exact_kernel(int n, const float *__restrict__ Y,
-                                                           float *__restrict__ Frep, RedWork rw,
-                                                           DevScalars *sc) {
+                                                           float *__restrict__ Frep, double *part) {
   __shared__ float tile[kBlock][D];
   __shared__ double lds[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void rep_exact_kernel(int n, const float *_
   for (int a = 0; a < D; ++a) f[a] = 0.0;
   double z = 0.0;
   for (int j0 = 0; j0 < n; j0 += kBlock) {
-    int j = j0 + threadIdx.x;
+    const int j = j0 + threadIdx.x;
     float yj[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) yj[a] = 0.0f;
@@ -401,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void rep_exact_kernel(int n, const float *_
         df[a] = (double)yi[a] - (double)tile[t][a];
         dist += df[a] * df[a];
       }
-      double q = 1.0 / (1.0 + dist);
+      const double q = 1.0 / (1.0 + dist);
 #pragma unroll
       for (int a = 0; a < D; ++a) f[a] += df[a] * q * q;
       z += q;
@@ -416,22 +363,92 @@ __global__ __launch_bounds__(kBlock) void rep_exact_kernel(int n, const float *_
   } else {
     z = 0.0;
   }
-  double zz[1] = {z};
-  block_reduce_sum<1>(zz, lds);
-  if (threadIdx.x == 0) store_partial(&rw.part[blockIdx.x * 8], zz[0]);
-  if (take_last_ticket(rw.ticket)) {
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int b = 0; b < (int)gridDim.x; ++b) t += load_partial(&rw.part[b * 8]);
-      sc->zexact = t;
+  z = wave_sum(z);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = z;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// ---- one-block finalisation of the block partials (fixed order) ---------------
+__global__ __launch_bounds__(kBlock) void finalize_kernel(int mode, int nblocks, int d, int n,
+                                                          int apply_mean, const double *__restrict__ part,
+                                                          DevScalars *sc) {
+  __shared__ double red[8][kBlock];
+  __shared__ double redc[kBlock];
+  const int t = threadIdx.x;
+  double acc[8];
+  Neu zn{0.0, 0.0};
+  for (int k = 0; k < 8; ++k) acc[k] = (mode == FIN_MINMAX) ? (k < 4 ? INFINITY : -INFINITY) : 0.0;
+  if (mode == FIN_MINMAX) acc[3] = 0.0;
+  for (int b = t; b < nblocks; b += kBlock) {
+    const double *pb = part + (size_t)b * 8;
+    if (mode == FIN_MINMAX) {
+      for (int a = 0; a < d; ++a) {
+        acc[a] = fmin(acc[a], pb[a]);
+        acc[4 + a] = fmax(acc[4 + a], pb[4 + a]);
+      }
+      acc[3] += pb[3];
+    } else if (mode == FIN_ZNEU) {
+      neu_merge(zn, Neu{pb[0], pb[1]});
+    } else {
+      for (int a = 0; a < 4; ++a) acc[a] += pb[a];
     }
-    reset_ticket(rw.ticket);
+  }
+  if (mode == FIN_ZNEU) { acc[0] = zn.s; acc[1] = zn.c; }
+  for (int k = 0; k < 8; ++k) red[k][t] = acc[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      if (mode == FIN_MINMAX) {
+        for (int a = 0; a < 3; ++a) {
+          red[a][t] = fmin(red[a][t], red[a][t + s]);
+          red[4 + a][t] = fmax(red[4 + a][t], red[4 + a][t + s]);
+        }
+        red[3][t] += red[3][t + s];
+      } else if (mode == FIN_ZNEU) {
+        Neu x{red[0][t], red[1][t]};
+        neu_merge(x, Neu{red[0][t + s], red[1][t + s]});
+        red[0][t] = x.s;
+        red[1][t] = x.c;
+      } else {
+        for (int a = 0; a < 4; ++a) red[a][t] += red[a][t + s];
+      }
+    }
+    __syncthreads();
+  }
+  (void)redc;
+  if (t == 0) {
+    if (mode == FIN_MINMAX) {
+      const bool bad = red[3][0] > 0.0;
+      for (int a = 0; a < d; ++a) {
+        sc->minv[a] = bad ? NAN : red[a][0];  // a non-finite coordinate poisons the extent
+        sc->maxv[a] = red[4 + a][0];
+        if (apply_mean) sc->mean[a] = 0.0;
+      }
+    } else if (mode == FIN_ZNEU) {
+      sc->Z = (red[0][0] + red[1][0]) - (double)n;
+    } else if (mode == FIN_MEAN) {
+      for (int a = 0; a < d; ++a) {
+        sc->sum[a] = red[a][0];
+        sc->mean[a] = red[a][0] / (double)n;
+      }
+    } else if (mode == FIN_ZEXACT) {
+      sc->zexact = red[0][0];
+    } else if (mode == FIN_KL) {
+      sc->kl = red[0][0];
+    }
   }
 }
 
 int grid_blocks(int n, int max_blocks) {
   int b = div_up(n > 0 ? n : 1, kBlock);
   return b < max_blocks ? b : max_blocks;
+}
+
+int finalize(hipStream_t s, int mode, int nblocks, int d, int n, int apply_mean, RedWork rw, DevScalars *sc) {
+  finalize_kernel<<<1, kBlock, 0, s>>>(mode, nblocks, d, n, apply_mean, rw.part, sc);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
 }
 
 }  // namespace
@@ -446,68 +463,103 @@ int launch_fill(hipStream_t s, float *p, int64_t count, float v) {
 
 int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, RedWork rw,
                          DevScalars *sc) {
-  int blocks = grid_blocks(n, std::min(rw.max_blocks, 1024));
+  const int blocks = grid_blocks(n, std::min(rw.max_blocks, 1024));
   switch (d) {
-    case 1: center_minmax_kernel<1><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw, sc); break;
-    case 2: center_minmax_kernel<2><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw, sc); break;
-    case 3: center_minmax_kernel<3><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw, sc); break;
+    case 1: center_minmax_kernel<1><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw.part, sc); break;
+    case 2: center_minmax_kernel<2><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw.part, sc); break;
+    case 3: center_minmax_kernel<3><<<blocks, kBlock, 0, s>>>(Y, n, apply_mean, rw.part, sc); break;
     default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
   }
   SGT_LAUNCH_CHECK();
-  return SGTSNE_OK;
+  return finalize(s, FIN_MINMAX, blocks, d, n, apply_mean, rw, sc);
 }
 
-int launch_s2g_atomic(hipStream_t s, const GridParams &g, int n, const float *Y, float *V) {
-  int blocks = div_up(n > 0 ? n : 1, kBlock);
-  switch (g.d) {
-    case 1: s2g_atomic_kernel<1><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
-    case 2: s2g_atomic_kernel<2><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
-    case 3: s2g_atomic_kernel<3><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
-    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+int s2g_lds_fits(int d, int ng, size_t *lds_bytes) {
+  const size_t G = d == 1 ? (size_t)ng : d == 2 ? (size_t)ng * ng : (size_t)ng * ng * ng;
+  *lds_bytes = G * (size_t)(d + 1) * sizeof(float);
+  return *lds_bytes <= 65536;
+}
+
+int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V) {
+  size_t lds = 0;
+  if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds)) {
+    // blocks: enough to fill the GPU, few enough that the flush of the private
+    // grids stays well below the point-wise atomic count
+    const int64_t cells = (int64_t)lds / sizeof(float);
+    const int64_t stencil = g.d == 1 ? 4 : g.d == 2 ? 16 : 64;
+    int64_t b = std::max<int64_t>(1, ((int64_t)n * stencil) / (cells * 4));
+    b = std::min<int64_t>(b, std::max<int64_t>(1, div_up(n, kBlock)));
+    b = std::min<int64_t>(b, 2048);
+    const int chunk = div_up(n, b);
+    const int blocks = div_up(n, chunk);
+    switch (g.d) {
+      case 1: s2g_lds_kernel<1><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
+      case 2: s2g_lds_kernel<2><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
+      case 3: s2g_lds_kernel<3><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
+      default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+    }
+  } else {
+    const int blocks = div_up(n > 0 ? n : 1, kBlock);
+    switch (g.d) {
+      case 1: s2g_atomic_kernel<1><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
+      case 2: s2g_atomic_kernel<2><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
+      case 3: s2g_atomic_kernel<3><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
+      default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
+    }
   }
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
 
 int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
-                     float *Frep, RedWork rw, DevScalars *sc) {
-  int blocks = grid_blocks(n, rw.max_blocks);
+                     float *Frep, int zdirect, RedWork rw, DevScalars *sc) {
+  const int blocks = grid_blocks(n, std::min(rw.max_blocks, 2048));
+#define G2S_CASE(DD)                                                                               \
+  case DD:                                                                                         \
+    if (zdirect) g2s_force_kernel<DD, true><<<blocks, kBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part); \
+    else g2s_force_kernel<DD, false><<<blocks, kBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part);        \
+    break;
   switch (g.d) {
-    case 1: g2s_force_kernel<1><<<blocks, kBlock, 0, s>>>(g, n, Y, Phi, Frep, rw, sc); break;
-    case 2: g2s_force_kernel<2><<<blocks, kBlock, 0, s>>>(g, n, Y, Phi, Frep, rw, sc); break;
-    case 3: g2s_force_kernel<3><<<blocks, kBlock, 0, s>>>(g, n, Y, Phi, Frep, rw, sc); break;
+    G2S_CASE(1)
+    G2S_CASE(2)
+    G2S_CASE(3)
     default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
   }
+#undef G2S_CASE
   SGT_LAUNCH_CHECK();
-  return SGTSNE_OK;
+  return finalize(s, FIN_ZNEU, blocks, g.d, n, 0, rw, sc);
+}
+
+int launch_finalize_kl(hipStream_t s, int nblocks, RedWork rw, DevScalars *sc) {
+  return finalize(s, FIN_KL, nblocks, 1, 0, 0, rw, sc);
 }
 
 int launch_update(hipStream_t s, int d, int n, float *Y, float *uY, float *gains,
                   const float *Fattr, const float *Frep, float alpha, float momentum, float eta,
                   RedWork rw, DevScalars *sc) {
-  int blocks = grid_blocks(n, rw.max_blocks);
+  const int blocks = grid_blocks(n, std::min(rw.max_blocks, 2048));
   switch (d) {
-    case 1: update_kernel<1><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, rw, sc); break;
-    case 2: update_kernel<2><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, rw, sc); break;
-    case 3: update_kernel<3><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, rw, sc); break;
+    case 1: update_kernel<1><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, sc, rw.part); break;
+    case 2: update_kernel<2><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, sc, rw.part); break;
+    case 3: update_kernel<3><<<blocks, kBlock, 0, s>>>(n, Y, uY, gains, Fattr, Frep, alpha, momentum, eta, sc, rw.part); break;
     default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
   }
   SGT_LAUNCH_CHECK();
-  return SGTSNE_OK;
+  return finalize(s, FIN_MEAN, blocks, d, n, 0, rw, sc);
 }
 
 int launch_rep_exact(hipStream_t s, int d, int n, const float *Y, float *Frep, RedWork rw,
                      DevScalars *sc) {
-  int blocks = div_up(n > 0 ? n : 1, kBlock);
+  const int blocks = div_up(n > 0 ? n : 1, kBlock);
   SGT_CHECK(blocks <= rw.max_blocks, SGTSNE_ERR_ARG, "exact repulsion: n too large for workspace");
   switch (d) {
-    case 1: rep_exact_kernel<1><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw, sc); break;
-    case 2: rep_exact_kernel<2><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw, sc); break;
-    case 3: rep_exact_kernel<3><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw, sc); break;
+    case 1: rep_exact_kernel<1><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw.part); break;
+    case 2: rep_exact_kernel<2><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw.part); break;
+    case 3: rep_exact_kernel<3><<<blocks, kBlock, 0, s>>>(n, Y, Frep, rw.part); break;
     default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
   }
   SGT_LAUNCH_CHECK();
-  return SGTSNE_OK;
+  return finalize(s, FIN_ZEXACT, blocks, d, n, 0, rw, sc);
 }
 
 }  // namespace sgt

@@ -333,6 +333,32 @@ __global__ void build_tail_kernel(int n, int width, const int32_t *__restrict__ 
   }
 }
 
+// ---- symmetric permutation of P: new row r = old row perm[r], columns
+// relabelled by iperm (locality order; the values are untouched) -----------
+__global__ void perm_len_kernel(int n, const int32_t *__restrict__ perm, const int32_t *__restrict__ rp,
+                                int32_t *__restrict__ len) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int o = perm[r];
+  len[r] = rp[o + 1] - rp[o];
+}
+
+__global__ void perm_fill_kernel(int n, const int32_t *__restrict__ perm, const int32_t *__restrict__ iperm,
+                                 const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                 const float *__restrict__ val, const int32_t *__restrict__ rp2,
+                                 int32_t *__restrict__ col2, float *__restrict__ val2) {
+  // one wavefront per new row
+  const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  const int o = perm[r];
+  const int s0 = rp[o], len = rp[o + 1] - s0, d0 = rp2[r];
+  for (int e = lane; e < len; e += 64) {
+    col2[d0 + e] = iperm[col[s0 + e]];
+    val2[d0 + e] = val[s0 + e];
+  }
+}
+
 __global__ void to_double_kernel(const float *__restrict__ in, double *__restrict__ out, int64_t m) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) out[t] = (double)in[t];
@@ -449,6 +475,22 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, c
   *col_out = cc;
   *val_out = vv;
   *nnz_out = runs;
+  return SGTSNE_OK;
+}
+
+int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
+                       const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
+                       int32_t *col2, float *val2) {
+  int32_t *len = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&len, sizeof(int32_t) * ((size_t)n + 1), s));
+  perm_len_kernel<<<div_up(n, 256), 256, 0, s>>>(n, perm, rp, len);
+  SGT_LAUNCH_CHECK();
+  SGT_HIP(hipMemsetAsync(len + n, 0, sizeof(int32_t), s));
+  SGT_TRY(device_exclusive_scan(s, len, rp2, n + 1));
+  perm_fill_kernel<<<div_up((int64_t)n * 64, 256), 256, 0, s>>>(n, perm, iperm, rp, col, val, rp2, col2, val2);
+  SGT_LAUNCH_CHECK();
+  SGT_HIP(hipFreeAsync(len, s));
+  (void)nnz;
   return SGTSNE_OK;
 }
 

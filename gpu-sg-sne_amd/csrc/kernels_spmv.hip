@@ -134,23 +134,14 @@ __global__ __launch_bounds__(kBlock) void attr_ell_kernel(int n, int width, cons
 }
 
 // KL = sum_nnz p log((p + FLT_MIN) / (q + FLT_MIN)), p = alpha P_ij,
-// q = 1 / (Z (1 + d_ij^2)); FP64 accumulation, deterministic block partials.
-__device__ __forceinline__ void store_partial(double *slot, double v) {
-  __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_partial(const double *slot) {
-  return __hip_atomic_load(const_cast<double *>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// q = 1 / (Z (1 + d_ij^2)); FP64 accumulation into per-block partials.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kl_kernel(int n, const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ col,
                                                     const float *__restrict__ val,
                                                     const float *__restrict__ Y, double alpha,
-                                                    const DevScalars *sc, RedWork rw,
-                                                    DevScalars *out) {
+                                                    const DevScalars *sc, double *part) {
   __shared__ double lds[4];
-  __shared__ int am_last;
   const double Z = sc->Z;
   double c = 0.0;
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
@@ -162,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void kl_kernel(int n, const int32_t *__rest
       double dist = 0.0;
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        double df = (double)yi[a] - (double)yj[a];
+        const double df = (double)yi[a] - (double)yj[a];
         dist += df * df;
       }
       const double p = (double)val[t] * alpha;
@@ -173,25 +164,10 @@ __global__ __launch_bounds__(kBlock) void kl_kernel(int n, const int32_t *__rest
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
   __syncthreads();
-  if (threadIdx.x == 0) store_partial(&rw.part[blockIdx.x * 8], lds[0] + lds[1] + lds[2] + lds[3]);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = __hip_atomic_fetch_add(rw.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last && threadIdx.x == 0) {
-    __threadfence();
-    double s = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) s += load_partial(&rw.part[b * 8]);
-    out->kl = s;
-    __hip_atomic_store(rw.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8] = lds[0] + lds[1] + lds[2] + lds[3];
 }
 
-template <template <int> class K>
-struct Dispatch;
+
 
 }  // namespace
 
@@ -267,13 +243,13 @@ int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t 
               DevScalars *out) {
   int blocks = std::min(div_up(n > 0 ? n : 1, kBlock), std::min(rw.max_blocks, 2048));
   switch (d) {
-    case 1: kl_kernel<1><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
-    case 2: kl_kernel<2><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
-    case 3: kl_kernel<3><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw, out); break;
+    case 1: kl_kernel<1><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw.part); break;
+    case 2: kl_kernel<2><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw.part); break;
+    case 3: kl_kernel<3><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, alpha, sc, rw.part); break;
     default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
   }
   SGT_LAUNCH_CHECK();
-  return SGTSNE_OK;
+  return launch_finalize_kl(s, blocks, rw, out);
 }
 
 }  // namespace sgt

@@ -25,6 +25,8 @@ GRID_H = 0
 GRID_FIXED = 1
 LAYOUT_SOA = 0
 LAYOUT_AOS = 1
+ZETA_DIRECT = 0
+ZETA_ALGEBRAIC = 1
 
 EXPORTED = [
     "sgtsne_params_default", "sgtsne_last_error", "sgtsne_version", "sgtsne_knn_exact",
@@ -46,7 +48,7 @@ class tsneparams(C.Structure):
         ("earlyIter", C.c_int), ("n", C.c_int), ("h", C.c_double), ("dropLeaf", C.c_int),
         ("np", C.c_int), ("format", C.c_int), ("bs", C.c_int), ("ComputeError", C.c_int),
         ("ng", C.c_int), ("grid_mode", C.c_int), ("layout", C.c_int), ("seed", C.c_uint64),
-        ("ell_width", C.c_int),
+        ("ell_width", C.c_int), ("zeta_mode", C.c_int), ("reorder", C.c_int),
     ]
 
     def __init__(self, **kw):
@@ -82,7 +84,7 @@ def lib():
         "sgtsne_version": (C.c_char_p, []),
         "sgtsne_knn_exact": (C.c_int, [f32p, C.c_int, C.c_int, C.c_int, i32p, f64p]),
         "sgtsne_perplexity_equalize": (C.c_int, [i32p, f64p, C.c_int, C.c_int, C.c_double, i32p, i32p, f32p]),
-        "sgtsne_grid_conv": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, f32p, f32p]),
+        "sgtsne_grid_conv": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, f32p, f32p]),
         "sgtsne_run": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, f32p, P, vp, f32p, vp]),
         "sgtsne_ctx_create": (C.c_int, [C.POINTER(vp), P]),
         "sgtsne_ctx_upload_P": (C.c_int, [vp, C.c_int, C.c_int64, i32p, i32p, f32p, C.c_int]),
@@ -142,12 +144,15 @@ def knn_exact(X, k):
     return I, D
 
 
-def grid_conv(V, h):
-    """GPU non-periodic convolution of V [m][ng]^d by (1 + h^2 r^2)^-2."""
+def grid_conv(V, h, zdirect=False):
+    """GPU non-periodic convolution of V [d+1][ng]^d by (1 + h^2 r^2)^-2; with
+    zdirect an extra last channel holds V[0] convolved by (1 + h^2 r^2)^-1."""
     V = np.ascontiguousarray(V, dtype=np.float32)
     m, ng, d = V.shape[0], V.shape[1], V.ndim - 1
-    Phi = np.empty_like(V)
-    _check(lib().sgtsne_grid_conv(d, ng, m, float(h), V.reshape(-1), Phi.reshape(-1)))
+    if m != d + 1:
+        raise ValueError("grid_conv needs d+1 channels")
+    Phi = np.empty((m + (1 if zdirect else 0),) + V.shape[1:], np.float32)
+    _check(lib().sgtsne_grid_conv(d, ng, float(h), int(bool(zdirect)), V.reshape(-1), Phi.reshape(-1)))
     return Phi
 
 

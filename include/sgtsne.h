@@ -44,6 +44,13 @@ extern "C" {
 #define SGTSNE_GRID_H 0
 #define SGTSNE_GRID_FIXED 1
 
+/* normalisation term Z: DIRECT = sum over points of the grid convolution of
+ * the unit charges with 1/(1+r^2), minus N (no cancellation in FP32);
+ * ALGEBRAIC = sum_i (1+2|y_i|^2) Phi0_i - 2 y_i . Phi_i - N from the squared
+ * kernel alone, the reference's formula (HY/qq.cpp:75-106, CU/Frep.cu:43-80). */
+#define SGTSNE_ZETA_DIRECT 0
+#define SGTSNE_ZETA_ALGEBRAIC 1
+
 /* Y layouts: SoA [d][N] (CU path) or AoS [N][d] (HYB path) */
 #define SGTSNE_LAYOUT_SOA 0
 #define SGTSNE_LAYOUT_AOS 1
@@ -70,6 +77,10 @@ typedef struct sgtsne_params {
   int layout;         /* SGTSNE_LAYOUT_* of y_in / y_out        (SOA)         */
   uint64_t seed;      /* seed of the N(0, 1e-4^2) start when y_in is NULL     */
   int ell_width;      /* hybrid ELL width; <= 0 = automatic                   */
+  int zeta_mode;      /* SGTSNE_ZETA_*                          (DIRECT)      */
+  int reorder;        /* 1: keep P and Y on the device in a breadth-first
+                         order of P (locality of the neighbour gathers); the
+                         API still reads and writes the caller's order  (1)   */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of
@@ -100,11 +111,13 @@ int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *
 int sgtsne_perplexity_equalize(const int32_t *I, const double *D, int n, int k, double u,
                                int32_t *rowptr, int32_t *col, float *val);
 
-/* Non-periodic convolution of an ng^d grid with m <= 4 channels by the
- * kernel (1 + h^2 r^2)^-2, r in grid steps, on the GPU: the G2G step of the
+/* Non-periodic convolution of an ng^d grid with d+1 channels by the kernel
+ * (1 + h^2 r^2)^-2, r in grid steps, on the GPU: the G2G step of the
  * repulsion (HY/non_periodic_conv.cpp:26-708, CU/non_periodic_convF.cu:162-517).
- * V and Phi are host arrays [m][ng^d], x fastest; 4 <= ng <= 512. */
-int sgtsne_grid_conv(int d, int ng, int m, double h, const float *V, float *Phi);
+ * V is a host array [d+1][ng^d], x fastest; Phi receives [d+1][ng^d] and,
+ * when zdirect != 0, a channel d+1 with channel 0 convolved by
+ * (1 + h^2 r^2)^-1.  4 <= ng <= 512. */
+int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float *Phi);
 
 /* ---- drop-in entry -------------------------------------------------- */
 

@@ -47,6 +47,11 @@ def lib():
         "or_s2g": (None, [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p]),
         "or_g2s": (None, [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p]),
         "or_conv_nopad_fft": (None, [C.c_int, C.c_int, C.c_int, C.c_double, _f64p, _f64p]),
+        "or_conv_nopad_fft_k": (None, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _f64p, _f64p]),
+        "or_repulsive_interp_z": (C.c_double, [C.c_int, C.c_int, _f64p, C.c_double, C.c_int, _f64p,
+                                               C.POINTER(C.c_int)]),
+        "or_run_z": (None, [C.c_int, C.c_int, _i64p, _i32p, _f64p, C.c_double, C.c_int, C.c_double, C.c_int,
+                            C.c_int, C.c_int, _f64p, _f64p, _f64p, C.c_void_p, C.c_void_p]),
         "or_conv_nopad_direct": (None, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _f64p, _f64p]),
         "or_fft": (None, [C.c_int, _f64p, C.c_int]),
         "or_repulsive_interp": (C.c_double, [C.c_int, C.c_int, _f64p, C.c_double, _f64p, C.POINTER(C.c_int)]),
@@ -201,12 +206,12 @@ def g2s(y, V):
     return Phi
 
 
-def conv_nopad_fft(V, h):
+def conv_nopad_fft(V, h, kpow=2):
     V = _f64(V)
     nvec, ng = V.shape[0], V.shape[1]
     d = V.ndim - 1
     Phi = np.empty_like(V)
-    lib().or_conv_nopad_fft(d, ng, nvec, float(h), V, Phi)
+    lib().or_conv_nopad_fft_k(d, ng, nvec, float(h), int(kpow), V, Phi)
     return Phi
 
 
@@ -226,14 +231,18 @@ def fft(x, sign=-1):
     return buf.view(np.complex128)
 
 
-def repulsive_interp(Y, h):
+ZETA_DIRECT = 1      # oracle zmode numbering: 0 = algebraic (reference), 1 = direct
+ZETA_ALGEBRAIC = 0
+
+
+def repulsive_interp(Y, h, zmode=ZETA_ALGEBRAIC):
     """Returns (Frep, Z, ng).  Y is shifted to min 0 in place (reference
     semantics); pass a copy to keep the original."""
     assert Y.dtype == np.float64 and Y.flags.c_contiguous
     n, d = Y.shape
     F = np.zeros_like(Y)
     ng = C.c_int()
-    Z = lib().or_repulsive_interp(n, d, Y, float(h), F, C.byref(ng))
+    Z = lib().or_repulsive_interp_z(n, d, Y, float(h), int(zmode), F, C.byref(ng))
     return F, Z, ng.value
 
 
@@ -248,7 +257,7 @@ def kl(P, Y, Z, alpha=1.0):
     return lib().or_kl(n, d, P.rowptr, P.col, P.val, Y, float(Z), float(alpha))
 
 
-def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None):
+def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None, zmode=ZETA_ALGEBRAIC):
     """Runs iterations [it0, it1) in place; returns (Z per iteration, ng per iteration)."""
     n, d = Y.shape
     if h is None or h <= 0:
@@ -256,6 +265,6 @@ def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None):
     cnt = it1 - it0
     Z = np.empty(max(cnt, 1))
     ng = np.empty(max(cnt, 1), np.int32)
-    lib().or_run(n, d, P.rowptr, P.col, P.val, float(alpha0), int(early_iter), float(h),
-                 int(it0), int(it1), Y, uY, gains, Z.ctypes.data, ng.ctypes.data)
+    lib().or_run_z(n, d, P.rowptr, P.col, P.val, float(alpha0), int(early_iter), float(h), int(zmode),
+                   int(it0), int(it1), Y, uY, gains, Z.ctypes.data, ng.ctypes.data)
     return Z[:cnt], ng[:cnt]

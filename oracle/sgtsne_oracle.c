@@ -550,15 +550,16 @@ static void fft_nd(int d, int ng, int howmany, cpx *A, int sign) {
   free(tw);
 }
 
-/* kernel{1,2,3}d: HY/non_periodic_conv.cpp:10-20 */
-static inline double kern(double hsq, double r2) {
-  double t = 1.0 + hsq * r2;
-  return 1.0 / (t * t);
+/* kernel{1,2,3}d: HY/non_periodic_conv.cpp:10-20 (kpow = 2); kpow = 1 is
+ * the plain Cauchy kernel 1/(1 + h^2 r^2) used for the direct Z. */
+static inline double kern(double hsq, double r2, int kpow) {
+  double t = 1.0 / (1.0 + hsq * r2);
+  return kpow == 2 ? t * t : t;
 }
 
 /* HY/non_periodic_conv.cpp:26-708: sum over the 2^d even/odd terms of
  * circulant (+) and skew-circulant (-, twiddled by exp(-i pi k / n)) parts. */
-void or_conv_nopad_fft(int d, int ng, int nVec, double h, const double *V, double *Phi) {
+void or_conv_nopad_fft_k(int d, int ng, int nVec, double h, int kpow, const double *V, double *Phi) {
   const double hsq = h * h;
   const size_t G = (size_t)(d == 1 ? ng : d == 2 ? ng * ng : ng * ng * ng);
   cpx *K = (cpx *)malloc(sizeof(cpx) * G);
@@ -588,7 +589,7 @@ void or_conv_nopad_fft(int d, int ng, int nVec, double h, const double *V, doubl
           }
           r2 += (double)c * c;
         }
-        if (ok) acc += sgn * kern(hsq, r2);
+        if (ok) acc += sgn * kern(hsq, r2, kpow);
       }
       cpx t = {acc, 0.0};
       for (int a = 0; a < d; a++)
@@ -646,6 +647,10 @@ void or_conv_nopad_fft(int d, int ng, int nVec, double h, const double *V, doubl
   free(K); free(X); free(w);
 }
 
+void or_conv_nopad_fft(int d, int ng, int nVec, double h, const double *V, double *Phi) {
+  or_conv_nopad_fft_k(d, ng, nVec, h, 2, V, Phi);
+}
+
 void or_conv_nopad_direct(int d, int ng, int nVec, double h, int kpow, const double *V, double *Phi) {
   const double hsq = h * h;
   const size_t G = (size_t)(d == 1 ? ng : d == 2 ? ng * ng : ng * ng * ng);
@@ -668,6 +673,10 @@ void or_conv_nopad_direct(int d, int ng, int nVec, double h, int kpow, const dou
 /* ------------------------------------------------------------------------ */
 /* Interpolated repulsion: HY/qq.cpp:132-244 + HY/nuconv.cpp:22-153.        */
 double or_repulsive_interp(int n, int d, double *y, double h, double *Frep, int *ng_out) {
+  return or_repulsive_interp_z(n, d, y, h, 0, Frep, ng_out);
+}
+
+double or_repulsive_interp_z(int n, int d, double *y, double h, int zmode, double *Frep, int *ng_out) {
   const int m = d + 1;
   double miny[3] = {INFINITY, INFINITY, INFINITY};
   for (int i = 0; i < n; i++)
@@ -710,19 +719,31 @@ double or_repulsive_interp(int n, int d, double *y, double h, double *Frep, int 
     or_s2g(n, d, ng, m, yt, VScat, VGrid);
     or_conv_nopad_fft(d, ng, m, hh, VGrid, PhiGrid);
     or_g2s(n, d, ng, m, yt, PhiGrid, PhiScat);
-    free(VGrid); free(PhiGrid);
-    /* zetaAndForce: HY/qq.cpp:75-106 */
-    Z = 0.0;
-    for (int i = 0; i < n; i++) {
-      double Ysq = 0.0;
-      for (int a = 0; a < d; a++) {
-        double ya = y[(size_t)i * d + a];
-        Ysq += ya * ya;
-        Z -= 2.0 * (ya * PhiScat[(size_t)i * m + a + 1]);
+    if (zmode == 1) {
+      /* direct Z: the unit charges convolved with 1/(1 + r^2), gathered */
+      double *Psi = (double *)calloc(G, sizeof(double));
+      double *psi = (double *)malloc(sizeof(double) * (size_t)n);
+      or_conv_nopad_fft_k(d, ng, 1, hh, 1, VGrid, Psi);
+      or_g2s(n, d, ng, 1, yt, Psi, psi);
+      Z = 0.0;
+      for (int i = 0; i < n; i++) Z += psi[i];
+      Z = Z - n;
+      free(Psi); free(psi);
+    } else {
+      /* zetaAndForce: HY/qq.cpp:75-106 */
+      Z = 0.0;
+      for (int i = 0; i < n; i++) {
+        double Ysq = 0.0;
+        for (int a = 0; a < d; a++) {
+          double ya = y[(size_t)i * d + a];
+          Ysq += ya * ya;
+          Z -= 2.0 * (ya * PhiScat[(size_t)i * m + a + 1]);
+        }
+        Z += (1.0 + 2.0 * Ysq) * PhiScat[(size_t)i * m];
       }
-      Z += (1.0 + 2.0 * Ysq) * PhiScat[(size_t)i * m];
+      Z = Z - n;
     }
-    Z = Z - n;
+    free(VGrid); free(PhiGrid);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
       for (int a = 0; a < d; a++)
@@ -782,6 +803,12 @@ double or_kl(int n, int d, const int64_t *rowptr, const int32_t *col, const doub
 void or_run(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
             double alpha0, int early_iter, double h, int it0, int it1,
             double *Y, double *uY, double *gains, double *zeta_out, int *ng_out) {
+  or_run_z(n, d, rowptr, col, val, alpha0, early_iter, h, 0, it0, it1, Y, uY, gains, zeta_out, ng_out);
+}
+
+void or_run_z(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
+              double alpha0, int early_iter, double h, int zmode, int it0, int it1,
+              double *Y, double *uY, double *gains, double *zeta_out, int *ng_out) {
   const int mom_switch_iter = 250;
   const double eta = 200.0;
   double *Fattr = (double *)malloc(sizeof(double) * (size_t)n * d);
@@ -790,7 +817,7 @@ void or_run(int n, int d, const int64_t *rowptr, const int32_t *col, const doubl
     double alpha = it <= early_iter ? alpha0 : 1.0;
     double momentum = it <= mom_switch_iter ? 0.5 : 0.8;
     int ng = 0;
-    double Z = or_repulsive_interp(n, d, Y, h, Frep, &ng);
+    double Z = or_repulsive_interp_z(n, d, Y, h, zmode, Frep, &ng);
     or_attractive(n, d, rowptr, col, val, Y, Fattr);
     or_update(n, d, Fattr, Frep, alpha, momentum, eta, Y, uY, gains);
     if (zeta_out) zeta_out[it - it0] = Z;

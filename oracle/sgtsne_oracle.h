@@ -83,6 +83,9 @@ void or_g2s(int n, int d, int ng, int nVec, const double *y, const double *V, do
  * HY/non_periodic_conv.cpp:26-708 (conv{1,2,3}dnopad). */
 void or_conv_nopad_fft(int d, int ng, int nVec, double h, const double *V, double *Phi);
 
+/* Same with kernel power kpow: 2 = (1 + h^2 r^2)^-2, 1 = (1 + h^2 r^2)^-1. */
+void or_conv_nopad_fft_k(int d, int ng, int nVec, double h, int kpow, const double *V, double *Phi);
+
 /* The same convolution as a direct Toeplitz sum (O(ng^{2d})). */
 void or_conv_nopad_direct(int d, int ng, int nVec, double h, int kpow, const double *V, double *Phi);
 
@@ -94,6 +97,12 @@ void or_fft(int n, double *data, int sign);
  * zetaAndForce HY/qq.cpp:75-106.  y is shifted to min 0 in place, exactly as
  * the reference does.  Returns Z; *ng_out receives the grid side nGrid+2. */
 double or_repulsive_interp(int n, int d, double *y, double h, double *Frep, int *ng_out);
+
+/* As above with the choice of Z: zmode 0 = the reference's algebraic Z from
+ * the squared kernel (HY/qq.cpp:75-106); zmode 1 = direct Z, the unit charges
+ * convolved with 1/(1 + r^2) on the same grid and gathered, minus N.  Frep is
+ * the same in both modes (only its normalisation differs). */
+double or_repulsive_interp_z(int n, int d, double *y, double h, int zmode, double *Frep, int *ng_out);
 
 /* Gradient + update with mean-centring: HY/gradient_descendHYB.cpp:29-80. */
 void or_update(int n, int d, const double *Fattr, const double *Frep, double alpha,
@@ -111,6 +120,10 @@ double or_kl(int n, int d, const int64_t *rowptr, const int32_t *col, const doub
 void or_run(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
             double alpha0, int early_iter, double h, int it0, int it1,
             double *Y, double *uY, double *gains, double *zeta_out, int *ng_out);
+
+void or_run_z(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
+              double alpha0, int early_iter, double h, int zmode, int it0, int it1,
+              double *Y, double *uY, double *gains, double *zeta_out, int *ng_out);
 
 /* Threads used by OpenMP regions (0 = leave the runtime default). */
 void or_set_threads(int nthreads);

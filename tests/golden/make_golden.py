@@ -6,7 +6,8 @@ mnist500.npz: the reference's own MATLAB data files converted to numpy
 
 mnist500_oracle.npz: outputs of the FP64 oracle on that input (k=90, u=30):
 kNN ids/distances, the perplexity-equalised P, P_sym, the per-iteration terms
-at Y0 and at a spread Y, and 100 iterations of the loop.  These are
+at Y0 and at a spread Y (with the reference's algebraic Z and with the direct
+Z), and 100 iterations of the loop (direct Z).  These are
 regression vectors for the GPU parity tests; the oracle itself is pinned by
 tests/test_oracle.py.
 
@@ -45,7 +46,10 @@ def oracle_vectors():
                S_rowptr=Ps.rowptr, S_col=Ps.col, S_val=Ps.val)
     for tag, Y in (("y0", Y0.astype(np.float64)), ("ys", Y0.astype(np.float64) * 2e4)):
         Fa = O.attractive(Ps, Y)
-        Fr, Z, ng = O.repulsive_interp(Y.copy(), 0.7)
+        Fr, Z, ng = O.repulsive_interp(Y.copy(), 0.7, O.ZETA_ALGEBRAIC)
+        Frd, Zd, _ = O.repulsive_interp(Y.copy(), 0.7, O.ZETA_DIRECT)
+        out[f"{tag}_Frep_direct"] = Frd
+        out[f"{tag}_Z_direct"] = np.float64(Zd)
         out[f"{tag}_Y"] = Y.astype(np.float32)
         out[f"{tag}_Fattr"] = Fa
         out[f"{tag}_Frep"] = Fr
@@ -55,7 +59,7 @@ def oracle_vectors():
     Y = Y0.astype(np.float64).copy()
     uY = np.zeros_like(Y)
     g = np.ones_like(Y)
-    Zs, ngs = O.run(Ps, Y, uY, g, 0, 100)
+    Zs, ngs = O.run(Ps, Y, uY, g, 0, 100, zmode=O.ZETA_DIRECT)
     out["run100_Y"] = Y
     out["run100_Z"] = Zs
     out["run100_ng"] = ngs

@@ -51,7 +51,7 @@ def test_argument_errors_without_device(sg):
     X = np.zeros((3, 2), np.float32)
     assert lib.sgtsne_knn_exact(X, 3, 2, 5, np.zeros((3, 6), np.int32), np.zeros((3, 6))) == -1
     V = np.zeros(4, np.float32)
-    assert lib.sgtsne_grid_conv(4, 8, 2, 1.0, V, V) == -1
+    assert lib.sgtsne_grid_conv(4, 8, 1.0, 0, V, V) == -1
     p = sg.tsneparams(d=4)
     h = C.c_void_p()
     assert lib.sgtsne_ctx_create(C.byref(h), C.byref(p)) == -1

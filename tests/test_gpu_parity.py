@@ -104,59 +104,89 @@ def test_grid_conv_vs_oracle(sg, O, d, ng):
     rng = np.random.default_rng(ng + d)
     V = rng.uniform(0, 1, size=(d + 1,) + (ng,) * d)
     for h in (0.05, 0.7):
-        Phi = sg.grid_conv(V.astype(np.float32), h)
+        Phi = sg.grid_conv(V.astype(np.float32), h, zdirect=True)
         if ng ** d <= 4096:
             ref = O.conv_nopad_direct(V, h)
+            ref1 = O.conv_nopad_direct(V[:1], h, kpow=1)
         else:
             ref = O.conv_nopad_fft(V, h)
-        assert rel_l2(Phi, ref) < 1e-5
+            ref1 = O.conv_nopad_fft(V[:1], h, kpow=1)
+        assert rel_l2(Phi[:d + 1], ref) < 1e-5
+        assert rel_l2(Phi[d + 1:], ref1) < 1e-5
+        Phi2 = sg.grid_conv(V.astype(np.float32), h, zdirect=False)
+        assert Phi2.shape[0] == d + 1 and rel_l2(Phi2, ref) < 1e-5
+
+
+_ROCFFT_SNIPPET = r"""
+import sys, numpy as np, torch
+f = np.load(sys.argv[1]); V = f["V"]; h = float(f["h"]); d = V.ndim - 1; ng = V.shape[1]
+L = 2 * ng
+idx = torch.arange(L, device="cuda", dtype=torch.float32)
+m = torch.minimum(idx, L - idx)
+r2 = sum(g ** 2 for g in torch.meshgrid(*([m] * d), indexing="ij"))
+K = (1.0 + h * h * r2) ** -2
+Vt = torch.zeros((d + 1,) + (L,) * d, device="cuda")
+Vt[(slice(None),) + (slice(0, ng),) * d] = torch.from_numpy(V).cuda()
+dims = tuple(range(1, d + 1))
+out = torch.fft.ifftn(torch.fft.fftn(Vt, dim=dims) * torch.fft.fftn(K), dim=dims).real
+np.save(sys.argv[2], out[(slice(None),) + (slice(0, ng),) * d].cpu().numpy())
+"""
 
 
 @pytest.mark.parametrize("d,ng", [(1, 200), (2, 64), (2, 90), (3, 20)])
-def test_grid_conv_vs_rocfft(sg, d, ng):
-    # a zero-padded convolution through torch.fft (rocFFT) in FP32
-    import torch
+def test_grid_conv_vs_rocfft(sg, tmp_path, d, ng):
+    # the same zero-padded convolution through torch.fft (rocFFT) in FP32, run
+    # in its own process: torch carries its own HIP runtime
+    import subprocess
+    import sys
 
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU for torch")
     rng = np.random.default_rng(3 * ng + d)
     V = rng.uniform(0, 1, size=(d + 1,) + (ng,) * d).astype(np.float32)
     h = 0.3
-    L = 2 * ng
-    idx = torch.arange(L, device="cuda", dtype=torch.float32)
-    m = torch.minimum(idx, L - idx)
-    r2 = sum(g ** 2 for g in torch.meshgrid(*([m] * d), indexing="ij"))
-    K = (1.0 + h * h * r2) ** -2
-    Vt = torch.zeros((d + 1,) + (L,) * d, device="cuda")
-    Vt[(slice(None),) + (slice(0, ng),) * d] = torch.from_numpy(V).cuda()
-    dims = tuple(range(1, d + 1))
-    out = torch.fft.ifftn(torch.fft.fftn(Vt, dim=dims) * torch.fft.fftn(K), dim=dims).real
-    ref = out[(slice(None),) + (slice(0, ng),) * d].cpu().numpy()
+    np.savez(tmp_path / "in.npz", V=V, h=h)
+    r = subprocess.run([sys.executable, "-c", _ROCFFT_SNIPPET, str(tmp_path / "in.npz"), str(tmp_path / "out.npy")],
+                       capture_output=True, text=True, timeout=280)
+    if r.returncode != 0 and "No module named" in r.stderr:
+        pytest.skip("torch is not importable")
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = np.load(tmp_path / "out.npy")
     Phi = sg.grid_conv(V, h)
     assert rel_l2(Phi, ref) < 1e-5
 
 
 # ---------------------------------------------------------------- per-iteration terms
-def _terms_oracle(O, S, Y, h):
+def _terms_oracle(O, S, Y, h, zmode=None):
+    # the GPU's default Z is the direct convolution with 1/(1 + r^2)
+    zmode = O.ZETA_DIRECT if zmode is None else zmode
     Yd = Y.astype(np.float64)
     Fa = O.attractive(S, Yd)
-    Fr, Z, ng = O.repulsive_interp(Yd.copy(), h)
+    Fr, Z, ng = O.repulsive_interp(Yd.copy(), h, zmode)
     return Fa, Fr, Z, ng
 
 
 @pytest.mark.parametrize("fmt", ["CSR_SEG", "CSR", "COO", "HYBRID"])
-def test_terms_golden(sg, mnist, fmt):
+@pytest.mark.parametrize("zeta", ["DIRECT", "ALGEBRAIC"])
+def test_terms_golden(sg, mnist, fmt, zeta):
     f, g = mnist
     S = sg.CSR(500, g["S_rowptr"], g["S_col"], g["S_val"].astype(np.float32))
-    with sg.Context(sg.tsneparams(format=getattr(sg, "FORMAT_" + fmt))) as ctx:
+    sfx = "_direct" if zeta == "DIRECT" else ""
+    prm = sg.tsneparams(format=getattr(sg, "FORMAT_" + fmt), zeta_mode=getattr(sg, "ZETA_" + zeta))
+    with sg.Context(prm) as ctx:
         ctx.upload_P(S, symmetrize=False)
         for tag in ("y0", "ys"):
             ctx.set_Y(g[f"{tag}_Y"])
             Fa, Fr, Z, ng = ctx.terms()
             assert ng == g[f"{tag}_ng"]
             assert rel_l2(Fa, g[f"{tag}_Fattr"]) < TOL
-            assert rel_l2(Fr, g[f"{tag}_Frep"]) < TOL
-            assert abs(Z - g[f"{tag}_Z"]) / g[f"{tag}_Z"] < TOL
+            assert rel_l2(Fr, g[f"{tag}_Frep{sfx}"]) < TOL
+            assert abs(Z - g[f"{tag}_Z{sfx}"]) / g[f"{tag}_Z{sfx}"] < TOL
+
+
+def _clustered_embedding(n, d, rng, spread=30.0, clusters=12):
+    # what t-SNE produces: dense unit-scale blobs scattered over a wide extent
+    centres = rng.uniform(-spread, spread, size=(clusters, d))
+    lab = rng.integers(0, clusters, n)
+    return (centres[lab] + rng.standard_normal((n, d)) * rng.uniform(0.3, 2.0, clusters)[lab, None]).astype(np.float32)
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
@@ -168,18 +198,44 @@ def test_terms_random(sg, O, d, fmt):
     So = O.symmetrize(O.perplexity_equalize(I, D, 20.0))
     S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
     rng = np.random.default_rng(d)
+    h = {1: 0.5, 2: 0.7, 3: 1.2}[d]
+    cases = [(rng.standard_normal((n, d)) * 1e-3).astype(np.float32),   # the start of a run
+             (rng.standard_normal((n, d)) * 3.0).astype(np.float32),    # early exaggeration
+             _clustered_embedding(n, d, rng)]                           # a formed embedding
     with sg.Context(sg.tsneparams(d=d, format=getattr(sg, "FORMAT_" + fmt))) as ctx:
         ctx.upload_P(S, symmetrize=False)
-        for scale in (1e-3, 3.0, 40.0):
-            Y = (rng.standard_normal((n, d)) * scale).astype(np.float32)
+        for Y in cases:
             ctx.set_Y(Y)
             Yc = ctx.get_Y()  # centred copy: the oracle sees the same Y
             Fa, Fr, Z, ng = ctx.terms()
-            Fao, Fro, Zo, ngo = _terms_oracle(O, So, Yc, {1: 0.5, 2: 0.7, 3: 1.2}[d])
+            Fao, Fro, Zo, ngo = _terms_oracle(O, So, Yc, h)
             assert ng == ngo
             assert rel_l2(Fa, Fao) < TOL
             assert rel_l2(Fr, Fro) < TOL
             assert abs(Z - Zo) / Zo < TOL
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_terms_reference_acceptance_setup(sg, O, d):
+    # the reference's repulsion self-test inputs (HY/test_modules.cpp:122-156):
+    # Y ~ U[0,1]^d, h in {0.05, 0.08, 0.13}; FP32 GPU against the FP64 oracle
+    n = 3000
+    rng = np.random.default_rng(40 + d)
+    Y = rng.uniform(0, 1, size=(n, d)).astype(np.float32)
+    S = sg.CSR(n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    for h in (0.05, 0.08, 0.13):
+        for zeta, zmode in (("DIRECT", O.ZETA_DIRECT), ("ALGEBRAIC", O.ZETA_ALGEBRAIC)):
+            with sg.Context(sg.tsneparams(d=d, h=h, zeta_mode=getattr(sg, "ZETA_" + zeta))) as ctx:
+                ctx.upload_P(S, symmetrize=False)
+                ctx.set_Y(Y)
+                Yc = ctx.get_Y().astype(np.float64)
+                _, Fr, Z, ng = ctx.terms()
+                Fro, Zo, ngo = O.repulsive_interp(Yc.copy(), h, zmode)
+                Fe, Ze = O.repulsive_exact(Yc)
+                assert ng == ngo
+                assert rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL, zeta
+                # and the reference's own acceptance bar against the exact sum
+                assert np.abs(Fr - Fe).max() <= 1e-7 and abs(Z - Ze) / Ze <= 1e-4, zeta
 
 
 def test_terms_ragged_rows(sg, O):
@@ -190,7 +246,7 @@ def test_terms_ragged_rows(sg, O):
     lens[:5] = 0
     lens[7] = 300
     rows = np.repeat(np.arange(n), lens)
-    cols = rng.integers(0, n, rows.size)
+    cols = rng.integers(5, n, rows.size)  # rows 0..4 stay empty after P + P^T
     A = {}
     for r, c in zip(rows, cols):
         if r != c:
@@ -243,21 +299,28 @@ def test_kl_golden(sg, mnist):
 
 # ---------------------------------------------------------------- iterations
 def test_iterations_golden(sg, O, mnist):
-    # 100 iterations from the reference's own start (initial2.mat)
+    # from the reference's own start (initial2.mat): 10 iterations track the
+    # FP64 loop closely; after 100 (trajectories of an exaggerated descent
+    # diverge) the KL agrees within 1%
     f, g = mnist
     S = sg.CSR(500, g["S_rowptr"], g["S_col"], g["S_val"].astype(np.float32))
+    So = O.CSR(500, g["S_rowptr"].astype(np.int64), g["S_col"], g["S_val"])
     with sg.Context(sg.tsneparams()) as ctx:
         ctx.upload_P(S, symmetrize=False)
         ctx.set_Y(f["Y0"])
-        ctx.step(100)
-        Y = ctx.get_Y()
+        Y0c = ctx.get_Y().astype(np.float64)
+        ctx.step(10)
+        Y10 = ctx.get_Y()
+        ctx.step(90)
         kl = ctx.kl(1.0)
+    Yo = Y0c.copy()
+    uY, gn = np.zeros_like(Yo), np.ones_like(Yo)
+    O.run(So, Yo, uY, gn, 0, 10, zmode=O.ZETA_DIRECT)
+    assert rel_l2(Y10, Yo) < 1e-3
     Yo = g["run100_Y"]
-    So = O.CSR(500, g["S_rowptr"].astype(np.int64), g["S_col"], g["S_val"])
-    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7)
+    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7, O.ZETA_DIRECT)
     klo = O.kl(So, Yo, Zo, 1.0)
     assert abs(kl - klo) / klo < 0.01
-    assert rel_l2(Y, Yo) < 0.05
 
 
 def test_full_run_final_kl(sg, O):
@@ -273,9 +336,9 @@ def test_full_run_final_kl(sg, O):
     assert np.all(np.isfinite(Y)) and ti[7 * prm.maxIter] > 0 and ti[0] > 0
     So = O.symmetrize(_oracle_csr(O, P))
     Yo = y0.astype(np.float64).copy()
-    O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, 1000)
-    _, Zg, _ = O.repulsive_interp(Y.astype(np.float64).copy(), 0.7)
-    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7)
+    O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, 1000, zmode=O.ZETA_DIRECT)
+    _, Zg, _ = O.repulsive_interp(Y.astype(np.float64).copy(), 0.7, O.ZETA_DIRECT)
+    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7, O.ZETA_DIRECT)
     klg = O.kl(So, Y.astype(np.float64), Zg)
     klo = O.kl(So, Yo, Zo)
     assert abs(klg - klo) / klo < 0.01, (klg, klo)

@@ -43,6 +43,16 @@ def test_conv_even_odd_equals_toeplitz(O, d, ng):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
 
 
+@pytest.mark.parametrize("d,ng", [(1, 40), (2, 18), (3, 9)])
+def test_conv_cauchy_kernel_even_odd_equals_toeplitz(O, d, ng):
+    # the same construction with the kernel 1/(1 + h^2 r^2) (direct Z)
+    rng = np.random.default_rng(7 * d + ng)
+    V = rng.standard_normal((2,) + (ng,) * d)
+    a = O.conv_nopad_fft(V, 0.3, kpow=1)
+    b = O.conv_nopad_direct(V, 0.3, kpow=1)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
 def test_conv_direct_is_toeplitz_product(O):
     # 1-D: the direct sum is the dense Toeplitz matrix times V
     rng = np.random.default_rng(3)
@@ -137,9 +147,22 @@ def test_interp_repulsion_reference_acceptance(O, d, n):
     Y = rng.uniform(0, 1, size=(n, d))
     Fe, Ze = O.repulsive_exact(Y)
     for h in (0.05, 0.08, 0.13):
-        Fi, Zi, ng = O.repulsive_interp(Y.copy(), h)
-        assert np.abs(Fi - Fe).max() <= 1e-7
-        assert abs(Zi - Ze) / Ze <= 1e-4
+        for zmode in (O.ZETA_ALGEBRAIC, O.ZETA_DIRECT):
+            Fi, Zi, ng = O.repulsive_interp(Y.copy(), h, zmode)
+            assert np.abs(Fi - Fe).max() <= 1e-7
+            assert abs(Zi - Ze) / Ze <= 1e-4
+
+
+def test_direct_z_matches_exact_sum_clustered(O):
+    # the direct Z approximates sum_{i != j} 1/(1 + d^2) on a clustered
+    # embedding as well as the algebraic one does
+    rng = np.random.default_rng(91)
+    c = rng.uniform(-20, 20, size=(8, 2))
+    Y = c[rng.integers(0, 8, 1500)] + rng.standard_normal((1500, 2))
+    _, Ze = O.repulsive_exact(Y)
+    _, Za, _ = O.repulsive_interp(Y.copy(), 0.3, O.ZETA_ALGEBRAIC)
+    _, Zd, _ = O.repulsive_interp(Y.copy(), 0.3, O.ZETA_DIRECT)
+    assert abs(Zd - Ze) / Ze < 1e-3 and abs(Za - Ze) / Ze < 1e-3
 
 
 def test_interp_repulsion_grid_and_shift(O):
