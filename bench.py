@@ -110,6 +110,8 @@ def main():
     t_sym = time.perf_counter() - t0
     info = ctx.info()
     nnz = info["nnz"]
+    tune = ctx.tuning()
+    log(f"[bench] attractive-kernel autotuner: {tune}")
     log(f"[bench] P_sym nnz={nnz} (symmetrise+upload {t_sym:.1f} s); setup {time.perf_counter() - t_setup:.1f} s")
 
     y0 = sg.synth.random_init(n, args.d, seed=1)
@@ -134,7 +136,7 @@ def main():
     log(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iter; phases {phases}")
     Y_end = ctx.get_Y()
     kl = ctx.kl(1.0)
-    grid = ctx.terms()[3]
+    Fa_g, Fr_g, Z_g, grid = ctx.terms()
     log(f"[bench] final KL {kl:.4f}, grid side {grid}")
 
     # roofline of the dominant kernel (attractive SpMV): algorithmic bytes
@@ -144,12 +146,27 @@ def main():
     achieved = bytes_b / t_spmv / 1e9 if t_spmv > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "attr_csr_seg_kernel", "bytes_per_launch": bytes_b,
+                "kernel": (f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
+                           else f"attr_csr_seg_kernel<{args.d},{tune['group']}>"),
+                "bytes_per_launch": bytes_b,
                 "avg_launch_ms": round(phases["attractive"], 5)}
 
     cpu = None
+    term_err = None
     if not args.no_cpu:
+        # FP32 GPU terms against the FP64 oracle at the final embedding
+        from oracle import oracle as O
+
         P_sym = ctx.get_P()
+        So = O.CSR(P_sym.n, P_sym.rowptr.astype(np.int64), P_sym.col, P_sym.val.astype(np.float64))
+        Yd = Y_end.astype(np.float64)
+        Fa_o = O.attractive(So, Yd)
+        Fr_o, Z_o, ng_o = O.repulsive_interp(Yd.copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d], O.ZETA_DIRECT)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        term_err = {"attractive_rel_l2": rel(Fa_g, Fa_o), "repulsive_rel_l2": rel(Fr_g, Fr_o),
+                    "Z_rel": abs(Z_g - Z_o) / Z_o, "grid_side_gpu": grid, "grid_side_oracle": ng_o}
+        log(f"[bench] FP32 vs FP64 oracle terms: {term_err}")
+        del So
         log(f"[bench] CPU baseline: FP64 oracle from the iteration-{it0} Y ...")
         ms_cpu, cores, cpu_iters = cpu_baseline(P_sym, Y_snap, args.d, it0, args.cpu_seconds, args.cpu_max_iters)
         cpu = {"value": round(ms_cpu, 2), "unit": "ms/iter", "cores": cores, "kind": "port",
@@ -173,12 +190,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
                    "nnz_sym": nnz, "format": args.format.lower(), "timed_iterations": [it0, it0 + args.steps - 1],
-                   "final_grid_side": grid, "final_kl": round(kl, 5)},
+                   "final_grid_side": grid, "final_kl": round(kl, 5),
+                   "attractive_tuning": tune},
         "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
         "full_run_ms": round((t_warm + t_run) * 1e3, 1),
         "setup_s": {"knn": round(t_knn, 2), "perplexity": round(t_perp, 2), "symmetrize_upload": round(t_sym, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "term_error_vs_fp64": term_err,
     }
     print(json.dumps(out), flush=True)
 

@@ -87,6 +87,7 @@ struct sgtsne_ctx {
   int64_t nnz = 0;
   int iter = 0;
   bool have_P = false, have_Y = false;
+  int points_n = 0;
 
   // P (CSR always; COO rows / hybrid as requested)
   int32_t *rowptr = nullptr, *col = nullptr;
@@ -99,6 +100,8 @@ struct sgtsne_ctx {
   int32_t *tail_row = nullptr, *tail_col = nullptr;
   float *tail_val = nullptr;
   int seg_group = 16;
+  int seg_unroll = 0;  // 0: plain segmented kernel; else the unrolled form
+  std::vector<double> tune_ms;  // autotuner timings (ms per candidate)
 
   // per-point state, AoS with stride ds
   float *Y = nullptr, *uY = nullptr, *gains = nullptr, *Fattr = nullptr, *Frep = nullptr;
@@ -112,6 +115,8 @@ struct sgtsne_ctx {
   // grid workspace
   size_t grid_cap = 0, w_cap = 0, k_cap = 0;
   float *V = nullptr, *Phi = nullptr;
+  float *s2g_part = nullptr;  // per-block private grids of the spread
+  int64_t s2g_cap = 0;        // floats
   ConvBuffers cb{};
 
   // timing
@@ -224,8 +229,14 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
   const size_t gb = G * m * sizeof(float);
   size_t wb = 0, kb = 0;
   conv_workspace_bytes(c->d, g.ng, g.L, &wb, &kb);
-  if (gb > c->grid_cap || wb > c->w_cap || kb > c->k_cap) {
+  const int64_t need = s2g_scratch_need(g, c->n);
+  if (gb > c->grid_cap || wb > c->w_cap || kb > c->k_cap || need > c->s2g_cap) {
     SGT_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (need > c->s2g_cap) {
+    free_dev(c->s2g_part);
+    SGT_TRY(alloc_dev((void **)&c->s2g_part, (size_t)need * sizeof(float)));
+    c->s2g_cap = need;
   }
   if (gb > c->grid_cap) {
     free_dev(c->V);
@@ -254,6 +265,9 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
 int attractive(sgtsne_ctx *c) {
   switch (c->p.format) {
     case SGTSNE_FORMAT_CSR_SEG:
+      if (c->seg_unroll > 0)
+        return launch_attr_csr_seg2(c->stream, c->d, c->seg_group, c->seg_unroll, c->n, c->rowptr, c->col,
+                                    c->val, c->Y, c->Fattr);
       return launch_attr_csr_seg(c->stream, c->d, c->seg_group, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
     case SGTSNE_FORMAT_CSR:
       return launch_attr_csr_row(c->stream, c->d, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
@@ -270,12 +284,10 @@ int attractive(sgtsne_ctx *c) {
 
 // Repulsive term for the current (centred) Y: S2G -> convolution -> G2S.
 int repulsive(sgtsne_ctx *c, const GridParams &g, PhaseEvents *pe) {
-  const int m = c->d + 1;
-  const int64_t G = c->d == 1 ? g.ng : c->d == 2 ? (int64_t)g.ng * g.ng : (int64_t)g.ng * g.ng * g.ng;
   const int zdirect = c->p.zeta_mode == SGTSNE_ZETA_DIRECT;
   SGT_TRY(ensure_grid(c, g));
-  SGT_TRY(launch_fill(c->stream, c->V, G * m, 0.0f));
-  SGT_TRY(launch_s2g(c->stream, g, c->n, c->Y, c->V));
+  int64_t need = 0;
+  SGT_TRY(launch_s2g(c->stream, g, c->n, c->Y, c->V, c->s2g_part, c->s2g_cap, &need));
   if (pe) SGT_HIP(hipEventRecord(pe->e[2], c->stream));
   SGT_TRY(launch_conv(c->stream, g, zdirect, c->V, c->Phi, c->cb));
   if (pe) SGT_HIP(hipEventRecord(pe->e[3], c->stream));
@@ -453,6 +465,64 @@ int apply_locality_order(sgtsne_ctx *c) {
   return st;
 }
 
+// Per-point state, sized by n (kept across P uploads of the same n).
+int alloc_points(sgtsne_ctx *c) {
+  const size_t nb = (size_t)c->n * c->ds;
+  if (c->Y && c->points_n == c->n) return SGTSNE_OK;
+  free_dev(c->Y);
+  free_dev(c->uY);
+  free_dev(c->gains);
+  free_dev(c->Fattr);
+  free_dev(c->Frep);
+  SGT_TRY(alloc_dev((void **)&c->Y, nb * sizeof(float)));
+  SGT_TRY(alloc_dev((void **)&c->uY, nb * sizeof(float)));
+  SGT_TRY(alloc_dev((void **)&c->gains, nb * sizeof(float)));
+  SGT_TRY(alloc_dev((void **)&c->Fattr, nb * sizeof(float)));
+  SGT_TRY(alloc_dev((void **)&c->Frep, nb * sizeof(float)));
+  SGT_HIP(hipMemsetAsync(c->Y, 0, nb * sizeof(float), c->stream));
+  SGT_HIP(hipMemsetAsync(c->Fattr, 0, nb * sizeof(float), c->stream));
+  SGT_HIP(hipMemsetAsync(c->Frep, 0, nb * sizeof(float), c->stream));
+  c->points_n = c->n;
+  return SGTSNE_OK;
+}
+
+// Autotuner of the attractive kernel: times every (lanes per row, unroll)
+// candidate on the uploaded matrix and keeps the fastest.  Runs once per
+// upload; the result is reported by sgtsne_ctx_tuning.
+int tune_attractive(sgtsne_ctx *c) {
+  struct Cand { int g, u; };
+  const Cand cands[] = {{8, 0}, {16, 0}, {32, 0}, {8, 2}, {8, 4}, {16, 2}, {16, 4}, {32, 2}, {32, 4}, {64, 2}};
+  if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
+  hipEvent_t e0, e1;
+  SGT_HIP(hipEventCreate(&e0));
+  SGT_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bg = c->seg_group, bu = c->seg_unroll;
+  c->tune_ms.clear();
+  int st = SGTSNE_OK;
+  for (const Cand &k : cands) {
+    float ms = 0.f, tot = 0.f;
+    for (int rep = 0; rep < 3 && st == SGTSNE_OK; rep++) {
+      (void)hipEventRecord(e0, c->stream);
+      st = k.u > 0 ? launch_attr_csr_seg2(c->stream, c->d, k.g, k.u, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
+                   : launch_attr_csr_seg(c->stream, c->d, k.g, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) tot += ms;  // the first run warms the caches
+    }
+    if (st != SGTSNE_OK) break;
+    c->tune_ms.push_back(tot / 2);
+    if (tot / 2 < best) { best = tot / 2; bg = k.g; bu = k.u; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SGT_TRY(st);
+  c->seg_group = bg;
+  c->seg_unroll = bu;
+  return SGTSNE_OK;
+}
+
 int build_formats(sgtsne_ctx *c) {
   if (c->p.format == SGTSNE_FORMAT_COO) {
     SGT_TRY(alloc_dev((void **)&c->coo_row, sizeof(int32_t) * (size_t)std::max<int64_t>(c->nnz, 1)));
@@ -589,6 +659,7 @@ int sgtsne_ctx_destroy(sgtsne_ctx *c) {
   free_dev(c->sc);
   free_dev(c->V);
   free_dev(c->Phi);
+  free_dev(c->s2g_part);
   free_dev(c->cb.W);
   free_dev(c->cb.Kt1);
   free_dev(c->cb.Kt2);
@@ -656,6 +727,8 @@ int sgtsne_ctx_upload_P(sgtsne_ctx *c, int n, int64_t nnz, const int32_t *rowptr
   }
   if (c->p.reorder) SGT_TRY(apply_locality_order(c));
   SGT_TRY(build_formats(c));
+  SGT_TRY(alloc_points(c));
+  if (c->p.format == SGTSNE_FORMAT_CSR_SEG) SGT_TRY(tune_attractive(c));
   SGT_HIP(hipStreamSynchronize(c->stream));
   c->have_P = true;
   return SGTSNE_OK;
@@ -709,16 +782,8 @@ int sgtsne_ctx_set_Y(sgtsne_ctx *c, const float *y, int layout, int reset_state)
   int n = c->n;
   SGT_CHECK(n > 0, SGTSNE_ERR_STATE, "upload P before Y (n unknown)");
   const size_t nb = (size_t)n * c->ds;
-  if (!c->Y) {
-    SGT_TRY(alloc_dev((void **)&c->Y, nb * sizeof(float)));
-    SGT_TRY(alloc_dev((void **)&c->uY, nb * sizeof(float)));
-    SGT_TRY(alloc_dev((void **)&c->gains, nb * sizeof(float)));
-    SGT_TRY(alloc_dev((void **)&c->Fattr, nb * sizeof(float)));
-    SGT_TRY(alloc_dev((void **)&c->Frep, nb * sizeof(float)));
-    SGT_HIP(hipMemsetAsync(c->Fattr, 0, nb * sizeof(float), c->stream));
-    SGT_HIP(hipMemsetAsync(c->Frep, 0, nb * sizeof(float), c->stream));
-    reset_state = 1;
-  }
+  SGT_TRY(alloc_points(c));
+  if (!c->have_Y) reset_state = 1;
   std::vector<float> h;
   pack_y(y, layout, n, c->d, c->ds, c->perm, h);
   double mean[4] = {0, 0, 0, 0};
@@ -841,6 +906,16 @@ int sgtsne_ctx_info(sgtsne_ctx *c, int *iter, int *n, int *d, int64_t *nnz) {
   if (n) *n = c->n;
   if (d) *d = c->d;
   if (nnz) *nnz = c->nnz;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_tuning(sgtsne_ctx *c, int *group, int *unroll, double *ms, int max, int *count) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  if (group) *group = c->seg_group;
+  if (unroll) *unroll = c->seg_unroll;
+  if (count) *count = (int)c->tune_ms.size();
+  if (ms)
+    for (int k = 0; k < std::min(max, (int)c->tune_ms.size()); k++) ms[k] = c->tune_ms[k];
   return SGTSNE_OK;
 }
 

@@ -43,6 +43,8 @@ struct GridParams {
 // ---- attractive term (kernels_spmv.hip) ----
 int launch_attr_csr_seg(hipStream_t s, int d, int group, int n, const int32_t *rowptr,
                         const int32_t *col, const float *val, const float *Y, float *F);
+int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, const int32_t *rowptr,
+                         const int32_t *col, const float *val, const float *Y, float *F);
 int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
                         const float *val, const float *Y, float *F);
 int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,
@@ -57,7 +59,12 @@ int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t 
 int launch_fill(hipStream_t s, float *p, int64_t count, float v);
 int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, RedWork rw,
                          DevScalars *sc);
-int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V);
+// S2G into V [d+1][ng^d] (V is overwritten).  scratch: float workspace for the
+// per-block private grids of the LDS path; *needed receives the floats it
+// wants (when larger than scratch_floats the global-atomic path runs).
+int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V, float *scratch,
+               int64_t scratch_floats, int64_t *needed);
+int64_t s2g_scratch_need(const GridParams &g, int n);
 int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
                      float *Frep, int zdirect, RedWork rw, DevScalars *sc);
 int launch_finalize_kl(hipStream_t s, int nblocks, RedWork rw, DevScalars *sc);

@@ -146,12 +146,12 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
 
 // ---- S2G, LDS-privatised whole grid -------------------------------------------
 // For grids that fit in LDS: each block spreads a contiguous chunk of points
-// into a private LDS copy of the grid (LDS float atomics), then adds its
-// non-zero cells to the global grid.  This removes the global-atomic
-// contention of clustered embeddings on small grids.
+// into a private LDS copy of the grid (LDS float atomics) and writes it out;
+// a second kernel sums the private grids.  No global atomics: clustered
+// embeddings on small grids no longer contend on a few addresses.
 template <int D>
 __global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
-                                                         float *__restrict__ V, int chunk) {
+                                                         float *__restrict__ V /* partials */, int chunk) {
   extern __shared__ float sgrid[];
   const int ng = g.ng;
   const int G = grid_cells<D>(ng);
@@ -176,9 +176,18 @@ __global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, co
     });
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < M * G; e += blockDim.x) {
-    const float v = sgrid[e];
-    if (v != 0.0f) unsafeAtomicAdd(&V[e], v);
+  // private grid -> this block's slice of the partial buffer (coalesced)
+  float *dst = V + (int64_t)blockIdx.x * M * G;
+  for (int e = threadIdx.x; e < M * G; e += blockDim.x) dst[e] = sgrid[e];
+}
+
+// Sum of the per-block private grids, in block order (deterministic).
+__global__ __launch_bounds__(kBlock) void grid_reduce_kernel(const float *__restrict__ part, int nparts,
+                                                             int cells, float *__restrict__ V) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cells; e += gridDim.x * blockDim.x) {
+    float acc = 0.0f;
+    for (int b = 0; b < nparts; ++b) acc += part[(int64_t)b * cells + e];
+    V[e] = acc;
   }
 }
 
@@ -480,30 +489,46 @@ int s2g_lds_fits(int d, int ng, size_t *lds_bytes) {
   return *lds_bytes <= 65536;
 }
 
-int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V) {
+int64_t s2g_scratch_need(const GridParams &g, int n) {
+  int64_t need = 0;
+  (void)launch_s2g(nullptr, g, -n, nullptr, nullptr, nullptr, 0, &need);
+  return need;
+}
+
+int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V, float *scratch,
+               int64_t scratch_floats, int64_t *needed) {
+  const bool query = n < 0;  // size query only
+  if (query) n = -n;
   size_t lds = 0;
+  const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);
+  int blocks = 0, chunk = 0;
   if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds)) {
-    // blocks: enough to fill the GPU, few enough that the flush of the private
-    // grids stays well below the point-wise atomic count
-    const int64_t cells = (int64_t)lds / sizeof(float);
-    const int64_t stencil = g.d == 1 ? 4 : g.d == 2 ? 16 : 64;
-    int64_t b = std::max<int64_t>(1, ((int64_t)n * stencil) / (cells * 4));
-    b = std::min<int64_t>(b, std::max<int64_t>(1, div_up(n, kBlock)));
-    b = std::min<int64_t>(b, 2048);
-    const int chunk = div_up(n, b);
-    const int blocks = div_up(n, chunk);
+    // about 8 points per thread, at most 512 private grids and 64 MiB of them
+    int64_t b = std::max<int64_t>(1, div_up(n, kBlock * 8));
+    b = std::min<int64_t>(b, 512);
+    b = std::min<int64_t>(b, std::max<int64_t>(1, (int64_t)(64ll << 20) / (cells * 4)));
+    chunk = div_up(n, b);
+    blocks = div_up(n, chunk);
+  }
+  if (needed) *needed = blocks > 0 ? (int64_t)blocks * cells : 0;
+  if (query) return SGTSNE_OK;
+  if (blocks > 0 && (int64_t)blocks * cells <= scratch_floats) {
     switch (g.d) {
-      case 1: s2g_lds_kernel<1><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
-      case 2: s2g_lds_kernel<2><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
-      case 3: s2g_lds_kernel<3><<<blocks, kBlock, lds, s>>>(g, n, Y, V, chunk); break;
+      case 1: s2g_lds_kernel<1><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 2: s2g_lds_kernel<2><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 3: s2g_lds_kernel<3><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
       default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
     }
+    SGT_LAUNCH_CHECK();
+    const int rb = (int)std::min<int64_t>(div_up(cells, kBlock), 4096);
+    grid_reduce_kernel<<<rb, kBlock, 0, s>>>(scratch, blocks, (int)cells, V);
   } else {
-    const int blocks = div_up(n > 0 ? n : 1, kBlock);
+    SGT_TRY(launch_fill(s, V, cells, 0.0f));
+    const int ab = div_up(n > 0 ? n : 1, kBlock);
     switch (g.d) {
-      case 1: s2g_atomic_kernel<1><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
-      case 2: s2g_atomic_kernel<2><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
-      case 3: s2g_atomic_kernel<3><<<blocks, kBlock, 0, s>>>(g, n, Y, V); break;
+      case 1: s2g_atomic_kernel<1><<<ab, kBlock, 0, s>>>(g, n, Y, V); break;
+      case 2: s2g_atomic_kernel<2><<<ab, kBlock, 0, s>>>(g, n, Y, V); break;
+      case 3: s2g_atomic_kernel<3><<<ab, kBlock, 0, s>>>(g, n, Y, V); break;
       default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
     }
   }

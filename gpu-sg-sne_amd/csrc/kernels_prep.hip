@@ -487,10 +487,26 @@ int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, c
   SGT_LAUNCH_CHECK();
   SGT_HIP(hipMemsetAsync(len + n, 0, sizeof(int32_t), s));
   SGT_TRY(device_exclusive_scan(s, len, rp2, n + 1));
-  perm_fill_kernel<<<div_up((int64_t)n * 64, 256), 256, 0, s>>>(n, perm, iperm, rp, col, val, rp2, col2, val2);
+  int32_t *ctmp = nullptr;
+  float *vtmp = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&ctmp, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1), s));
+  SGT_HIP(hipMallocAsync((void **)&vtmp, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1), s));
+  perm_fill_kernel<<<div_up((int64_t)n * 64, 256), 256, 0, s>>>(n, perm, iperm, rp, col, val, rp2, ctmp, vtmp);
   SGT_LAUNCH_CHECK();
+  // columns ascending within each row: neighbouring lanes gather neighbouring points
+  int nbits = 1;
+  while ((1ll << nbits) < (long long)n) ++nbits;
+  size_t tb = 0;
+  SGT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, ctmp, col2, vtmp, val2, (int)nnz, n, rp2,
+                                                      rp2 + 1, 0, nbits, s));
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, tb, ctmp, col2, vtmp, val2, (int)nnz, n, rp2,
+                                                      rp2 + 1, 0, nbits, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  SGT_HIP(hipFreeAsync(ctmp, s));
+  SGT_HIP(hipFreeAsync(vtmp, s));
   SGT_HIP(hipFreeAsync(len, s));
-  (void)nnz;
   return SGTSNE_OK;
 }
 

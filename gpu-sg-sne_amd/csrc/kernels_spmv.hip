@@ -13,6 +13,7 @@
 #include "../../include/sgtsne.h"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace sgt {
 
@@ -56,6 +57,53 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg_kernel(int n, const int32
       float yj[D];
       load_pt<D>(Y, col[t], yj);
       accum_pair<D>(yi, yj, val[t], acc);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
+// Format 4, tuned form: as above, but each lane issues the (col, val) loads of
+// U strides before the U neighbour gathers, so U independent loads per lane
+// are in flight; (col, val) are streamed once per iteration with nontemporal
+// loads and do not evict Y from the caches.
+template <int D, int G, int U>
+__global__ __launch_bounds__(kBlock) void attr_csr_seg2_kernel(int n, const int32_t *__restrict__ rowptr,
+                                                               const int32_t *__restrict__ col,
+                                                               const float *__restrict__ val,
+                                                               const float *__restrict__ Y,
+                                                               float *__restrict__ F) {
+  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  const int ngroups = gridDim.x * blockDim.x / G;
+  for (int row = gid; row < n; row += ngroups) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    const int e = rowptr[row + 1];
+    int t = rowptr[row] + lane;
+    for (; t + (U - 1) * G < e; t += U * G) {
+      int c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
+      float yj[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
+    }
+    for (; t < e; t += G) {
+      float yj[D];
+      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
     }
 #pragma unroll
     for (int a = 0; a < D; ++a)
@@ -192,6 +240,38 @@ int launch_attr_csr_seg(hipStream_t s, int d, int group, int n, const int32_t *r
   }
 #undef SEG_D
 #undef SEG_CASE
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, const int32_t *rowptr,
+                         const int32_t *col, const float *val, const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  const int blocks = div_up((int64_t)n * group, kBlock);
+  bool ok = false;
+  auto go = [&](auto dd, auto gg, auto uu) {
+    constexpr int DD = decltype(dd)::value, GG = decltype(gg)::value, UU = decltype(uu)::value;
+    if (d == DD && group == GG && unroll == UU) {
+      attr_csr_seg2_kernel<DD, GG, UU><<<blocks, kBlock, 0, s>>>(n, rowptr, col, val, Y, F);
+      ok = true;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  using I8 = std::integral_constant<int, 8>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  auto per_g = [&](auto dd, auto uu) {
+    go(dd, I4{}, uu); go(dd, I8{}, uu); go(dd, I16{}, uu); go(dd, I32{}, uu); go(dd, I64{}, uu);
+  };
+  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); per_g(dd, I4{}); };
+  per_d(I1{});
+  per_d(I2{});
+  per_d(I3{});
+  SGT_CHECK(ok, SGTSNE_ERR_ARG, "no attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }

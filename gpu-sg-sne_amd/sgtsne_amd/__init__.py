@@ -34,6 +34,7 @@ EXPORTED = [
     "sgtsne_ctx_get_P", "sgtsne_ctx_set_Y", "sgtsne_ctx_get_Y", "sgtsne_ctx_step",
     "sgtsne_ctx_terms", "sgtsne_ctx_kl", "sgtsne_ctx_repulsive_exact", "sgtsne_ctx_timing",
     "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
+    "sgtsne_ctx_tuning",
 ]
 
 
@@ -97,6 +98,8 @@ def lib():
         "sgtsne_ctx_repulsive_exact": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
         "sgtsne_ctx_timing": (C.c_int, [vp, f64p]),
         "sgtsne_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+        "sgtsne_ctx_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, C.c_int,
+                                        C.POINTER(C.c_int)]),
         "sgtsne_ctx_errors": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_ctx_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
@@ -264,6 +267,12 @@ class Context:
         out = np.zeros(7, np.float64)
         _check(lib().sgtsne_ctx_timing(self._h, out))
         return out
+
+    def tuning(self):
+        g, u, cnt = C.c_int(), C.c_int(), C.c_int()
+        ms = np.zeros(32)
+        _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), ms.ctypes.data, 32, C.byref(cnt)))
+        return {"group": g.value, "unroll": u.value, "candidates_ms": ms[:cnt.value].tolist()}
 
     def set_timing(self, on):
         _check(lib().sgtsne_ctx_set_timing(self._h, int(bool(on))))
