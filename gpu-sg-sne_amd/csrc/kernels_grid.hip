@@ -146,39 +146,97 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
 
 // ---- S2G, LDS-privatised whole grid -------------------------------------------
 // For grids that fit in LDS: each block spreads a contiguous chunk of points
-// into a private LDS copy of the grid (LDS float atomics) and writes it out;
-// a second kernel sums the private grids.  No global atomics: clustered
-// embeddings on small grids no longer contend on a few addresses.
+// into a private LDS copy of the grid and writes it out; a second kernel sums
+// the private grids.  No global atomics.  Lanes map to stencil nodes, not to
+// points: the 4^d lanes of one point touch 4^d distinct cells, so an LDS
+// atomic instruction conflicts only between the 64/4^d points of a wave
+// (at most 4-way in 2-D, never in 3-D) even when every point of a clustered
+// embedding falls into the same few cells.
+template <int D>
+struct NodeLanes {
+  static constexpr int NODES = D == 1 ? 4 : D == 2 ? 16 : 64;
+  static constexpr int PPW = 64 / NODES;  // points per wave per step
+};
+
+template <int D>
+__device__ __forceinline__ void node_of_lane(const GridParams &g, const float (&y)[D], int node, int &idx,
+                                             float &weight) {
+  int f[D];
+  float w[D][4];
+  stencil<D>(g, y, f, w);
+  const int a0 = node & 3, a1 = (node >> 2) & 3, a2 = node >> 4;
+  const int ng = g.ng;
+  if constexpr (D == 1) {
+    idx = f[0] + a0;
+    weight = w[0][a0];
+  } else if constexpr (D == 2) {
+    idx = (f[1] + a1) * ng + f[0] + a0;
+    weight = w[1][a1] * w[0][a0];
+  } else {
+    idx = ((f[2] + a2) * ng + f[1] + a1) * ng + f[0] + a0;
+    weight = w[2][a2] * w[1][a1] * w[0][a0];
+  }
+}
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
-                                                         float *__restrict__ V /* partials */, int chunk) {
+                                                         float *__restrict__ part, int chunk) {
   extern __shared__ float sgrid[];
-  const int ng = g.ng;
-  const int G = grid_cells<D>(ng);
   constexpr int M = D + 1;
+  constexpr int PPW = NodeLanes<D>::PPW;
+  constexpr int NODES = NodeLanes<D>::NODES;
+  const int G = grid_cells<D>(g.ng);
   for (int e = threadIdx.x; e < M * G; e += blockDim.x) sgrid[e] = 0.0f;
   __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int node = lane % NODES, slot = lane / NODES;
   const int i0 = blockIdx.x * chunk;
   const int i1 = min(n, i0 + chunk);
-  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    float y[D];
-    load_pt<D>(Y, i, y);
-    int f[D];
-    float w[D][4];
-    stencil<D>(g, y, f, w);
-    float q[M];
-    q[0] = 1.0f;
+  for (int base = i0 + wave * PPW; base < i1; base += nwaves * PPW) {
+    const int i = base + slot;
+    if (i < i1) {
+      float y[D];
+      load_pt<D>(Y, i, y);
+      int idx;
+      float ww;
+      node_of_lane<D>(g, y, node, idx, ww);
+      atomicAdd(&sgrid[idx], ww);
 #pragma unroll
-    for (int a = 0; a < D; ++a) q[a + 1] = y[a] - g.center[a];
-    for_stencil<D>(ng, f, w, [&](int idx, float ww) {
-#pragma unroll
-      for (int c = 0; c < M; ++c) atomicAdd(&sgrid[c * G + idx], q[c] * ww);
-    });
+      for (int a = 0; a < D; ++a) atomicAdd(&sgrid[(a + 1) * G + idx], (y[a] - g.center[a]) * ww);
+    }
   }
   __syncthreads();
   // private grid -> this block's slice of the partial buffer (coalesced)
-  float *dst = V + (int64_t)blockIdx.x * M * G;
+  float *dst = part + (int64_t)blockIdx.x * M * G;
   for (int e = threadIdx.x; e < M * G; e += blockDim.x) dst[e] = sgrid[e];
+}
+
+// ---- S2G with global float atomics (grids too large for LDS) -------------------
+// Same lane-per-node mapping; the grid is large here, so contention is low.
+template <int D>
+__global__ __launch_bounds__(kBlock) void s2g_atomic_kernel(GridParams g, int n,
+                                                            const float *__restrict__ Y,
+                                                            float *__restrict__ V) {
+  constexpr int PPW = NodeLanes<D>::PPW;
+  constexpr int NODES = NodeLanes<D>::NODES;
+  const int G = grid_cells<D>(g.ng);
+  const int lane = threadIdx.x & 63;
+  const int node = lane % NODES, slot = lane / NODES;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int base = gw * PPW; base < n; base += nw * PPW) {
+    const int i = base + slot;
+    if (i < n) {
+      float y[D];
+      load_pt<D>(Y, i, y);
+      int idx;
+      float ww;
+      node_of_lane<D>(g, y, node, idx, ww);
+      unsafeAtomicAdd(&V[idx], ww);
+#pragma unroll
+      for (int a = 0; a < D; ++a) unsafeAtomicAdd(&V[(a + 1) * G + idx], (y[a] - g.center[a]) * ww);
+    }
+  }
 }
 
 // Sum of the per-block private grids, in block order (deterministic).
@@ -188,31 +246,6 @@ __global__ __launch_bounds__(kBlock) void grid_reduce_kernel(const float *__rest
     float acc = 0.0f;
     for (int b = 0; b < nparts; ++b) acc += part[(int64_t)b * cells + e];
     V[e] = acc;
-  }
-}
-
-// ---- S2G with global float atomics (large grids) -------------------------------
-template <int D>
-__global__ __launch_bounds__(kBlock) void s2g_atomic_kernel(GridParams g, int n,
-                                                            const float *__restrict__ Y,
-                                                            float *__restrict__ V) {
-  const int ng = g.ng;
-  const int G = grid_cells<D>(ng);
-  constexpr int M = D + 1;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float y[D];
-    load_pt<D>(Y, i, y);
-    int f[D];
-    float w[D][4];
-    stencil<D>(g, y, f, w);
-    float q[M];
-    q[0] = 1.0f;
-#pragma unroll
-    for (int a = 0; a < D; ++a) q[a + 1] = y[a] - g.center[a];
-    for_stencil<D>(ng, f, w, [&](int idx, float ww) {
-#pragma unroll
-      for (int c = 0; c < M; ++c) unsafeAtomicAdd(&V[c * G + idx], q[c] * ww);
-    });
   }
 }
 
@@ -503,8 +536,8 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);
   int blocks = 0, chunk = 0;
   if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds)) {
-    // about 8 points per thread, at most 512 private grids and 64 MiB of them
-    int64_t b = std::max<int64_t>(1, div_up(n, kBlock * 8));
+    // about 32 points per wave, at most 512 private grids and 64 MiB of them
+    int64_t b = std::max<int64_t>(1, div_up(n, (kBlock / 64) * 32));
     b = std::min<int64_t>(b, 512);
     b = std::min<int64_t>(b, std::max<int64_t>(1, (int64_t)(64ll << 20) / (cells * 4)));
     chunk = div_up(n, b);
@@ -524,7 +557,8 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
     grid_reduce_kernel<<<rb, kBlock, 0, s>>>(scratch, blocks, (int)cells, V);
   } else {
     SGT_TRY(launch_fill(s, V, cells, 0.0f));
-    const int ab = div_up(n > 0 ? n : 1, kBlock);
+    const int ppb = (kBlock / 64) * (g.d == 1 ? 16 : g.d == 2 ? 4 : 1);  // points per block per step
+    const int ab = (int)std::min<int64_t>(div_up(n > 0 ? n : 1, ppb), 65536);
     switch (g.d) {
       case 1: s2g_atomic_kernel<1><<<ab, kBlock, 0, s>>>(g, n, Y, V); break;
       case 2: s2g_atomic_kernel<2><<<ab, kBlock, 0, s>>>(g, n, Y, V); break;
