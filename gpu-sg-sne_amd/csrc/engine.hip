@@ -132,6 +132,7 @@ struct sgtsne_ctx {
 
   // locality order: device row r holds the caller's point perm[r]
   bool permuted = false;
+  int reorders = 0;
   std::vector<int32_t> perm, iperm;
 };
 
@@ -333,8 +334,81 @@ void harvest_timing(sgtsne_ctx *c, PhaseEvents &pe, int it) {
   pe.valid = false;
 }
 
+// Iterations at which the points are re-ordered along a Morton curve of the
+// embedding (reorder >= 2, CSR formats): by then the clusters have formed and
+// the neighbours of a point in P are its neighbours in Y, so the gathers of
+// the attractive term and the spread / gather touch small windows of memory.
+constexpr int kSpatialReorderIters[] = {50, 150, 300, 600};
+
+int spatial_reorder(sgtsne_ctx *c) {
+  const int n = c->n, d = c->d, ds = c->ds;
+  SGT_HIP(hipEventSynchronize(c->ev_scal));
+  const DevScalars sh = *c->sc_host;
+  double lo[3] = {0, 0, 0}, extent = 0.0;
+  for (int a = 0; a < d; a++) {
+    lo[a] = sh.minv[a];
+    extent = std::max(extent, sh.maxv[a] - sh.minv[a]);
+  }
+  if (!(extent > 0.0) || !std::isfinite(extent)) return SGTSNE_OK;  // the step reports it
+  int32_t *order = nullptr, *inv = nullptr, *rp2 = nullptr, *col2 = nullptr;
+  float *val2 = nullptr, *tmp = nullptr;
+  SGT_TRY(alloc_dev((void **)&order, sizeof(int32_t) * (size_t)n));
+  SGT_TRY(alloc_dev((void **)&inv, sizeof(int32_t) * (size_t)n));
+  SGT_TRY(alloc_dev((void **)&rp2, sizeof(int32_t) * ((size_t)n + 1)));
+  SGT_TRY(alloc_dev((void **)&col2, sizeof(int32_t) * (size_t)std::max<int64_t>(c->nnz, 1)));
+  SGT_TRY(alloc_dev((void **)&val2, sizeof(float) * (size_t)std::max<int64_t>(c->nnz, 1)));
+  SGT_TRY(alloc_dev((void **)&tmp, sizeof(float) * (size_t)n * ds));
+  int st = spatial_order(c->stream, n, d, ds, c->Y, lo, extent, order, inv);
+  if (!st) st = launch_permute_csr(c->stream, n, c->nnz, order, inv, c->rowptr, c->col, c->val, rp2, col2, val2);
+  float **state[3] = {&c->Y, &c->uY, &c->gains};
+  for (int k = 0; k < 3 && !st; k++) {
+    st = launch_gather_points(c->stream, n, ds, order, *state[k], tmp);
+    std::swap(*state[k], tmp);
+  }
+  std::vector<int32_t> ord(n);
+  if (!st && hipMemcpyAsync(ord.data(), order, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+    set_error("copy of the order failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !st) {
+    set_error("re-ordering failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  free_dev(order);
+  free_dev(inv);
+  free_dev(tmp);
+  if (st) {
+    free_dev(rp2);
+    free_dev(col2);
+    free_dev(val2);
+    return st;
+  }
+  free_dev(c->rowptr);
+  free_dev(c->col);
+  free_dev(c->val);
+  c->rowptr = rp2;
+  c->col = col2;
+  c->val = val2;
+  // compose with the caller's order
+  if (c->perm.empty()) {
+    c->perm.resize(n);
+    for (int r = 0; r < n; r++) c->perm[r] = r;
+  }
+  std::vector<int32_t> np(n);
+  for (int r = 0; r < n; r++) np[r] = c->perm[ord[r]];
+  c->perm.swap(np);
+  c->iperm.assign(n, 0);
+  for (int r = 0; r < n; r++) c->iperm[c->perm[r]] = r;
+  c->permuted = true;
+  c->reorders++;
+  return SGTSNE_OK;
+}
+
 int iterate(sgtsne_ctx *c) {
   const int it = c->iter;
+  if (c->p.reorder >= 2 && (c->p.format == SGTSNE_FORMAT_CSR_SEG || c->p.format == SGTSNE_FORMAT_CSR))
+    for (int r : kSpatialReorderIters)
+      if (it == r) SGT_TRY(spatial_reorder(c));
   const double alpha = it <= c->p.early_iter ? c->p.alpha : 1.0;
   const double momentum = it <= 250 ? 0.5 : 0.8;
   const double eta = 200.0;
@@ -586,7 +660,7 @@ int sgtsne_params_default(sgtsne_params *p) {
   p->seed = 0;
   p->ell_width = 0;
   p->zeta_mode = SGTSNE_ZETA_DIRECT;
-  p->reorder = 1;
+  p->reorder = 2;
   return SGTSNE_OK;
 }
 

@@ -104,6 +104,11 @@ int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, co
                       const float *val, const int32_t *tail_off, int32_t *trow, int32_t *tcol,
                       float *tval);
 int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n);
+// order[r] = current index of the point placed at r (Morton order of Y over
+// the box [lo, lo + extent]^d); inv = order^-1.
+int spatial_order(hipStream_t s, int n, int d, int ds, const float *Y, const double *lo, double extent,
+                  int32_t *order, int32_t *inv);
+int launch_gather_points(hipStream_t s, int n, int ds, const int32_t *order, const float *in, float *out);
 int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
                        const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
                        int32_t *col2, float *val2);

@@ -178,13 +178,17 @@ __device__ __forceinline__ void node_of_lane(const GridParams &g, const float (&
   }
 }
 
+constexpr int kS2GBlock = 1024;  // 16 waves share one private grid
+constexpr int kS2GUnroll = 4;    // points per lane whose loads are issued together
+
 template <int D>
-__global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
-                                                         float *__restrict__ part, int chunk) {
+__global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
+                                                            float *__restrict__ part, int chunk) {
   extern __shared__ float sgrid[];
   constexpr int M = D + 1;
   constexpr int PPW = NodeLanes<D>::PPW;
   constexpr int NODES = NodeLanes<D>::NODES;
+  constexpr int U = kS2GUnroll;
   const int G = grid_cells<D>(g.ng);
   for (int e = threadIdx.x; e < M * G; e += blockDim.x) sgrid[e] = 0.0f;
   __syncthreads();
@@ -192,17 +196,24 @@ __global__ __launch_bounds__(kBlock) void s2g_lds_kernel(GridParams g, int n, co
   const int node = lane % NODES, slot = lane / NODES;
   const int i0 = blockIdx.x * chunk;
   const int i1 = min(n, i0 + chunk);
-  for (int base = i0 + wave * PPW; base < i1; base += nwaves * PPW) {
-    const int i = base + slot;
-    if (i < i1) {
-      float y[D];
-      load_pt<D>(Y, i, y);
-      int idx;
-      float ww;
-      node_of_lane<D>(g, y, node, idx, ww);
-      atomicAdd(&sgrid[idx], ww);
+  for (int base = i0 + wave * PPW * U; base < i1; base += nwaves * PPW * U) {
+    float y[U][D];
 #pragma unroll
-      for (int a = 0; a < D; ++a) atomicAdd(&sgrid[(a + 1) * G + idx], (y[a] - g.center[a]) * ww);
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * PPW + slot;
+      if (i < i1) load_pt<D>(Y, i, y[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * PPW + slot;
+      if (i < i1) {
+        int idx;
+        float ww;
+        node_of_lane<D>(g, y[u], node, idx, ww);
+        atomicAdd(&sgrid[idx], ww);
+#pragma unroll
+        for (int a = 0; a < D; ++a) atomicAdd(&sgrid[(a + 1) * G + idx], (y[u][a] - g.center[a]) * ww);
+      }
     }
   }
   __syncthreads();
@@ -536,9 +547,9 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);
   int blocks = 0, chunk = 0;
   if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds)) {
-    // about 32 points per wave, at most 512 private grids and 64 MiB of them
-    int64_t b = std::max<int64_t>(1, div_up(n, (kBlock / 64) * 32));
-    b = std::min<int64_t>(b, 512);
+    // about 64 points per wave, at most 1024 private grids and 64 MiB of them
+    int64_t b = std::max<int64_t>(1, div_up(n, (kS2GBlock / 64) * 64));
+    b = std::min<int64_t>(b, 1024);
     b = std::min<int64_t>(b, std::max<int64_t>(1, (int64_t)(64ll << 20) / (cells * 4)));
     chunk = div_up(n, b);
     blocks = div_up(n, chunk);
@@ -547,9 +558,9 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   if (query) return SGTSNE_OK;
   if (blocks > 0 && (int64_t)blocks * cells <= scratch_floats) {
     switch (g.d) {
-      case 1: s2g_lds_kernel<1><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
-      case 2: s2g_lds_kernel<2><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
-      case 3: s2g_lds_kernel<3><<<blocks, kBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 1: s2g_lds_kernel<1><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 2: s2g_lds_kernel<2><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 3: s2g_lds_kernel<3><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
       default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
     }
     SGT_LAUNCH_CHECK();

@@ -359,6 +359,59 @@ __global__ void perm_fill_kernel(int n, const int32_t *__restrict__ perm, const 
   }
 }
 
+// ---- spatial order of the embedding -------------------------------------------
+// Morton key of each point's position on a 2^B-per-axis lattice over the
+// bounding box (B = 16 / 10 / 32 bits per axis for d = 2 / 3 / 1).
+__device__ __forceinline__ uint32_t spread_bits2(uint32_t x) {
+  x &= 0xffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+__device__ __forceinline__ uint32_t spread_bits3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ void morton_kernel(int n, int d, int ds, const float *__restrict__ Y, float lo0, float lo1, float lo2,
+                              float inv, uint32_t *__restrict__ key, int32_t *__restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float lo[3] = {lo0, lo1, lo2};
+  uint32_t q[3] = {0, 0, 0};
+  const float top = d == 1 ? 4294967040.0f : d == 2 ? 65535.0f : 1023.0f;
+  for (int a = 0; a < d; ++a) {
+    float t = (Y[(size_t)i * ds + a] - lo[a]) * inv * top;
+    t = fminf(fmaxf(t, 0.0f), top);  // NaN -> 0
+    q[a] = (uint32_t)t;
+  }
+  uint32_t k;
+  if (d == 1) k = q[0];
+  else if (d == 2) k = spread_bits2(q[0]) | (spread_bits2(q[1]) << 1);
+  else k = spread_bits3(q[0]) | (spread_bits3(q[1]) << 1) | (spread_bits3(q[2]) << 2);
+  key[i] = k;
+  idx[i] = i;
+}
+
+__global__ void invert_perm_kernel(int n, const int32_t *__restrict__ order, int32_t *__restrict__ inv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) inv[order[r]] = r;
+}
+
+__global__ void gather_points_kernel(int n, int ds, const int32_t *__restrict__ order, const float *__restrict__ in,
+                                     float *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * ds) return;
+  const int r = (int)(t / ds), a = (int)(t % ds);
+  out[t] = in[(int64_t)order[r] * ds + a];
+}
+
 __global__ void to_double_kernel(const float *__restrict__ in, double *__restrict__ out, int64_t m) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) out[t] = (double)in[t];
@@ -507,6 +560,37 @@ int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, c
   SGT_HIP(hipFreeAsync(ctmp, s));
   SGT_HIP(hipFreeAsync(vtmp, s));
   SGT_HIP(hipFreeAsync(len, s));
+  return SGTSNE_OK;
+}
+
+int spatial_order(hipStream_t s, int n, int d, int ds, const float *Y, const double *lo, double extent,
+                  int32_t *order, int32_t *inv) {
+  uint32_t *k0 = nullptr, *k1 = nullptr;
+  int32_t *i0 = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&k0, sizeof(uint32_t) * (size_t)n, s));
+  SGT_HIP(hipMallocAsync((void **)&k1, sizeof(uint32_t) * (size_t)n, s));
+  SGT_HIP(hipMallocAsync((void **)&i0, sizeof(int32_t) * (size_t)n, s));
+  const float invx = extent > 0 ? (float)(1.0 / extent) : 0.0f;
+  morton_kernel<<<div_up(n, 256), 256, 0, s>>>(n, d, ds, Y, (float)lo[0], d > 1 ? (float)lo[1] : 0.f,
+                                                d > 2 ? (float)lo[2] : 0.f, invx, k0, i0);
+  SGT_LAUNCH_CHECK();
+  size_t tb = 0;
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, order, n, 0, 32, s));
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, order, n, 0, 32, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  invert_perm_kernel<<<div_up(n, 256), 256, 0, s>>>(n, order, inv);
+  SGT_LAUNCH_CHECK();
+  SGT_HIP(hipFreeAsync(k0, s));
+  SGT_HIP(hipFreeAsync(k1, s));
+  SGT_HIP(hipFreeAsync(i0, s));
+  return SGTSNE_OK;
+}
+
+int launch_gather_points(hipStream_t s, int n, int ds, const int32_t *order, const float *in, float *out) {
+  gather_points_kernel<<<div_up((int64_t)n * ds, 256), 256, 0, s>>>(n, ds, order, in, out);
+  SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
 

@@ -78,9 +78,11 @@ typedef struct sgtsne_params {
   uint64_t seed;      /* seed of the N(0, 1e-4^2) start when y_in is NULL     */
   int ell_width;      /* hybrid ELL width; <= 0 = automatic                   */
   int zeta_mode;      /* SGTSNE_ZETA_*                          (DIRECT)      */
-  int reorder;        /* 1: keep P and Y on the device in a breadth-first
-                         order of P (locality of the neighbour gathers); the
-                         API still reads and writes the caller's order  (1)   */
+  int reorder;        /* device order of the points (the API always reads and
+                         writes the caller's order): 0 as given; 1 breadth-
+                         first order of P at upload; 2 also re-ordered along
+                         a Morton curve of Y at iterations 50, 150, 300, 600
+                         (CSR formats)                                  (2)   */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of

@@ -361,3 +361,23 @@ def test_soa_layout_and_errors(sg):
         ctx.set_Y(Yb)
         with pytest.raises(sg.SgtsneError, match="-4"):
             ctx.step(1)
+
+
+def test_reorder_modes_agree(sg, O):
+    # the device order (as given / breadth-first / re-sorted along Y) changes
+    # only summation order: 60 iterations (a spatial re-order at 50) agree
+    n = 3000
+    X = _mixture(n, 20, 6, 14)
+    I, D = O.knn_exact(X, 45)
+    P = sg.perplexityEqualization(I, D, n, 45, 15.0)
+    y0 = (1e-4 * np.random.default_rng(2).standard_normal((n, 2))).astype(np.float32)
+    out = {}
+    for mode in (0, 1, 2):
+        with sg.Context(sg.tsneparams(reorder=mode)) as ctx:
+            ctx.upload_P(P, symmetrize=True)
+            ctx.set_Y(y0)
+            ctx.step(60)
+            out[mode] = (ctx.get_Y(), ctx.terms())
+    for mode in (1, 2):
+        assert rel_l2(out[mode][0], out[0][0]) < 1e-3
+        assert rel_l2(out[mode][1][0], out[0][1][0]) < 1e-3
