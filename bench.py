@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--d", type=int, default=2)
     ap.add_argument("--format", default="CSR_SEG")
+    ap.add_argument("--reorder", type=int, default=2, help="device point order: 0 given, 1 BFS of P, 2 BFS + Morton of Y")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-iters", type=int, default=30)
@@ -103,7 +104,7 @@ def main():
     P = sg.perplexityEqualization(I, D, n, k, u)
     t_perp = time.perf_counter() - t0
     del I, D
-    prm = sg.tsneparams(d=args.d, format=getattr(sg, "FORMAT_" + args.format))
+    prm = sg.tsneparams(d=args.d, format=getattr(sg, "FORMAT_" + args.format), reorder=args.reorder)
     ctx = sg.Context(prm)
     t0 = time.perf_counter()
     ctx.upload_P(P, symmetrize=True)
@@ -146,7 +147,8 @@ def main():
     achieved = bytes_b / t_spmv / 1e9 if t_spmv > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": (f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
+                "kernel": (f"attr_csr_blk_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
+                           else f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
                            else f"attr_csr_seg_kernel<{args.d},{tune['group']}>"),
                 "bytes_per_launch": bytes_b,
                 "avg_launch_ms": round(phases["attractive"], 5)}
@@ -189,7 +191,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
-                   "nnz_sym": nnz, "format": args.format.lower(), "timed_iterations": [it0, it0 + args.steps - 1],
+                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder, "timed_iterations": [it0, it0 + args.steps - 1],
                    "final_grid_side": grid, "final_kl": round(kl, 5),
                    "attractive_tuning": tune},
         "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},

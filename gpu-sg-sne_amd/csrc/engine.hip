@@ -101,6 +101,7 @@ struct sgtsne_ctx {
   float *tail_val = nullptr;
   int seg_group = 16;
   int seg_unroll = 0;  // 0: plain segmented kernel; else the unrolled form
+  int seg_rpb = 0;     // > 0: blocked form with this many rows per work-group
   std::vector<double> tune_ms;  // autotuner timings (ms per candidate)
 
   // per-point state, AoS with stride ds
@@ -266,6 +267,9 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
 int attractive(sgtsne_ctx *c) {
   switch (c->p.format) {
     case SGTSNE_FORMAT_CSR_SEG:
+      if (c->seg_rpb > 0)
+        return launch_attr_csr_blk(c->stream, c->d, c->seg_group, c->seg_unroll, c->seg_rpb, c->n, c->rowptr,
+                                   c->col, c->val, c->Y, c->Fattr);
       if (c->seg_unroll > 0)
         return launch_attr_csr_seg2(c->stream, c->d, c->seg_group, c->seg_unroll, c->n, c->rowptr, c->col,
                                     c->val, c->Y, c->Fattr);
@@ -564,22 +568,24 @@ int alloc_points(sgtsne_ctx *c) {
 // candidate on the uploaded matrix and keeps the fastest.  Runs once per
 // upload; the result is reported by sgtsne_ctx_tuning.
 int tune_attractive(sgtsne_ctx *c) {
-  struct Cand { int g, u; };
-  const Cand cands[] = {{8, 0}, {16, 0}, {32, 0}, {8, 2}, {8, 4}, {16, 2}, {16, 4}, {32, 2}, {32, 4}, {64, 2}};
+  struct Cand { int g, u, rpb; };  // lanes per row, unroll (0 = plain), rows per work-group (0 = grid-stride)
+  const Cand cands[] = {{16, 0, 0},  {32, 0, 0},  {16, 4, 0},  {32, 2, 0},  {64, 2, 0},
+                        {16, 1, 64}, {32, 1, 64}, {32, 2, 64}, {16, 2, 128}, {32, 2, 256}, {64, 1, 256}};
   if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  int bg = c->seg_group, bu = c->seg_unroll;
+  int bg = c->seg_group, bu = c->seg_unroll, brpb = c->seg_rpb;
   c->tune_ms.clear();
   int st = SGTSNE_OK;
   for (const Cand &k : cands) {
     float ms = 0.f, tot = 0.f;
     for (int rep = 0; rep < 3 && st == SGTSNE_OK; rep++) {
       (void)hipEventRecord(e0, c->stream);
-      st = k.u > 0 ? launch_attr_csr_seg2(c->stream, c->d, k.g, k.u, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
-                   : launch_attr_csr_seg(c->stream, c->d, k.g, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
+      st = k.rpb > 0 ? launch_attr_csr_blk(c->stream, c->d, k.g, k.u, k.rpb, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
+           : k.u > 0 ? launch_attr_csr_seg2(c->stream, c->d, k.g, k.u, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
+                     : launch_attr_csr_seg(c->stream, c->d, k.g, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
       (void)hipEventRecord(e1, c->stream);
       (void)hipEventSynchronize(e1);
       (void)hipEventElapsedTime(&ms, e0, e1);
@@ -587,13 +593,14 @@ int tune_attractive(sgtsne_ctx *c) {
     }
     if (st != SGTSNE_OK) break;
     c->tune_ms.push_back(tot / 2);
-    if (tot / 2 < best) { best = tot / 2; bg = k.g; bu = k.u; }
+    if (tot / 2 < best) { best = tot / 2; bg = k.g; bu = k.u; brpb = k.rpb; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   SGT_TRY(st);
   c->seg_group = bg;
   c->seg_unroll = bu;
+  c->seg_rpb = brpb;
   return SGTSNE_OK;
 }
 
@@ -983,8 +990,10 @@ int sgtsne_ctx_info(sgtsne_ctx *c, int *iter, int *n, int *d, int64_t *nnz) {
   return SGTSNE_OK;
 }
 
-int sgtsne_ctx_tuning(sgtsne_ctx *c, int *group, int *unroll, double *ms, int max, int *count) {
+int sgtsne_ctx_tuning(sgtsne_ctx *c, int *group, int *unroll, int *rows_per_block, double *ms, int max,
+                      int *count) {
   SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  if (rows_per_block) *rows_per_block = c->seg_rpb;
   if (group) *group = c->seg_group;
   if (unroll) *unroll = c->seg_unroll;
   if (count) *count = (int)c->tune_ms.size();

@@ -45,6 +45,8 @@ int launch_attr_csr_seg(hipStream_t s, int d, int group, int n, const int32_t *r
                         const int32_t *col, const float *val, const float *Y, float *F);
 int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, const int32_t *rowptr,
                          const int32_t *col, const float *val, const float *Y, float *F);
+int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_per_block, int n,
+                        const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
 int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
                         const float *val, const float *Y, float *F);
 int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,

@@ -183,14 +183,15 @@ constexpr int kS2GUnroll = 4;    // points per lane whose loads are issued toget
 
 template <int D>
 __global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
-                                                            float *__restrict__ part, int chunk) {
+                                                            float *__restrict__ part, int chunk, int copies) {
   extern __shared__ float sgrid[];
   constexpr int M = D + 1;
   constexpr int PPW = NodeLanes<D>::PPW;
   constexpr int NODES = NodeLanes<D>::NODES;
   constexpr int U = kS2GUnroll;
   const int G = grid_cells<D>(g.ng);
-  for (int e = threadIdx.x; e < M * G; e += blockDim.x) sgrid[e] = 0.0f;
+  const int MG = M * G;
+  for (int e = threadIdx.x; e < copies * MG; e += blockDim.x) sgrid[e] = 0.0f;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int node = lane % NODES, slot = lane / NODES;
@@ -210,16 +211,23 @@ __global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n,
         int idx;
         float ww;
         node_of_lane<D>(g, y[u], node, idx, ww);
-        atomicAdd(&sgrid[idx], ww);
+        // the point slots of one instruction (and consecutive instructions of
+        // one slot) write different copies: no same-address LDS atomics
+        float *cp = sgrid + ((slot + u) % copies) * MG;
+        atomicAdd(&cp[idx], ww);
 #pragma unroll
-        for (int a = 0; a < D; ++a) atomicAdd(&sgrid[(a + 1) * G + idx], (y[u][a] - g.center[a]) * ww);
+        for (int a = 0; a < D; ++a) atomicAdd(&cp[(a + 1) * G + idx], (y[u][a] - g.center[a]) * ww);
       }
     }
   }
   __syncthreads();
-  // private grid -> this block's slice of the partial buffer (coalesced)
-  float *dst = part + (int64_t)blockIdx.x * M * G;
-  for (int e = threadIdx.x; e < M * G; e += blockDim.x) dst[e] = sgrid[e];
+  // sum of the copies -> this block's slice of the partial buffer (coalesced)
+  float *dst = part + (int64_t)blockIdx.x * MG;
+  for (int e = threadIdx.x; e < MG; e += blockDim.x) {
+    float v = 0.0f;
+    for (int k = 0; k < copies; ++k) v += sgrid[k * MG + e];
+    dst[e] = v;
+  }
 }
 
 // ---- S2G with global float atomics (grids too large for LDS) -------------------
@@ -250,13 +258,29 @@ __global__ __launch_bounds__(kBlock) void s2g_atomic_kernel(GridParams g, int n,
   }
 }
 
-// Sum of the per-block private grids, in block order (deterministic).
-__global__ __launch_bounds__(kBlock) void grid_reduce_kernel(const float *__restrict__ part, int nparts,
-                                                             int cells, float *__restrict__ V) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cells; e += gridDim.x * blockDim.x) {
-    float acc = 0.0f;
-    for (int b = 0; b < nparts; ++b) acc += part[(int64_t)b * cells + e];
-    V[e] = acc;
+// Sum of the per-block private grids (deterministic): 64 cells per block,
+// 16 partial groups per cell summed in a fixed order, then combined in LDS.
+__global__ __launch_bounds__(1024) void grid_reduce_kernel(const float *__restrict__ part, int nparts,
+                                                           int cells, float *__restrict__ V) {
+  __shared__ float red[16][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int gidx = threadIdx.x >> 6;
+  float acc = 0.0f;
+  if (c < cells) {
+    int b = gidx;
+    for (; b + 48 < nparts; b += 64) {
+      const float v0 = part[(int64_t)b * cells + c], v1 = part[(int64_t)(b + 16) * cells + c];
+      const float v2 = part[(int64_t)(b + 32) * cells + c], v3 = part[(int64_t)(b + 48) * cells + c];
+      acc += (v0 + v1) + (v2 + v3);
+    }
+    for (; b < nparts; b += 16) acc += part[(int64_t)b * cells + c];
+  }
+  red[gidx][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64 && c < cells) {
+    float t = 0.0f;
+    for (int k = 0; k < 16; ++k) t += red[k][threadIdx.x];
+    V[c] = t;
   }
 }
 
@@ -527,9 +551,14 @@ int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, 
   return finalize(s, FIN_MINMAX, blocks, d, n, apply_mean, rw, sc);
 }
 
-int s2g_lds_fits(int d, int ng, size_t *lds_bytes) {
+// LDS grid copies for the spread: one per point slot of a wave when they fit.
+int s2g_lds_fits(int d, int ng, size_t *lds_bytes, int *copies) {
   const size_t G = d == 1 ? (size_t)ng : d == 2 ? (size_t)ng * ng : (size_t)ng * ng * ng;
-  *lds_bytes = G * (size_t)(d + 1) * sizeof(float);
+  const size_t one = G * (size_t)(d + 1) * sizeof(float);
+  int c = d == 1 ? 16 : d == 2 ? 4 : 1;
+  while (c > 1 && one * c > 65536) c >>= 1;
+  *copies = c;
+  *lds_bytes = one * c;
   return *lds_bytes <= 65536;
 }
 
@@ -544,9 +573,10 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   const bool query = n < 0;  // size query only
   if (query) n = -n;
   size_t lds = 0;
+  int copies = 1;
   const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);
   int blocks = 0, chunk = 0;
-  if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds)) {
+  if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds, &copies)) {
     // about 64 points per wave, at most 1024 private grids and 64 MiB of them
     int64_t b = std::max<int64_t>(1, div_up(n, (kS2GBlock / 64) * 64));
     b = std::min<int64_t>(b, 1024);
@@ -558,14 +588,13 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   if (query) return SGTSNE_OK;
   if (blocks > 0 && (int64_t)blocks * cells <= scratch_floats) {
     switch (g.d) {
-      case 1: s2g_lds_kernel<1><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
-      case 2: s2g_lds_kernel<2><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
-      case 3: s2g_lds_kernel<3><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk); break;
+      case 1: s2g_lds_kernel<1><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk, copies); break;
+      case 2: s2g_lds_kernel<2><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk, copies); break;
+      case 3: s2g_lds_kernel<3><<<blocks, kS2GBlock, lds, s>>>(g, n, Y, scratch, chunk, copies); break;
       default: SGT_CHECK(false, SGTSNE_ERR_ARG, "d must be 1..3");
     }
     SGT_LAUNCH_CHECK();
-    const int rb = (int)std::min<int64_t>(div_up(cells, kBlock), 4096);
-    grid_reduce_kernel<<<rb, kBlock, 0, s>>>(scratch, blocks, (int)cells, V);
+    grid_reduce_kernel<<<div_up(cells, 64), 1024, 0, s>>>(scratch, blocks, (int)cells, V);
   } else {
     SGT_TRY(launch_fill(s, V, cells, 0.0f));
     const int ppb = (kBlock / 64) * (g.d == 1 ? 16 : g.d == 2 ? 4 : 1);  // points per block per step

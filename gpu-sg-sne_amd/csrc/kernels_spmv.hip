@@ -113,6 +113,62 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg2_kernel(int n, const int3
   }
 }
 
+// Format 4, blocked form: each work-group owns a contiguous range of rows,
+// so the neighbour gathers of one CU cover one window of the (locality-
+// ordered) points and hit its L1; ranges are dealt to XCDs in contiguous
+// runs (work-groups are placed round-robin over the 8 XCDs) so that
+// neighbouring ranges share an L2.
+constexpr int kXcds = 8;
+
+template <int D, int G, int U>
+__global__ __launch_bounds__(kBlock) void attr_csr_blk_kernel(int n, int rows_per_block, int nranges,
+                                                              const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ col,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ Y,
+                                                              float *__restrict__ F) {
+  const int per_xcd = (gridDim.x + kXcds - 1) / kXcds;
+  const int range = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (range >= nranges) return;
+  const int r0 = range * rows_per_block;
+  const int r1 = min(n, r0 + rows_per_block);
+  const int grp = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kBlock / G;
+  for (int row = r0 + grp; row < r1; row += NG) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    const int e = rowptr[row + 1];
+    int t = rowptr[row] + lane;
+    for (; t + (U - 1) * G < e; t += U * G) {
+      int c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
+      float yj[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
+    }
+    for (; t < e; t += G) {
+      float yj[D];
+      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
 // Format 0: one thread per row (CU/pq.cuh:16-75).
 template <int D>
 __global__ __launch_bounds__(kBlock) void attr_csr_row_kernel(int n, const int32_t *__restrict__ rowptr,
@@ -272,6 +328,35 @@ int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, con
   per_d(I2{});
   per_d(I3{});
   SGT_CHECK(ok, SGTSNE_ERR_ARG, "no attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_per_block, int n,
+                        const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  const int nranges = div_up(n, rows_per_block);
+  const int blocks = div_up(nranges, kXcds) * kXcds;
+  bool ok = false;
+  auto go = [&](auto dd, auto gg, auto uu) {
+    constexpr int DD = decltype(dd)::value, GG = decltype(gg)::value, UU = decltype(uu)::value;
+    if (d == DD && group == GG && unroll == UU) {
+      attr_csr_blk_kernel<DD, GG, UU><<<blocks, kBlock, 0, s>>>(n, rows_per_block, nranges, rowptr, col, val, Y, F);
+      ok = true;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  auto per_g = [&](auto dd, auto uu) { go(dd, I16{}, uu); go(dd, I32{}, uu); go(dd, I64{}, uu); };
+  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); };
+  per_d(I1{});
+  per_d(I2{});
+  per_d(I3{});
+  SGT_CHECK(ok, SGTSNE_ERR_ARG, "no blocked attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }

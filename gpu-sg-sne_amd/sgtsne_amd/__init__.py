@@ -98,8 +98,8 @@ def lib():
         "sgtsne_ctx_repulsive_exact": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
         "sgtsne_ctx_timing": (C.c_int, [vp, f64p]),
         "sgtsne_ctx_set_timing": (C.c_int, [vp, C.c_int]),
-        "sgtsne_ctx_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, C.c_int,
-                                        C.POINTER(C.c_int)]),
+        "sgtsne_ctx_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp,
+                                        C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_ctx_errors": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_ctx_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
@@ -269,10 +269,12 @@ class Context:
         return out
 
     def tuning(self):
-        g, u, cnt = C.c_int(), C.c_int(), C.c_int()
+        g, u, rpb, cnt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         ms = np.zeros(32)
-        _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), ms.ctypes.data, 32, C.byref(cnt)))
-        return {"group": g.value, "unroll": u.value, "candidates_ms": ms[:cnt.value].tolist()}
+        _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), ms.ctypes.data, 32,
+                                       C.byref(cnt)))
+        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value,
+                "candidates_ms": [round(x, 4) for x in ms[:cnt.value]]}
 
     def set_timing(self, on):
         _check(lib().sgtsne_ctx_set_timing(self._h, int(bool(on))))

@@ -184,9 +184,11 @@ int sgtsne_ctx_repulsive_exact(sgtsne_ctx *ctx, float *Frep, double *Z, int layo
 int sgtsne_ctx_timing(sgtsne_ctx *ctx, double *out7);
 
 /* Choice of the attractive-kernel autotuner (run at upload for the default
- * format): lanes per row, unroll (0 = plain form), and the measured ms of
- * each candidate (up to max; *count receives how many were timed). */
-int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, double *ms, int max, int *count);
+ * format): lanes per row, unroll (0 = plain form), rows per work-group (0 =
+ * grid-stride form), and the measured ms of each candidate (up to max;
+ * *count receives how many were timed). */
+int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, int *rows_per_block, double *ms, int max,
+                      int *count);
 
 /* Enables (default) or disables the per-phase event timing of ctx_step. */
 int sgtsne_ctx_set_timing(sgtsne_ctx *ctx, int on);
