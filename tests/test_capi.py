@@ -71,3 +71,27 @@ def test_product_never_uses_the_oracle():
     if os.path.exists(so):
         out = subprocess.run(["ldd", so], capture_output=True, text=True).stdout
         assert "oracle" not in out
+
+
+def test_cpp_mirror_header_compiles_and_links(sg, tmp_path):
+    # include/sgtsne.hpp (the reference-shaped C++ entry points) builds and
+    # links against the library; the program makes no call that needs a GPU.
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "sgtsne.hpp"\n'
+        "int main(int argc, char **) {\n"
+        "  tsneparams p; sparse_matrix<float> P;\n"
+        "  if (argc > 99) {\n"
+        "    float *y = sgtsneCUDA<float>(&P, p, nullptr, nullptr); free(y);\n"
+        "    auto Q = perplexityEqualization<float>(nullptr, nullptr, 0, 0, 30.f); (void)Q;\n"
+        "    sgtsne_hyb::sparse_matrix H{}; sgtsne_hyb::tsneparams hp;\n"
+        "    double *yy = sgtsne_hyb::sgtsne(H, hp); free(yy);\n"
+        "  }\n"
+        "  return (p.d == 2 && p.maxIter == 1000 && p.earlyIter == 250 && p.ng == 50) ? 0 : 1;\n"
+        "}\n")
+    exe = tmp_path / "use"
+    libdir = os.path.dirname(sg.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", libdir, "-lsgtsne_hip", "-o", str(exe)], check=True)
+    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    assert subprocess.run([str(exe)], env=env).returncode == 0
