@@ -1,0 +1,862 @@
+// batch.hip -- the batched engine behind sgtsne_batch_* (include/sgtsne.h):
+// many independent embedding jobs of uneven size advanced together, one
+// launch per phase for all of them, iterations replayed from a captured
+// HIP graph.  The single-job pipeline it batches is sgtsneCUDA
+// (CU/sgtsne.cu:26-128) / kl_minimization (CU/gradient_descend.cu:102-245).
+//
+// Memory: one arena of points (Y, uY, gains, Fattr, Frep, perm; stride ds per
+// point) and one block-diagonal CSR of P_sym with global column indices.  A
+// job occupies consecutive rows [off, off + n) and nnz [nnz_off, nnz_off + nnz).
+// Jobs are appended at the tail; a removed job leaves a hole (its rows stay
+// valid and are still multiplied, harmlessly) until a compaction moves the
+// later jobs down.  Each job has its own grid / FFT buffers sized for the grid
+// side ng_cap it may reach; the sum of them is bounded by a byte budget.
+//
+// Schedule: the host knows every job's iteration without reading the device
+// (a job spends one batch iteration in INIT, then advances one iteration per
+// batch iteration until max_iter), so it can cut the replay at the locality
+// re-orders of each job (its iterations 50, 150, 300, 600) without syncing.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "batch.hpp"
+#include "../../include/sgtsne.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+using namespace sgt;
+
+namespace {
+
+constexpr int kReorderIters[] = {50, 150, 300, 600};
+
+struct HostJob {
+  bool live = false;
+  int off = 0, n = 0;
+  int64_t nnz_off = 0, nnz = 0;
+  int max_iter = 0;
+  int64_t t = 0;         // batch iterations run since admission (INIT included)
+  int ng_cap = 0;
+  size_t gbytes = 0;
+  void *gmem = nullptr;  // vfix | phi | W | Kt1 | Kt2 | Kh
+  int reorders = 0;
+};
+
+}  // namespace
+
+struct sgtsne_batch {
+  sgtsne_params p{};
+  int d = 2, ds = 2;
+  hipStream_t stream = nullptr;  // iterations and every arena write
+  hipStream_t prep = nullptr;    // front end of new jobs (kNN, perplexity, P + P^T)
+  hipEvent_t ev_prep = nullptr;
+  int max_jobs = 0;
+  int64_t max_points = 0, max_nnz = 0;
+  size_t grid_budget = 0, grid_used = 0;
+
+  // arena
+  int32_t *rowptr = nullptr, *col = nullptr;
+  float *val = nullptr;
+  float *Y = nullptr, *uY = nullptr, *gains = nullptr, *Fattr = nullptr, *Frep = nullptr;
+  int32_t *perm = nullptr;  // device row -> the caller's index within its job
+  int64_t used_points = 0, used_nnz = 0;
+
+  // job table and work items
+  BatchJob *jobs_d = nullptr;
+  std::vector<BatchJob> jobs_h;  // static part mirrored on the host
+  std::vector<HostJob> hj;
+  std::vector<int> arena;        // live slots in ascending off
+  BatchItem *items_d = nullptr;
+  int items_cap = 0, nitems = 0;
+  double *zpart = nullptr, *spart = nullptr, *mpart = nullptr;
+  void *args_d = nullptr;
+  int *prefix_d = nullptr;
+  float2 *tw = nullptr;
+  double *scal_d = nullptr;
+
+  // launch shapes (autotuned for the attractive term; the others are
+  // persistent work-group counts)
+  int spmv_group = 16, spmv_unroll = 2, spmv_rpb = 0;
+  int64_t tuned_points = 0;
+  std::vector<double> tune_ms;
+  int s2g_blocks = 1024, s2g_lds = 32768, conv_blocks = 1024, pt_blocks = 2048;
+
+  // captured iteration
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+  bool use_graph = true;
+  int64_t graph_replays = 0;
+};
+
+namespace {
+
+int balloc(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return SGTSNE_ERR_NOMEM;
+  }
+  return SGTSNE_OK;
+}
+
+template <class T>
+void bfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+double bdefault_h(int d) { return d == 1 ? 0.5 : d == 2 ? 0.7 : 1.2; }
+
+// Largest grid side a job may reach: the embedding extent grows with N, so
+// small jobs get small grid buffers.
+int auto_ng_cap(int d, int n) {
+  if (d == 1) return 512;
+  if (d == 2) return n <= 20000 ? 256 : 512;
+  return n <= 20000 ? 64 : 128;
+}
+
+void invalidate_graph(sgtsne_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_valid = false;
+}
+
+// Writes the static part of slot j's record (everything before the dynamic
+// fields) so that a running job's iteration state is left alone.
+int write_static(sgtsne_batch *b, int j) {
+  SGT_HIP(hipMemcpyAsync(b->jobs_d + j, &b->jobs_h[j], offsetof(BatchJob, iter), hipMemcpyHostToDevice, b->stream));
+  return SGTSNE_OK;
+}
+
+int rebuild_items(sgtsne_batch *b) {
+  std::vector<BatchItem> items;
+  for (int j : b->arena) {
+    HostJob &h = b->hj[j];
+    BatchJob &J = b->jobs_h[j];
+    J.item0 = (int)items.size();
+    for (int s = 0; s < h.n; s += kBatchChunk)
+      items.push_back(BatchItem{j, h.off + s, h.off + std::min(h.n, s + kBatchChunk), 0});
+    J.nitems = (int)items.size() - J.item0;
+    J.off = h.off;
+    J.n = h.n;
+  }
+  SGT_CHECK((int)items.size() <= b->items_cap, SGTSNE_ERR_ARG, "work items exceed capacity");
+  b->nitems = (int)items.size();
+  // the H2D copies read pageable host memory: wait for them before it goes away
+  if (!items.empty())
+    SGT_HIP(hipMemcpyAsync(b->items_d, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                           b->stream));
+  for (int j : b->arena) SGT_TRY(write_static(b, j));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  invalidate_graph(b);
+  return SGTSNE_OK;
+}
+
+int attractive(sgtsne_batch *b) {
+  const int n = (int)b->used_points;
+  if (n == 0) return SGTSNE_OK;
+  if (b->spmv_rpb > 0)
+    return launch_attr_csr_blk(b->stream, b->d, b->spmv_group, b->spmv_unroll, b->spmv_rpb, n, b->rowptr, b->col,
+                               b->val, b->Y, b->Fattr);
+  return launch_attr_csr_seg2(b->stream, b->d, b->spmv_group, b->spmv_unroll, n, b->rowptr, b->col, b->val, b->Y,
+                              b->Fattr);
+}
+
+// The gradient terms of every running job at its current Y (no update).
+int enqueue_terms(sgtsne_batch *b) {
+  const int zd = b->p.zeta_mode == SGTSNE_ZETA_DIRECT;
+  SGT_TRY(attractive(b));
+  SGT_TRY(launch_bs2g(b->stream, b->d, b->jobs_d, b->items_d, b->nitems, b->Y, std::min(b->nitems, b->s2g_blocks),
+                      b->s2g_lds));
+  SGT_TRY(launch_bconv(b->stream, b->d, zd, b->args_d, b->prefix_d, b->max_jobs, b->conv_blocks));
+  SGT_TRY(launch_bg2s(b->stream, b->d, zd, b->jobs_d, b->items_d, b->nitems, b->Y, b->Frep, b->zpart,
+                      std::min(b->nitems, b->pt_blocks)));
+  return SGTSNE_OK;
+}
+
+int enqueue_iteration(sgtsne_batch *b) {
+  const int blocks = std::min(b->nitems, b->pt_blocks);
+  SGT_TRY(enqueue_terms(b));
+  SGT_TRY(launch_bupdate(b->stream, b->d, b->jobs_d, b->items_d, b->nitems, b->Y, b->uY, b->gains, b->Fattr,
+                         b->Frep, b->zpart, b->spart, blocks));
+  SGT_TRY(launch_bcenter(b->stream, b->d, b->jobs_d, b->items_d, b->nitems, b->Y, b->spart, b->mpart, blocks));
+  SGT_TRY(launch_bplan(b->stream, b->d, b->jobs_d, b->max_jobs, b->mpart, b->args_d, b->prefix_d, b->tw));
+  return SGTSNE_OK;
+}
+
+int capture(sgtsne_batch *b) {
+  invalidate_graph(b);
+  if (!b->use_graph || b->nitems == 0) return SGTSNE_OK;
+  SGT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int st = enqueue_iteration(b);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  SGT_TRY(st);
+  if (e != hipSuccess || g == nullptr) {
+    set_error("graph capture failed: %s", hipGetErrorString(e));
+    return SGTSNE_ERR_HIP;
+  }
+  b->graph = g;
+  SGT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_valid = true;
+  return SGTSNE_OK;
+}
+
+int run_iterations(sgtsne_batch *b, int64_t count) {
+  if (count <= 0 || b->nitems == 0) return SGTSNE_OK;
+  if (b->use_graph && !b->graph_valid) SGT_TRY(capture(b));
+  for (int64_t k = 0; k < count; ++k) {
+    if (b->graph_valid) {
+      SGT_HIP(hipGraphLaunch(b->exec, b->stream));
+      b->graph_replays++;
+    } else {
+      SGT_TRY(enqueue_iteration(b));
+    }
+  }
+  for (int j : b->arena) {
+    HostJob &h = b->hj[j];
+    const int64_t end = (int64_t)h.max_iter + 1;
+    h.t = std::min<int64_t>(h.t + count, end);
+  }
+  return SGTSNE_OK;
+}
+
+// Attractive-kernel shape for the current arena.  The lanes per row stay
+// fixed (16): every candidate then sums each row in the same order, so the
+// choice never changes a result; only rows per work-group and unroll vary.
+int tune(sgtsne_batch *b) {
+  if (b->used_points < 4096) return SGTSNE_OK;
+  struct Cand { int unroll, rpb; };
+  const Cand cands[] = {{1, 0}, {2, 0}, {4, 0}, {2, 32}, {2, 64}, {1, 64}, {2, 128}, {4, 128}, {2, 256}};
+  hipEvent_t e0, e1;
+  SGT_HIP(hipEventCreate(&e0));
+  SGT_HIP(hipEventCreate(&e1));
+  b->tune_ms.clear();
+  float best = 1e30f;
+  Cand pick{b->spmv_unroll, b->spmv_rpb};
+  const Cand keep = pick;
+  int st = SGTSNE_OK;
+  for (const Cand &c : cands) {
+    b->spmv_unroll = c.unroll;
+    b->spmv_rpb = c.rpb;
+    st = attractive(b);  // warm
+    if (st) break;
+    (void)hipEventRecord(e0, b->stream);
+    for (int r = 0; r < 3 && !st; ++r) st = attractive(b);
+    (void)hipEventRecord(e1, b->stream);
+    if (st) break;
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= 3.0f;
+    b->tune_ms.push_back(ms);
+    if (ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    b->spmv_unroll = keep.unroll;
+    b->spmv_rpb = keep.rpb;
+    return st;
+  }
+  b->spmv_unroll = pick.unroll;
+  b->spmv_rpb = pick.rpb;
+  b->tuned_points = b->used_points;
+  invalidate_graph(b);
+  return SGTSNE_OK;
+}
+
+// Moves the live jobs down over the holes (stream-ordered, through a staging
+// buffer so that overlapping moves are safe).
+int compact(sgtsne_batch *b) {
+  int64_t cur_p = 0, cur_z = 0;
+  size_t need = 0;
+  for (int j : b->arena) {
+    const HostJob &h = b->hj[j];
+    need = std::max(need, std::max((size_t)h.n * b->ds * sizeof(float), (size_t)(h.nnz + h.n + 1) * sizeof(int32_t)));
+  }
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, std::max<size_t>(need, 16), b->stream));
+  for (int j : b->arena) {
+    HostJob &h = b->hj[j];
+    if (h.off != cur_p || h.nnz_off != cur_z) {
+      const size_t pb = (size_t)h.n * b->ds * sizeof(float);
+      float *arr[3] = {b->Y, b->uY, b->gains};
+      for (float *a : arr) {
+        SGT_HIP(hipMemcpyAsync(tmp, a + (size_t)h.off * b->ds, pb, hipMemcpyDeviceToDevice, b->stream));
+        SGT_HIP(hipMemcpyAsync(a + (size_t)cur_p * b->ds, tmp, pb, hipMemcpyDeviceToDevice, b->stream));
+      }
+      int32_t *ti = static_cast<int32_t *>(tmp);
+      SGT_HIP(hipMemcpyAsync(ti, b->perm + h.off, (size_t)h.n * 4, hipMemcpyDeviceToDevice, b->stream));
+      SGT_HIP(hipMemcpyAsync(b->perm + cur_p, ti, (size_t)h.n * 4, hipMemcpyDeviceToDevice, b->stream));
+      SGT_HIP(hipMemcpyAsync(ti, b->rowptr + h.off, (size_t)(h.n + 1) * 4, hipMemcpyDeviceToDevice, b->stream));
+      SGT_TRY(launch_offset_copy_i32(b->stream, ti, b->rowptr + cur_p, h.n + 1, (int32_t)(cur_z - h.nnz_off)));
+      SGT_HIP(hipMemcpyAsync(ti, b->col + h.nnz_off, (size_t)h.nnz * 4, hipMemcpyDeviceToDevice, b->stream));
+      SGT_TRY(launch_offset_copy_i32(b->stream, ti, b->col + cur_z, h.nnz, (int32_t)(cur_p - h.off)));
+      SGT_HIP(hipMemcpyAsync(tmp, b->val + h.nnz_off, (size_t)h.nnz * 4, hipMemcpyDeviceToDevice, b->stream));
+      SGT_HIP(hipMemcpyAsync(b->val + cur_z, tmp, (size_t)h.nnz * 4, hipMemcpyDeviceToDevice, b->stream));
+      h.off = (int)cur_p;
+      h.nnz_off = cur_z;
+    }
+    cur_p += h.n;
+    cur_z += h.nnz;
+  }
+  SGT_HIP(hipFreeAsync(tmp, b->stream));
+  b->used_points = cur_p;
+  b->used_nnz = cur_z;
+  return rebuild_items(b);
+}
+
+// Morton re-order of one job's points (its rows of P, its Y / uY / gains and
+// the device->caller map), entirely on the device.
+int reorder_job(sgtsne_batch *b, int j) {
+  HostJob &h = b->hj[j];
+  const int n = h.n;
+  const int64_t nnz = h.nnz;
+  const hipStream_t s = b->stream;
+  uint32_t *k0 = nullptr, *k1 = nullptr;
+  int32_t *i0 = nullptr, *order = nullptr, *inv = nullptr, *rpl = nullptr, *coll = nullptr, *rp2 = nullptr,
+          *col2 = nullptr, *itmp = nullptr;
+  float *val2 = nullptr, *ftmp = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&k0, sizeof(uint32_t) * n, s));
+  SGT_HIP(hipMallocAsync((void **)&k1, sizeof(uint32_t) * n, s));
+  SGT_HIP(hipMallocAsync((void **)&i0, sizeof(int32_t) * n, s));
+  SGT_HIP(hipMallocAsync((void **)&order, sizeof(int32_t) * n, s));
+  SGT_HIP(hipMallocAsync((void **)&inv, sizeof(int32_t) * n, s));
+  SGT_TRY(launch_bmorton(s, b->d, b->jobs_d, j, h.off, n, b->Y, k0, i0));
+  size_t tb = 0;
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, order, n, 0, 32, s));
+  void *tmp = nullptr;
+  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, order, n, 0, 32, s));
+  SGT_HIP(hipFreeAsync(tmp, s));
+  SGT_TRY(launch_invert(s, n, order, inv));
+  // the job's rows as a local CSR, permuted, written back with offsets
+  SGT_HIP(hipMallocAsync((void **)&rpl, sizeof(int32_t) * (n + 1), s));
+  SGT_HIP(hipMallocAsync((void **)&coll, sizeof(int32_t) * std::max<int64_t>(nnz, 1), s));
+  SGT_HIP(hipMallocAsync((void **)&rp2, sizeof(int32_t) * (n + 1), s));
+  SGT_HIP(hipMallocAsync((void **)&col2, sizeof(int32_t) * std::max<int64_t>(nnz, 1), s));
+  SGT_HIP(hipMallocAsync((void **)&val2, sizeof(float) * std::max<int64_t>(nnz, 1), s));
+  SGT_TRY(launch_offset_copy_i32(s, b->rowptr + h.off, rpl, n + 1, (int32_t)(-h.nnz_off)));
+  SGT_TRY(launch_offset_copy_i32(s, b->col + h.nnz_off, coll, nnz, -h.off));
+  SGT_TRY(launch_permute_csr(s, n, nnz, order, inv, rpl, coll, b->val + h.nnz_off, rp2, col2, val2));
+  SGT_TRY(launch_offset_copy_i32(s, rp2, b->rowptr + h.off, n + 1, (int32_t)h.nnz_off));
+  SGT_TRY(launch_offset_copy_i32(s, col2, b->col + h.nnz_off, nnz, h.off));
+  if (nnz > 0)
+    SGT_HIP(hipMemcpyAsync(b->val + h.nnz_off, val2, sizeof(float) * nnz, hipMemcpyDeviceToDevice, s));
+  // point state
+  SGT_HIP(hipMallocAsync((void **)&ftmp, sizeof(float) * (size_t)n * b->ds, s));
+  float *arr[3] = {b->Y, b->uY, b->gains};
+  for (float *a : arr) {
+    SGT_TRY(launch_gather_points(s, n, b->ds, order, a + (size_t)h.off * b->ds, ftmp));
+    SGT_HIP(hipMemcpyAsync(a + (size_t)h.off * b->ds, ftmp, sizeof(float) * (size_t)n * b->ds,
+                           hipMemcpyDeviceToDevice, s));
+  }
+  SGT_HIP(hipMallocAsync((void **)&itmp, sizeof(int32_t) * n, s));
+  SGT_TRY(launch_gather_i32(s, n, order, b->perm + h.off, itmp));
+  SGT_HIP(hipMemcpyAsync(b->perm + h.off, itmp, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  void *fr[] = {k0, k1, i0, order, inv, rpl, coll, rp2, col2, val2, ftmp, itmp};
+  for (void *q : fr) SGT_HIP(hipFreeAsync(q, s));
+  h.reorders++;
+  return SGTSNE_OK;
+}
+
+// Iterations until the next job reaches a re-order point (or `limit`).
+int64_t next_event(const sgtsne_batch *b, int64_t limit) {
+  int64_t best = limit;
+  if (b->p.reorder < 2) return best;
+  for (int j : b->arena) {
+    const HostJob &h = b->hj[j];
+    for (int r : kReorderIters) {
+      const int64_t te = (int64_t)r + 1;  // before the job's iteration r runs
+      if (r < h.max_iter && te > h.t) {
+        best = std::min(best, te - h.t);
+        break;
+      }
+    }
+  }
+  return best;
+}
+
+int do_events(sgtsne_batch *b) {
+  if (b->p.reorder < 2) return SGTSNE_OK;
+  for (int j : b->arena) {
+    const HostJob &h = b->hj[j];
+    for (int r : kReorderIters)
+      if (r < h.max_iter && h.t == (int64_t)r + 1) SGT_TRY(reorder_job(b, j));
+  }
+  return SGTSNE_OK;
+}
+
+int check_batch(sgtsne_batch *b) {
+  SGT_CHECK(b != nullptr, SGTSNE_ERR_ARG, "batch is NULL");
+  return SGTSNE_OK;
+}
+
+int check_job(sgtsne_batch *b, int job) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(job >= 0 && job < b->max_jobs && b->hj[job].live, SGTSNE_ERR_ARG, "no job %d", job);
+  return SGTSNE_OK;
+}
+
+// Places a job whose symmetric P (local CSR on the device, rows starting at 0)
+// and initial Y (host, caller order, layout per params) are given.
+int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_t *col_d, const float *val_d,
+          const float *y_in, int max_iter, int ng_cap, int *job_out) {
+  SGT_CHECK(n > 0 && nnz >= 0, SGTSNE_ERR_ARG, "bad job size");
+  int slot = -1;
+  for (int j = 0; j < b->max_jobs; ++j)
+    if (!b->hj[j].live) { slot = j; break; }
+  SGT_CHECK(slot >= 0, SGTSNE_ERR_FULL, "all %d job slots are in use", b->max_jobs);
+  int64_t live_p = 0, live_z = 0;
+  for (int j : b->arena) { live_p += b->hj[j].n; live_z += b->hj[j].nnz; }
+  SGT_CHECK(live_p + n <= b->max_points && live_z + nnz <= b->max_nnz, SGTSNE_ERR_FULL,
+            "arena full (%lld + %d points, %lld + %lld nnz)", (long long)live_p, n, (long long)live_z,
+            (long long)nnz);
+  if (ng_cap <= 0) ng_cap = auto_ng_cap(b->d, n);
+  if (b->p.grid_mode == SGTSNE_GRID_FIXED) ng_cap = std::max(ng_cap, b->p.ng + 2);
+  SGT_CHECK(ng_cap >= 16 && ng_cap <= 512, SGTSNE_ERR_ARG, "grid capacity %d outside [16, 512]", ng_cap);
+  size_t vb, pb, wb, kb;
+  const size_t gbytes = batch_grid_bytes(b->d, ng_cap, &vb, &pb, &wb, &kb);
+  SGT_CHECK(b->grid_used + gbytes <= b->grid_budget, SGTSNE_ERR_FULL, "grid budget exhausted");
+  if (b->used_points + n > b->max_points || b->used_nnz + nnz > b->max_nnz) SGT_TRY(compact(b));
+
+  HostJob &h = b->hj[slot];
+  h = HostJob{};
+  h.off = (int)b->used_points;
+  h.n = n;
+  h.nnz_off = b->used_nnz;
+  h.nnz = nnz;
+  h.max_iter = max_iter;
+  h.ng_cap = ng_cap;
+  h.gbytes = gbytes;
+  SGT_HIP(hipMallocAsync(&h.gmem, gbytes, b->stream));
+  SGT_HIP(hipMemsetAsync(h.gmem, 0, vb, b->stream));  // the spread grid starts (and stays) zero
+  char *gm = static_cast<char *>(h.gmem);
+
+  // P rows into the arena (global row offsets and column indices)
+  SGT_TRY(launch_offset_copy_i32(b->stream, rp_d, b->rowptr + h.off, n + 1, (int32_t)h.nnz_off));
+  SGT_TRY(launch_offset_copy_i32(b->stream, col_d, b->col + h.nnz_off, nnz, h.off));
+  if (nnz > 0)
+    SGT_HIP(hipMemcpyAsync(b->val + h.nnz_off, val_d, sizeof(float) * nnz, hipMemcpyDeviceToDevice, b->stream));
+
+  // Y (caller order -> AoS stride ds), centred by the admission mean
+  const int d = b->d, ds = b->ds;
+  std::vector<float> yh((size_t)n * ds, 0.0f);
+  std::vector<float> y0;
+  const float *yi = y_in;
+  if (!yi) {
+    std::mt19937_64 gen(b->p.seed + 0x9e3779b97f4a7c15ull * (uint64_t)(slot + 1));
+    std::normal_distribution<double> nd(0.0, 1.0);
+    y0.resize((size_t)n * d);
+    for (auto &v : y0) v = (float)(1e-4 * nd(gen));
+    yi = y0.data();
+  }
+  const bool soa = b->p.layout == SGTSNE_LAYOUT_SOA && y_in != nullptr;
+  double mean[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < d; ++a) {
+      const float v = soa ? yi[(size_t)a * n + i] : yi[(size_t)i * d + a];
+      yh[(size_t)i * ds + a] = v;
+      mean[a] += v;
+    }
+  BatchJob &J = b->jobs_h[slot];
+  J = BatchJob{};
+  for (int a = 0; a < 4; ++a) J.mean0[a] = a < d ? (float)(mean[a] / n) : 0.0f;
+  SGT_HIP(hipMemcpyAsync(b->Y + (size_t)h.off * ds, yh.data(), yh.size() * sizeof(float), hipMemcpyHostToDevice,
+                         b->stream));
+  SGT_TRY(launch_fill(b->stream, b->uY + (size_t)h.off * ds, (int64_t)n * ds, 0.0f));
+  SGT_TRY(launch_fill(b->stream, b->gains + (size_t)h.off * ds, (int64_t)n * ds, 1.0f));
+  SGT_TRY(launch_iota(b->stream, n, b->perm + h.off));
+
+  J.off = h.off;
+  J.n = n;
+  J.max_iter = max_iter;
+  J.early_iter = b->p.early_iter;
+  J.grid_mode = b->p.grid_mode;
+  J.ng_fixed = b->p.ng;
+  J.ng_cap = ng_cap;
+  J.zdirect = b->p.zeta_mode == SGTSNE_ZETA_DIRECT;
+  J.alpha0 = (float)b->p.alpha;
+  J.h = b->p.h > 0 ? b->p.h : bdefault_h(d);
+  J.vfix = reinterpret_cast<long long *>(gm);
+  J.phi = reinterpret_cast<float *>(gm + vb);
+  J.W = reinterpret_cast<float2 *>(gm + vb + pb);
+  J.Kt1 = reinterpret_cast<float2 *>(gm + vb + pb + wb);
+  J.Kt2 = reinterpret_cast<float2 *>(gm + vb + pb + wb + kb);
+  J.Kh = reinterpret_cast<float2 *>(gm + vb + pb + wb + 2 * kb);
+  J.iter = 0;
+  J.status = BJ_INIT;
+  SGT_HIP(hipMemcpyAsync(b->jobs_d + slot, &J, sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
+
+  h.live = true;
+  b->used_points += n;
+  b->used_nnz += nnz;
+  b->grid_used += gbytes;
+  // rowptr[used_points] is this job's end: the next job starts there
+  b->arena.push_back(slot);
+  SGT_TRY(rebuild_items(b));  // synchronises the stream (yh, J are host locals)
+  if (job_out) *job_out = slot;
+  return SGTSNE_OK;
+}
+
+void destroy_all(sgtsne_batch *b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  invalidate_graph(b);
+  for (auto &h : b->hj)
+    if (h.gmem) (void)hipFree(h.gmem);
+  bfree(b->rowptr);
+  bfree(b->col);
+  bfree(b->val);
+  bfree(b->Y);
+  bfree(b->uY);
+  bfree(b->gains);
+  bfree(b->Fattr);
+  bfree(b->Frep);
+  bfree(b->perm);
+  bfree(b->jobs_d);
+  bfree(b->items_d);
+  bfree(b->zpart);
+  bfree(b->spart);
+  bfree(b->mpart);
+  bfree(b->args_d);
+  bfree(b->prefix_d);
+  bfree(b->tw);
+  bfree(b->scal_d);
+  if (b->ev_prep) (void)hipEventDestroy(b->ev_prep);
+  if (b->prep) (void)hipStreamDestroy(b->prep);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+// Reads one job's rows of an arena array back in the caller's order.
+int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int layout) {
+  const HostJob &h = b->hj[job];
+  const int n = h.n, d = b->d, ds = b->ds;
+  std::vector<float> v((size_t)n * ds);
+  std::vector<int32_t> pm(n);
+  SGT_HIP(hipMemcpyAsync(v.data(), src + (size_t)h.off * ds, v.size() * sizeof(float), hipMemcpyDeviceToHost,
+                         b->stream));
+  SGT_HIP(hipMemcpyAsync(pm.data(), b->perm + h.off, sizeof(int32_t) * n, hipMemcpyDeviceToHost, b->stream));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  for (int r = 0; r < n; ++r) {
+    const int i = pm[r];
+    for (int a = 0; a < d; ++a) {
+      const float x = v[(size_t)r * ds + a];
+      if (layout == SGTSNE_LAYOUT_SOA) out[(size_t)a * n + i] = x;
+      else out[(size_t)i * d + a] = x;
+    }
+  }
+  return SGTSNE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max_jobs, int64_t max_points,
+                        int64_t max_nnz, int64_t grid_budget_bytes) {
+  SGT_CHECK(out && params, SGTSNE_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  SGT_CHECK(params->d >= 1 && params->d <= 3, SGTSNE_ERR_ARG, "d must be 1..3");
+  SGT_CHECK(max_jobs >= 1 && max_jobs <= 1024, SGTSNE_ERR_ARG, "max_jobs must be 1..1024");
+  SGT_CHECK(max_points >= 1 && max_points < (int64_t)INT32_MAX / 4, SGTSNE_ERR_ARG, "bad max_points");
+  SGT_CHECK(max_nnz >= 0 && max_nnz < (int64_t)INT32_MAX, SGTSNE_ERR_ARG, "max_nnz exceeds the int32 range");
+  SGT_CHECK(params->zeta_mode == SGTSNE_ZETA_DIRECT || params->zeta_mode == SGTSNE_ZETA_ALGEBRAIC, SGTSNE_ERR_ARG,
+            "bad zeta_mode");
+  SGT_CHECK(params->grid_mode == SGTSNE_GRID_H || params->grid_mode == SGTSNE_GRID_FIXED, SGTSNE_ERR_ARG,
+            "bad grid_mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no GPU visible");
+    return SGTSNE_ERR_NODEV;
+  }
+  sgtsne_batch *b = new sgtsne_batch();
+  b->p = *params;
+  b->d = params->d;
+  b->ds = point_stride(params->d);
+  b->max_jobs = max_jobs;
+  b->max_points = max_points;
+  b->max_nnz = max_nnz;
+  b->grid_budget = grid_budget_bytes > 0 ? (size_t)grid_budget_bytes : ((size_t)64 << 30);
+  b->hj.resize(max_jobs);
+  b->jobs_h.resize(max_jobs);
+  b->items_cap = (int)(max_points / kBatchChunk + max_jobs + 1);
+  const int np = batch_npasses(b->d);
+  int st = SGTSNE_OK;
+  auto chk = [&](hipError_t e, const char *what) {
+    if (st == SGTSNE_OK && e != hipSuccess) {
+      set_error("%s failed: %s", what, hipGetErrorString(e));
+      st = SGTSNE_ERR_HIP;
+    }
+  };
+  chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+  chk(hipStreamCreateWithFlags(&b->prep, hipStreamNonBlocking), "stream");
+  chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
+  const size_t P = (size_t)max_points;
+  if (!st) st = balloc((void **)&b->rowptr, sizeof(int32_t) * (P + 1));
+  if (!st) st = balloc((void **)&b->col, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1));
+  if (!st) st = balloc((void **)&b->val, sizeof(float) * std::max<int64_t>(max_nnz, 1));
+  float **pts[5] = {&b->Y, &b->uY, &b->gains, &b->Fattr, &b->Frep};
+  for (float **q : pts)
+    if (!st) st = balloc((void **)q, sizeof(float) * P * b->ds);
+  if (!st) st = balloc((void **)&b->perm, sizeof(int32_t) * P);
+  if (!st) st = balloc((void **)&b->jobs_d, sizeof(BatchJob) * max_jobs);
+  if (!st) st = balloc((void **)&b->items_d, sizeof(BatchItem) * b->items_cap);
+  if (!st) st = balloc((void **)&b->zpart, sizeof(double) * 2 * b->items_cap);
+  if (!st) st = balloc((void **)&b->spart, sizeof(double) * 4 * b->items_cap);
+  if (!st) st = balloc((void **)&b->mpart, sizeof(double) * 8 * b->items_cap);
+  if (!st) st = balloc(&b->args_d, batch_pass_args_bytes() * np * max_jobs);
+  if (!st) st = balloc((void **)&b->prefix_d, sizeof(int) * np * (max_jobs + 1));
+  if (!st) st = balloc((void **)&b->tw, 512 * sizeof(float2));
+  if (!st) st = balloc((void **)&b->scal_d, 8 * sizeof(double));
+  if (!st) chk(hipMemsetAsync(b->rowptr, 0, sizeof(int32_t) * (P + 1), b->stream), "memset");
+  if (!st) chk(hipMemsetAsync(b->col, 0, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1), b->stream), "memset");
+  if (!st) chk(hipMemsetAsync(b->val, 0, sizeof(float) * std::max<int64_t>(max_nnz, 1), b->stream), "memset");
+  for (float **q : pts)
+    if (!st) chk(hipMemsetAsync(*q, 0, sizeof(float) * P * b->ds, b->stream), "memset");
+  if (!st) chk(hipMemsetAsync(b->jobs_d, 0, sizeof(BatchJob) * max_jobs, b->stream), "memset");
+  if (!st) chk(hipMemsetAsync(b->prefix_d, 0, sizeof(int) * np * (max_jobs + 1), b->stream), "memset");
+  if (!st) st = init_twiddles(b->stream, b->tw);
+  if (!st) chk(hipStreamSynchronize(b->stream), "create");
+  if (st) {
+    destroy_all(b);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_destroy(sgtsne_batch *b) {
+  if (!b) return SGTSNE_OK;
+  destroy_all(b);
+  delete b;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                     const float *val, int symmetrize, const float *y_in, int max_iter, int ng_cap, int *job) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(n > 0 && nnz >= 0 && rowptr && (nnz == 0 || (col && val)), SGTSNE_ERR_ARG, "bad P arguments");
+  SGT_CHECK(nnz < (int64_t)INT32_MAX / 2, SGTSNE_ERR_ARG, "nnz exceeds the int32 index range");
+  SGT_CHECK(rowptr[0] == 0 && rowptr[n] == nnz, SGTSNE_ERR_ARG, "rowptr inconsistent with nnz");
+  for (int i = 0; i < n; i++) SGT_CHECK(rowptr[i] <= rowptr[i + 1], SGTSNE_ERR_ARG, "rowptr decreases at row %d", i);
+  {
+    int64_t bad = 0;
+    for (int64_t t = 0; t < nnz; t++) bad += (col[t] < 0 || col[t] >= n);
+    SGT_CHECK(bad == 0, SGTSNE_ERR_ARG, "%lld column indices out of range [0, %d)", (long long)bad, n);
+  }
+  if (max_iter <= 0) max_iter = b->p.max_iter;
+  const hipStream_t s = b->prep;
+  int32_t *rp = nullptr, *cc = nullptr;
+  float *vv = nullptr;
+  SGT_TRY(balloc((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
+  SGT_TRY(balloc((void **)&cc, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+  SGT_TRY(balloc((void **)&vv, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)));
+  int st = SGTSNE_OK;
+  if (hipMemcpyAsync(rp, rowptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+      (nnz > 0 && (hipMemcpyAsync(cc, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess ||
+                   hipMemcpyAsync(vv, val, sizeof(float) * nnz, hipMemcpyHostToDevice, s) != hipSuccess))) {
+    set_error("copy of P failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  int32_t *srp = rp, *scc = cc;
+  float *svv = vv;
+  int64_t m = nnz;
+  if (!st && symmetrize) {
+    if (b->p.lambda != 1.0) st = launch_lambda_rescale(s, n, rp, vv, b->p.lambda, b->p.drop_leaf);
+    if (!st) st = gpu_symmetrize(s, n, nnz, rp, cc, vv, &srp, &scc, &svv, &m);
+  }
+  if (!st && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("preparation of P failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
+  (void)hipStreamSynchronize(b->stream);
+  if (srp != rp) { bfree(srp); bfree(scc); bfree(svv); }
+  bfree(rp);
+  bfree(cc);
+  bfree(vv);
+  return st;
+}
+
+int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int k, double u, const float *y_in,
+                            int max_iter, int ng_cap, int *job) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(X && n > k && k >= 1 && dim >= 1 && u > 0, SGTSNE_ERR_ARG, "bad kNN arguments");
+  if (max_iter <= 0) max_iter = b->p.max_iter;
+  const hipStream_t s = b->prep;
+  const size_t nx = (size_t)n * dim;
+  float *Xf = nullptr;
+  double *Xd = nullptr, *Dd = nullptr;
+  int32_t *Id = nullptr, *rp = nullptr, *cc = nullptr, *srp = nullptr, *scc = nullptr;
+  float *vv = nullptr, *svv = nullptr;
+  int64_t m = 0;
+  int st = balloc((void **)&Xf, nx * sizeof(float));
+  if (!st) st = balloc((void **)&Xd, nx * sizeof(double));
+  if (!st) st = balloc((void **)&Dd, (size_t)n * (k + 1) * sizeof(double));
+  if (!st) st = balloc((void **)&Id, (size_t)n * (k + 1) * sizeof(int32_t));
+  if (!st) st = balloc((void **)&rp, ((size_t)n + 1) * sizeof(int32_t));
+  if (!st) st = balloc((void **)&cc, (size_t)n * k * sizeof(int32_t));
+  if (!st) st = balloc((void **)&vv, (size_t)n * k * sizeof(float));
+  if (!st && hipMemcpyAsync(Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
+    set_error("copy of X failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (!st) st = launch_to_double(s, Xf, Xd, (int64_t)nx);
+  if (!st) st = launch_knn_exact(s, Xd, n, dim, k, Id, Dd);
+  if (!st) st = launch_perplexity(s, Dd, n, k, u, vv, nullptr);
+  if (!st) st = launch_knn_csr(s, n, k, Id, rp, cc);
+  if (!st && b->p.lambda != 1.0) st = launch_lambda_rescale(s, n, rp, vv, b->p.lambda, b->p.drop_leaf);
+  if (!st) st = gpu_symmetrize(s, n, (int64_t)n * k, rp, cc, vv, &srp, &scc, &svv, &m);
+  if (!st && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("front end failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  bfree(Xf);
+  bfree(Xd);
+  bfree(Dd);
+  bfree(Id);
+  bfree(rp);
+  bfree(cc);
+  bfree(vv);
+  if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
+  (void)hipStreamSynchronize(b->stream);
+  bfree(srp);
+  bfree(scc);
+  bfree(svv);
+  return st;
+}
+
+int sgtsne_batch_step(sgtsne_batch *b, int iters) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(iters >= 0, SGTSNE_ERR_ARG, "iters < 0");
+  if (b->arena.empty()) return SGTSNE_OK;
+  if (b->tuned_points == 0 || b->used_points > 2 * b->tuned_points || 2 * b->used_points < b->tuned_points)
+    SGT_TRY(tune(b));
+  int64_t left = iters;
+  while (left > 0) {
+    const int64_t chunk = next_event(b, left);
+    SGT_TRY(run_iterations(b, chunk));
+    left -= chunk;
+    SGT_TRY(do_events(b));
+  }
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_sync(sgtsne_batch *b) {
+  SGT_TRY(check_batch(b));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_job_status(sgtsne_batch *b, int job, int *status, int *iter) {
+  SGT_TRY(check_job(b, job));
+  int v[2] = {0, 0};
+  SGT_HIP(hipMemcpyAsync(v, &b->jobs_d[job].iter, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  if (iter) *iter = v[0];
+  if (status) {
+    const int s = v[1];
+    *status = s == BJ_DONE ? SGTSNE_JOB_DONE
+              : s == BJ_ERR_NUMERIC ? SGTSNE_ERR_NUMERIC
+              : s == BJ_ERR_GRID ? SGTSNE_ERR_FULL
+              : SGTSNE_JOB_RUNNING;
+  }
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_get_Y(sgtsne_batch *b, int job, float *y, int layout) {
+  SGT_TRY(check_job(b, job));
+  SGT_CHECK(y != nullptr && (layout == SGTSNE_LAYOUT_SOA || layout == SGTSNE_LAYOUT_AOS), SGTSNE_ERR_ARG,
+            "bad arguments");
+  return fetch_points(b, job, b->Y, y, layout);
+}
+
+int sgtsne_batch_terms(sgtsne_batch *b, int job, float *Fattr, float *Frep, double *Z, int *ng, int layout) {
+  SGT_TRY(check_job(b, job));
+  SGT_CHECK(layout == SGTSNE_LAYOUT_SOA || layout == SGTSNE_LAYOUT_AOS, SGTSNE_ERR_ARG, "bad layout");
+  int st_iter[2] = {0, 0};
+  SGT_HIP(hipMemcpyAsync(st_iter, &b->jobs_d[job].iter, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  SGT_CHECK(st_iter[1] == BJ_RUN, SGTSNE_ERR_STATE, "job %d is not iterating (status %d)", job, st_iter[1]);
+  SGT_TRY(enqueue_terms(b));
+  SGT_TRY(launch_bzeta(b->stream, b->jobs_d, job, b->zpart, b->scal_d));
+  double z = 0.0;
+  GridParams g{};
+  SGT_HIP(hipMemcpyAsync(&z, b->scal_d, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  SGT_HIP(hipMemcpyAsync(&g, &b->jobs_d[job].g, sizeof(GridParams), hipMemcpyDeviceToHost, b->stream));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  if (Z) *Z = z;
+  if (ng) *ng = g.ng;
+  if (Fattr) SGT_TRY(fetch_points(b, job, b->Fattr, Fattr, layout));
+  if (Frep) SGT_TRY(fetch_points(b, job, b->Frep, Frep, layout));
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_remove(sgtsne_batch *b, int job) {
+  SGT_TRY(check_job(b, job));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  HostJob &h = b->hj[job];
+  BatchJob empty{};
+  empty.status = BJ_EMPTY;
+  SGT_HIP(hipMemcpyAsync(b->jobs_d + job, &empty, sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
+  SGT_HIP(hipFreeAsync(h.gmem, b->stream));
+  b->grid_used -= h.gbytes;
+  b->arena.erase(std::find(b->arena.begin(), b->arena.end(), job));
+  const bool tail = h.off + h.n == b->used_points;
+  if (tail) {
+    b->used_points = h.off;
+    b->used_nnz = h.nnz_off;
+  }
+  h = HostJob{};
+  b->jobs_h[job] = BatchJob{};
+  // compact when holes hold more than a quarter of the arena
+  int64_t live = 0;
+  for (int j : b->arena) live += b->hj[j].n;
+  if (4 * (b->used_points - live) > b->used_points) SGT_TRY(compact(b));
+  else SGT_TRY(rebuild_items(b));
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_info(sgtsne_batch *b, int *jobs, int64_t *points, int64_t *nnz, int64_t *grid_bytes,
+                      int64_t *graph_replays) {
+  SGT_TRY(check_batch(b));
+  if (jobs) *jobs = (int)b->arena.size();
+  if (points) *points = b->used_points;
+  if (nnz) *nnz = b->used_nnz;
+  if (grid_bytes) *grid_bytes = (int64_t)b->grid_used;
+  if (graph_replays) *graph_replays = b->graph_replays;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_block, int *use_graph) {
+  SGT_TRY(check_batch(b));
+  if (group) *group = b->spmv_group;
+  if (unroll) *unroll = b->spmv_unroll;
+  if (rows_per_block) *rows_per_block = b->spmv_rpb;
+  if (use_graph) *use_graph = b->use_graph ? 1 : 0;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_set_graph(sgtsne_batch *b, int on) {
+  SGT_TRY(check_batch(b));
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  b->use_graph = on != 0;
+  invalidate_graph(b);
+  return SGTSNE_OK;
+}
+
+}  // extern "C"

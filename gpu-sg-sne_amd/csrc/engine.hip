@@ -11,6 +11,7 @@
 // waits for it, so the GPU stays busy across the round trip.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "grid_device.hpp"
 #include "../../include/sgtsne.h"
 
 #include <algorithm>
@@ -34,24 +35,10 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
-// getBestGridSize (HY/qq.cpp:108-130, CU/utils.cu:194-215): the FFT-friendly
-// size table fixes the interpolation grid and must match the reference.
-static const int kGridSizes[] = {
-    8,   9,   10,  11,  12,  13,  14,  15,  16,  20,  25,  26,  28,  32,  33,  35,  36,
-    39,  40,  42,  44,  45,  48,  49,  50,  52,  54,  55,  56,  60,  63,  64,  65,  66,
-    70,  72,  75,  77,  78,  80,  84,  88,  90,  91,  96,  98,  99,  100, 104, 105, 108,
-    110, 112, 117, 120, 125, 126, 130, 132, 135, 140, 144, 147, 150, 154, 156, 160, 165,
-    168, 175, 176, 180, 182, 189, 192, 195, 196, 198, 200, 208, 210, 216, 220, 224, 225,
-    231, 234, 240, 245, 250, 252, 260, 264, 270, 273, 275, 280, 288, 294, 297, 300, 308,
-    312, 315, 320, 325, 330, 336, 343, 350, 351, 352, 360, 364, 375, 378, 385, 390, 392,
-    396, 400, 416, 420, 432, 440, 441, 448, 450, 455, 462, 468, 480, 490, 495, 500, 504,
-    512};
+static const int kGridSizes[] = {SGT_GRID_SIZE_LIST};
 
 int best_grid_size(int nGrid) {
-  const int cnt = (int)(sizeof(kGridSizes) / sizeof(kGridSizes[0]));
-  for (int i = 0; i < cnt; i++)
-    if (nGrid + 2 <= kGridSizes[i]) return kGridSizes[i] - 2;
-  return kGridSizes[cnt - 1] - 2;
+  return best_grid_size_in(kGridSizes, (int)(sizeof(kGridSizes) / sizeof(kGridSizes[0])), nGrid);
 }
 
 // CUSP's compute_optimal_entries_per_row rule (the reference calls it at
@@ -208,20 +195,7 @@ int make_grid(sgtsne_ctx *c, const DevScalars &s, GridParams &g) {
     nGrid = std::max((int)std::ceil(maxy / h), 14);
     nGrid = best_grid_size(nGrid);
   }
-  g.d = d;
-  g.nGrid = nGrid;
-  g.ng = nGrid + 2;
-  int L = 1, lg = 0;
-  while (L < 2 * g.ng - 1) { L <<= 1; ++lg; }
-  g.L = L;
-  g.logL = lg;
-  for (int a = 0; a < 3; a++) {
-    g.minv[a] = a < d ? (float)s.minv[a] : 0.0f;
-    g.center[a] = a < d ? (float)(0.5 * (s.minv[a] + s.maxv[a])) : 0.0f;
-  }
-  g.scale = (float)((double)(nGrid - 1) / maxy);
-  g.gmax = std::nextafter((float)(nGrid - 1), 0.0f);
-  g.hgrid = maxy / ((double)(nGrid - 1) - DBL_EPSILON);
+  grid_from_box(d, nGrid, s.minv, s.maxv, maxy, g);
   return SGTSNE_OK;
 }
 

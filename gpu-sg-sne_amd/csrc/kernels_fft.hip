@@ -1,26 +1,10 @@
-// kernels_fft.hip -- non-periodic grid convolution with the kernel
-// K(r) = (1 + h^2 r^2)^-2 by zero-padded FFTs computed in LDS.
-//
-// The reference evaluates the same Toeplitz (non-periodic) convolution as a
-// sum of 2^d circulant / skew-circulant terms of size ng
-// (HY/non_periodic_conv.cpp:26-708, CU/non_periodic_convF.cu:162-517).  For
-// an FFT length L >= 2 ng - 1 a single circulant of size L with the symmetric
-// embedding K_circ[i] = K(min(i, L - i)) gives exactly the same product on the
-// ng outputs, so here L is the next power of two and every line transform is a
-// radix-2 Stockham FFT held in LDS.  K_circ is even in every axis, so its
-// spectrum is real and even: only its (L/2+1)^d half-spectrum is kept.  Two
-// real grid channels share one complex line (their results are the real and
-// imaginary parts of the inverse transform).  The two kernels K2 = (1+h^2r^2)^-2
-// and K1 = (1+h^2r^2)^-1 are transformed together as K2 + i K1 (both spectra
-// are real), and the unit-charge channel V0 is transformed alone and multiplied
-// by K2^ + i K1^, which returns K2*V0 + i K1*V0: the potential and the direct
-// normalisation term from one line.
-//
-// Layouts: grid V/Phi [m][ng^d] (x fastest, HY/gridding.cpp:22-24);
-// complex work W per channel pair: 2-D [ng][L] (rows y < ng, all kx),
-// 3-D [ng][L][L] (z < ng).
+// kernels_fft.hip -- non-periodic grid convolution of one job with the kernels
+// K2 = (1 + h^2 r^2)^-2 and K1 = (1 + h^2 r^2)^-1 by zero-padded FFTs in LDS
+// (the passes themselves are in fft_device.hpp, shared with the batched
+// engine).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "fft_device.hpp"
 #include "../../include/sgtsne.h"
 
 #include <cmath>
@@ -29,165 +13,12 @@ namespace sgt {
 
 namespace {
 
-constexpr int kFftThreads = 256;
-constexpr int kLdsElems = 4096;  // complex elements per ping-pong buffer (32 KiB)
-constexpr int kTwLog = 10;       // twiddle table built for length 1024
+using namespace fft;
 
-enum InKind { IN_VPAIR = 0, IN_W = 1, IN_KGEN = 2, IN_KT = 3 };
-enum OutKind { OUT_W = 0, OUT_PHIPAIR = 1, OUT_KT = 2 };
-enum MidKind { MID_FWD = 0, MID_INV = 1, MID_CONV = 2 };
-
-struct PassArgs {
-  int L, logL, lpw_log;  // FFT length, lines per work-group = 1 << lpw_log
-  int nlines, ninner;    // line = outer * ninner + inner
-  int64_t i_os, i_is, i_es, pair_in;
-  int n_in;
-  int64_t o_os, o_is, o_es, pair_out;
-  int n_out;
-  float scale;  // scale of OUT_PHIPAIR
-  float hsq;    // h^2 for IN_KGEN
-  int H;        // L / 2 + 1
-  int kdim;     // spectrum index form for MID_CONV
-  int kpow;     // IN_KGEN: 0 -> (K2, K1) pair, 2 -> K2 only
-  int in_c[4][2], out_c[4][2], kcplx[4];  // per-pair channels (-1: none) and spectrum kind
-  const void *in;
-  void *out;
-  const float2 *Kh;
-  const float2 *tw;
-};
-
-__device__ __forceinline__ int mirror(int e, int L) { return e <= (L >> 1) ? e : L - e; }
-
-// Radix-2 Stockham autosort on nl lines of length L (natural order in and
-// out).  Returns the buffer holding the result.
-__device__ float2 *stockham(float2 *x, float2 *y, int L, int logL, int nl, bool inv,
-                            const float2 *__restrict__ tw) {
-  const int half = L >> 1;
-  const int total = nl * half;
-  for (int s = 0; s < logL; ++s) {
-    const int Ns = 1 << s;
-    for (int b = threadIdx.x; b < total; b += blockDim.x) {
-      const int l = b >> (logL - 1);
-      const int j = b & (half - 1);
-      const float2 *src = x + l * L;
-      float2 *dst = y + l * L;
-      float2 a0 = src[j], a1 = src[j + half];
-      const int k = j & (Ns - 1);
-      float2 w = tw[k << (kTwLog - 1 - s)];
-      if (inv) w.y = -w.y;
-      float2 t = make_float2(a1.x * w.x - a1.y * w.y, a1.x * w.y + a1.y * w.x);
-      const int o = ((j - k) << 1) + k;
-      dst[o] = make_float2(a0.x + t.x, a0.y + t.y);
-      dst[o + Ns] = make_float2(a0.x - t.x, a0.y - t.y);
-    }
-    __syncthreads();
-    float2 *tmp = x;
-    x = y;
-    y = tmp;
-  }
-  return x;
-}
-
-template <int IN, int OUT, int MID, bool ECONTIG>
+template <int IN, int OUT, int MID, bool EC>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
-  const int L = a.L, logL = a.logL;
-  const int nl = 1 << a.lpw_log;
-  float2 *b0 = lds;
-  float2 *b1 = lds + nl * L;
-  const int p = blockIdx.y;
-  const int line0 = blockIdx.x * nl;
-  const int total = nl * L;
-
-  // ---- load (zero padding beyond n_in) ----
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int l, e;
-    if (ECONTIG) { l = idx >> logL; e = idx & (L - 1); }
-    else { l = idx & (nl - 1); e = idx >> a.lpw_log; }
-    const int line = line0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (line < a.nlines && e < a.n_in) {
-      const int outer = line / a.ninner;
-      const int inner = line - outer * a.ninner;
-      if (IN == IN_VPAIR) {
-        const float *V = static_cast<const float *>(a.in);
-        const int64_t off = outer * a.i_os + inner * a.i_is + e * a.i_es;
-        v.x = V[(int64_t)a.in_c[p][0] * a.pair_in + off];
-        if (a.in_c[p][1] >= 0) v.y = V[(int64_t)a.in_c[p][1] * a.pair_in + off];
-      } else if (IN == IN_W) {
-        const float2 *W = static_cast<const float2 *>(a.in);
-        v = W[(int64_t)p * a.pair_in + outer * a.i_os + inner * a.i_is + e * a.i_es];
-      } else if (IN == IN_KGEN) {
-        const int me = mirror(e, L);
-        const float r2 = (float)me * me + (float)inner * inner + (float)outer * outer;
-        const float k1 = 1.0f / (1.0f + a.hsq * r2);
-        v.x = k1 * k1;
-        if (a.kpow == 0) v.y = k1;
-      } else {  // IN_KT: (K2, K1) half-spectra, mirrored along the pass axis
-        const float2 *K = static_cast<const float2 *>(a.in);
-        const int me = mirror(e, L);
-        v = K[outer * a.i_os + inner * a.i_is + me * a.i_es];
-      }
-    }
-    b0[l * L + e] = v;
-  }
-  __syncthreads();
-
-  float2 *r;
-  if (MID == MID_INV) {
-    r = stockham(b0, b1, L, logL, nl, true, a.tw);
-  } else {
-    r = stockham(b0, b1, L, logL, nl, false, a.tw);
-  }
-  if (MID == MID_CONV) {
-    // multiply by the real kernel spectrum, then inverse transform
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int l = idx >> logL, e = idx & (L - 1);
-      const int line = line0 + l;
-      float kv = 0.f, kvi = 0.f;
-      if (line < a.nlines) {
-        const int outer = line / a.ninner;
-        const int inner = line - outer * a.ninner;
-        const int me = mirror(e, L);
-        int64_t ki;
-        if (a.kdim == 1) ki = me;
-        else if (a.kdim == 2) ki = (int64_t)me * a.H + mirror(inner, L);
-        else ki = ((int64_t)me * a.H + mirror(outer, L)) * a.H + mirror(inner, L);
-        const float2 k = a.Kh[ki];
-        kv = k.x;
-        kvi = a.kcplx[p] ? k.y : 0.0f;
-      }
-      const float2 v = r[idx];
-      r[idx] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
-    }
-    __syncthreads();
-    float2 *other = (r == b0) ? b1 : b0;
-    r = stockham(r, other, L, logL, nl, true, a.tw);
-  }
-
-  // ---- store ----
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int l, e;
-    if (ECONTIG) { l = idx >> logL; e = idx & (L - 1); }
-    else { l = idx & (nl - 1); e = idx >> a.lpw_log; }
-    const int line = line0 + l;
-    if (line >= a.nlines || e >= a.n_out) continue;
-    const int outer = line / a.ninner;
-    const int inner = line - outer * a.ninner;
-    const int64_t off = outer * a.o_os + inner * a.o_is + e * a.o_es;
-    const float2 v = r[l * L + e];
-    if (OUT == OUT_W) {
-      float2 *W = static_cast<float2 *>(a.out);
-      W[(int64_t)p * a.pair_out + off] = v;
-    } else if (OUT == OUT_PHIPAIR) {
-      float *Phi = static_cast<float *>(a.out);
-      Phi[(int64_t)a.out_c[p][0] * a.pair_out + off] = v.x * a.scale;
-      if (a.out_c[p][1] >= 0) Phi[(int64_t)a.out_c[p][1] * a.pair_out + off] = v.y * a.scale;
-    } else {  // the spectra of real even kernels are real: keep (Re, Im) = (K2^, K1^)
-      float2 *K = static_cast<float2 *>(a.out);
-      K[off] = v;
-    }
-  }
+  pass_body<IN, OUT, MID, EC>(a, blockIdx.x, blockIdx.y, lds);
 }
 
 __global__ void twiddle_kernel(float2 *tw) {
@@ -199,72 +30,28 @@ __global__ void twiddle_kernel(float2 *tw) {
 }
 
 template <int IN, int OUT, int MID, bool EC>
-int run_pass(hipStream_t s, PassArgs a, int pairs) {
-  const int nl = 1 << a.lpw_log;
-  const int blocks = div_up(a.nlines, nl);
+int run_pass(hipStream_t s, const PassArgs &a, int pairs) {
+  const int blocks = pass_blocks(a);
   if (blocks == 0) return SGTSNE_OK;
-  const size_t lds = 2 * (size_t)nl * a.L * sizeof(float2);
+  const size_t lds = 2 * (size_t)(1 << a.lpw_log) * a.L * sizeof(float2);
   dim3 grid(blocks, pairs);
   hipLaunchKernelGGL((fft_pass_kernel<IN, OUT, MID, EC>), grid, dim3(kFftThreads), lds, s, a);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
 
-// Channel plan of one convolution.  zdirect: pair 0 = (V0, -) with the
-// spectrum K2^ + i K1^ -> (Phi0, Psi0 in channel d+1); then the charge
-// channels 1..d in pairs with K2^.  Otherwise: channels 0..d in pairs with K2^.
-struct Plan {
-  int pairs;
-  int in_c[4][2], out_c[4][2], kcplx[4];
-};
-
-Plan make_plan(int d, bool zdirect) {
-  Plan pl{};
-  int p = 0;
-  int c = 0;
-  if (zdirect) {
-    pl.in_c[0][0] = 0; pl.in_c[0][1] = -1;
-    pl.out_c[0][0] = 0; pl.out_c[0][1] = d + 1;
-    pl.kcplx[0] = 1;
-    p = 1;
-    c = 1;
+int run_kind(hipStream_t s, int kind, const PassArgs &a, int pairs) {
+  switch (kind) {
+    case PK_KGEN: return run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, a, pairs);
+    case PK_KT_T: return run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, a, pairs);
+    case PK_V_W: return run_pass<IN_VPAIR, OUT_W, MID_FWD, true>(s, a, pairs);
+    case PK_W_W_CONV: return run_pass<IN_W, OUT_W, MID_CONV, false>(s, a, pairs);
+    case PK_W_PHI: return run_pass<IN_W, OUT_PHIPAIR, MID_INV, true>(s, a, pairs);
+    case PK_V_PHI: return run_pass<IN_VPAIR, OUT_PHIPAIR, MID_CONV, true>(s, a, pairs);
+    case PK_W_W_FWD: return run_pass<IN_W, OUT_W, MID_FWD, false>(s, a, pairs);
+    case PK_W_W_INV: return run_pass<IN_W, OUT_W, MID_INV, false>(s, a, pairs);
+    default: SGT_CHECK(false, SGTSNE_ERR_ARG, "bad FFT pass kind");
   }
-  for (; c <= d; c += 2, ++p) {
-    pl.in_c[p][0] = c;
-    pl.in_c[p][1] = c + 1 <= d ? c + 1 : -1;
-    pl.out_c[p][0] = pl.in_c[p][0];
-    pl.out_c[p][1] = pl.in_c[p][1];
-    pl.kcplx[p] = 0;
-  }
-  pl.pairs = p;
-  return pl;
-}
-
-PassArgs base_args(const GridParams &g, const Plan &pl, const ConvBuffers &b) {
-  PassArgs a{};
-  a.L = g.L;
-  a.logL = g.logL;
-  int lpw = kLdsElems >> g.logL;
-  if (lpw < 1) lpw = 1;
-  if (lpw > 64) lpw = 64;
-  int lg = 0;
-  while ((1 << (lg + 1)) <= lpw) ++lg;
-  a.lpw_log = lg;
-  a.H = g.L / 2 + 1;
-  a.hsq = (float)(g.hgrid * g.hgrid);
-  a.tw = b.tw;
-  a.Kh = b.Kh;
-  a.scale = 1.0f;
-  a.ninner = 1;
-  a.kpow = 0;
-  for (int p = 0; p < 4; ++p) {
-    a.in_c[p][0] = pl.in_c[p][0];
-    a.in_c[p][1] = pl.in_c[p][1];
-    a.out_c[p][0] = pl.out_c[p][0];
-    a.out_c[p][1] = pl.out_c[p][1];
-    a.kcplx[p] = pl.kcplx[p];
-  }
-  return a;
 }
 
 }  // namespace
@@ -287,101 +74,14 @@ int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes)
 
 int launch_conv(hipStream_t s, const GridParams &g, int zdirect, const float *V, float *Phi,
                 const ConvBuffers &b) {
-  const int d = g.d, ng = g.ng, L = g.L, H = L / 2 + 1;
-  const Plan pl = make_plan(d, zdirect != 0);
-  const int pairs = pl.pairs;
-  SGT_CHECK(L <= (1 << kTwLog) && L >= 2 * ng - 1, SGTSNE_ERR_ARG, "FFT length %d invalid for ng %d", L, ng);
-  const int64_t G = d == 1 ? ng : d == 2 ? (int64_t)ng * ng : (int64_t)ng * ng * ng;
-  const int kpow = zdirect ? 0 : 2;
-
-  if (d == 1) {
-    PassArgs k = base_args(g, pl, b);  // kernel spectra
-    k.nlines = 1; k.ninner = 1; k.n_in = L; k.n_out = H; k.o_es = 1; k.kpow = kpow;
-    k.out = b.Kh;
-    SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, k, 1)));
-    PassArgs c = base_args(g, pl, b);
-    c.nlines = 1; c.ninner = 1;
-    c.in = V; c.i_es = 1; c.n_in = ng; c.pair_in = G;
-    c.out = Phi; c.o_es = 1; c.n_out = ng; c.pair_out = G;
-    c.kdim = 1; c.scale = 1.0f / (float)L;
-    SGT_TRY((run_pass<IN_VPAIR, OUT_PHIPAIR, MID_CONV, true>(s, c, pairs)));
-    return SGTSNE_OK;
-  }
-
-  if (d == 2) {
-    PassArgs ka = base_args(g, pl, b);
-    ka.nlines = H; ka.ninner = H; ka.n_in = L; ka.o_is = H; ka.o_es = 1; ka.n_out = H; ka.kpow = kpow;
-    ka.out = b.Kt1;
-    SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, ka, 1)));
-    PassArgs kb = base_args(g, pl, b);
-    kb.nlines = H; kb.ninner = H; kb.in = b.Kt1; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
-    kb.out = b.Kh; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
-    SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kb, 1)));
-
-    const int64_t wp = (int64_t)ng * L;
-    PassArgs pa = base_args(g, pl, b);  // rows: V -> W
-    pa.nlines = ng; pa.ninner = ng;
-    pa.in = V; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
-    pa.out = b.W; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
-    SGT_TRY((run_pass<IN_VPAIR, OUT_W, MID_FWD, true>(s, pa, pairs)));
-    PassArgs pb = base_args(g, pl, b);  // columns: fwd, x spectrum, inv
-    pb.nlines = L; pb.ninner = L;
-    pb.in = b.W; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
-    pb.out = b.W; pb.o_is = 1; pb.o_es = L; pb.n_out = ng; pb.pair_out = wp;
-    pb.kdim = 2;
-    SGT_TRY((run_pass<IN_W, OUT_W, MID_CONV, false>(s, pb, pairs)));
-    PassArgs pc = base_args(g, pl, b);  // rows: inverse -> Phi
-    pc.nlines = ng; pc.ninner = ng;
-    pc.in = b.W; pc.i_is = L; pc.i_es = 1; pc.n_in = L; pc.pair_in = wp;
-    pc.out = Phi; pc.o_is = ng; pc.o_es = 1; pc.n_out = ng; pc.pair_out = G;
-    pc.scale = 1.0f / ((float)L * (float)L);
-    SGT_TRY((run_pass<IN_W, OUT_PHIPAIR, MID_INV, true>(s, pc, pairs)));
-    return SGTSNE_OK;
-  }
-
-  // d == 3
-  const int64_t H2 = (int64_t)H * H, L2 = (int64_t)L * L;
-  PassArgs ka = base_args(g, pl, b);
-  ka.nlines = (int)H2; ka.ninner = H; ka.n_in = L; ka.kpow = kpow;
-  ka.out = b.Kt1; ka.o_os = H2; ka.o_is = H; ka.o_es = 1; ka.n_out = H;
-  SGT_TRY((run_pass<IN_KGEN, OUT_KT, MID_FWD, true>(s, ka, 1)));
-  PassArgs kb = base_args(g, pl, b);
-  kb.nlines = (int)H2; kb.ninner = H; kb.in = b.Kt1; kb.i_os = H2; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
-  kb.out = b.Kt2; kb.o_os = H2; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
-  SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kb, 1)));
-  PassArgs kc = base_args(g, pl, b);
-  kc.nlines = (int)H2; kc.ninner = H; kc.in = b.Kt2; kc.i_os = H; kc.i_is = 1; kc.i_es = H2; kc.n_in = L;
-  kc.out = b.Kh; kc.o_os = H; kc.o_is = 1; kc.o_es = H2; kc.n_out = H;
-  SGT_TRY((run_pass<IN_KT, OUT_KT, MID_FWD, false>(s, kc, 1)));
-
-  const int64_t wp = (int64_t)ng * L2;
-  PassArgs pa = base_args(g, pl, b);  // x: V -> W
-  pa.nlines = ng * ng; pa.ninner = ng;
-  pa.in = V; pa.i_os = (int64_t)ng * ng; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
-  pa.out = b.W; pa.o_os = L2; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
-  SGT_TRY((run_pass<IN_VPAIR, OUT_W, MID_FWD, true>(s, pa, pairs)));
-  PassArgs pb = base_args(g, pl, b);  // y forward (all ky kept)
-  pb.nlines = ng * L; pb.ninner = L;
-  pb.in = b.W; pb.i_os = L2; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
-  pb.out = b.W; pb.o_os = L2; pb.o_is = 1; pb.o_es = L; pb.n_out = L; pb.pair_out = wp;
-  SGT_TRY((run_pass<IN_W, OUT_W, MID_FWD, false>(s, pb, pairs)));
-  PassArgs pc = base_args(g, pl, b);  // z: fwd, x spectrum, inv (z < ng kept)
-  pc.nlines = (int)L2; pc.ninner = L;
-  pc.in = b.W; pc.i_os = L; pc.i_is = 1; pc.i_es = L2; pc.n_in = ng; pc.pair_in = wp;
-  pc.out = b.W; pc.o_os = L; pc.o_is = 1; pc.o_es = L2; pc.n_out = ng; pc.pair_out = wp;
-  pc.kdim = 3;
-  SGT_TRY((run_pass<IN_W, OUT_W, MID_CONV, false>(s, pc, pairs)));
-  PassArgs pd = base_args(g, pl, b);  // y inverse (y < ng kept)
-  pd.nlines = ng * L; pd.ninner = L;
-  pd.in = b.W; pd.i_os = L2; pd.i_is = 1; pd.i_es = L; pd.n_in = L; pd.pair_in = wp;
-  pd.out = b.W; pd.o_os = L2; pd.o_is = 1; pd.o_es = L; pd.n_out = ng; pd.pair_out = wp;
-  SGT_TRY((run_pass<IN_W, OUT_W, MID_INV, false>(s, pd, pairs)));
-  PassArgs pe = base_args(g, pl, b);  // x inverse -> Phi
-  pe.nlines = ng * ng; pe.ninner = ng;
-  pe.in = b.W; pe.i_os = L2; pe.i_is = L; pe.i_es = 1; pe.n_in = L; pe.pair_in = wp;
-  pe.out = Phi; pe.o_os = (int64_t)ng * ng; pe.o_is = ng; pe.o_es = 1; pe.n_out = ng; pe.pair_out = G;
-  pe.scale = 1.0f / ((float)L * (float)L * (float)L);
-  SGT_TRY((run_pass<IN_W, OUT_PHIPAIR, MID_INV, true>(s, pe, pairs)));
+  SGT_CHECK(g.L <= (1 << kTwLog) && g.L >= 2 * g.ng - 1, SGTSNE_ERR_ARG, "FFT length %d invalid for ng %d", g.L,
+            g.ng);
+  ConvPtrs ptr{V, Phi, b.W, b.Kt1, b.Kt2, b.Kh, b.tw};
+  PassArgs args[kMaxPasses];
+  int pairs[kMaxPasses], kinds[kMaxPasses];
+  const int np = conv_setup(g, zdirect != 0, ptr, args, pairs);
+  pass_kinds(g.d, kinds);
+  for (int i = 0; i < np; ++i) SGT_TRY(run_kind(s, kinds[i], args[i], pairs[i]));
   return SGTSNE_OK;
 }
 

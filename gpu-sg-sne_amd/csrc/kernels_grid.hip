@@ -15,6 +15,7 @@
 // so no kernel needs a device-wide fence.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "grid_device.hpp"
 #include "../../include/sgtsne.h"
 
 namespace sgt {
@@ -24,58 +25,6 @@ namespace {
 constexpr int kBlock = 256;
 
 enum FinMode { FIN_MINMAX = 0, FIN_ZNEU = 1, FIN_MEAN = 2, FIN_ZEXACT = 3, FIN_KL = 4 };
-
-// Lagrange cubic weights for the 4 stencil nodes f..f+3 of a point at
-// f + 1 + delta (HY/gridding.cpp:28-40: g1 at distance < 1, g2 at 1..2).
-__device__ __forceinline__ float lg1(float t) { return ((0.5f * t - 1.0f) * t - 0.5f) * t + 1.0f; }
-__device__ __forceinline__ float lg2(float t) {
-  return ((-(1.0f / 6.0f) * t + 1.0f) * t - (11.0f / 6.0f)) * t + 1.0f;
-}
-
-template <int D>
-__device__ __forceinline__ void stencil(const GridParams &g, const float (&y)[D], int (&f)[D],
-                                        float (&w)[D][4]) {
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    float t = (y[a] - g.minv[a]) * g.scale;
-    t = fminf(fmaxf(t, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
-    int fi = (int)t;
-    float dl = t - (float)fi;
-    f[a] = fi;
-    w[a][0] = lg2(1.0f + dl);
-    w[a][1] = lg1(dl);
-    w[a][2] = lg1(1.0f - dl);
-    w[a][3] = lg2(2.0f - dl);
-  }
-}
-
-// Visits the 4^D stencil nodes of one point: fn(linear node index, weight).
-template <int D, class Fn>
-__device__ __forceinline__ void for_stencil(int ng, const int (&f)[D], const float (&w)[D][4], Fn fn) {
-  if constexpr (D == 1) {
-#pragma unroll
-    for (int i0 = 0; i0 < 4; ++i0) fn(f[0] + i0, w[0][i0]);
-  } else if constexpr (D == 2) {
-#pragma unroll
-    for (int i1 = 0; i1 < 4; ++i1)
-#pragma unroll
-      for (int i0 = 0; i0 < 4; ++i0) fn((f[1] + i1) * ng + f[0] + i0, w[1][i1] * w[0][i0]);
-  } else {
-#pragma unroll
-    for (int i2 = 0; i2 < 4; ++i2)
-#pragma unroll
-      for (int i1 = 0; i1 < 4; ++i1) {
-        const float w21 = w[2][i2] * w[1][i1];
-#pragma unroll
-        for (int i0 = 0; i0 < 4; ++i0) fn(((f[2] + i2) * ng + f[1] + i1) * ng + f[0] + i0, w21 * w[0][i0]);
-      }
-  }
-}
-
-template <int D>
-__device__ __forceinline__ int grid_cells(int ng) {
-  return D == 1 ? ng : D == 2 ? ng * ng : ng * ng * ng;
-}
 
 // ---- fill ------------------------------------------------------------------
 __global__ void fill_kernel(float *__restrict__ p, int64_t count, float v) {
@@ -152,32 +101,6 @@ __global__ __launch_bounds__(kBlock) void center_minmax_kernel(float *__restrict
 // atomic instruction conflicts only between the 64/4^d points of a wave
 // (at most 4-way in 2-D, never in 3-D) even when every point of a clustered
 // embedding falls into the same few cells.
-template <int D>
-struct NodeLanes {
-  static constexpr int NODES = D == 1 ? 4 : D == 2 ? 16 : 64;
-  static constexpr int PPW = 64 / NODES;  // points per wave per step
-};
-
-template <int D>
-__device__ __forceinline__ void node_of_lane(const GridParams &g, const float (&y)[D], int node, int &idx,
-                                             float &weight) {
-  int f[D];
-  float w[D][4];
-  stencil<D>(g, y, f, w);
-  const int a0 = node & 3, a1 = (node >> 2) & 3, a2 = node >> 4;
-  const int ng = g.ng;
-  if constexpr (D == 1) {
-    idx = f[0] + a0;
-    weight = w[0][a0];
-  } else if constexpr (D == 2) {
-    idx = (f[1] + a1) * ng + f[0] + a0;
-    weight = w[1][a1] * w[0][a0];
-  } else {
-    idx = ((f[2] + a2) * ng + f[1] + a1) * ng + f[0] + a0;
-    weight = w[2][a2] * w[1][a1] * w[0][a0];
-  }
-}
-
 constexpr int kS2GBlock = 1024;  // 16 waves share one private grid
 constexpr int kS2GUnroll = 4;    // points per lane whose loads are issued together
 
@@ -343,7 +266,6 @@ __global__ __launch_bounds__(kBlock) void g2s_force_kernel(GridParams g, int n,
 // ---- fused gradient + gains + momentum update ----------------------------------
 // dy = alpha Fattr - Frep / Z; gains, uy, y exactly as CU/gradient_descend.cu:29-44;
 // per-block sums of the new Y feed the mean of the next centring.
-__device__ __forceinline__ float sgnf(float x) { return x == 0.0f ? 0.0f : (x < 0.0f ? -1.0f : 1.0f); }
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void update_kernel(int n, float *__restrict__ Y,

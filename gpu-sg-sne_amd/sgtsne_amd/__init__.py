@@ -27,6 +27,9 @@ LAYOUT_SOA = 0
 LAYOUT_AOS = 1
 ZETA_DIRECT = 0
 ZETA_ALGEBRAIC = 1
+ERR_FULL = -8
+JOB_RUNNING = 0
+JOB_DONE = 1
 
 EXPORTED = [
     "sgtsne_params_default", "sgtsne_last_error", "sgtsne_version", "sgtsne_knn_exact",
@@ -35,6 +38,10 @@ EXPORTED = [
     "sgtsne_ctx_terms", "sgtsne_ctx_kl", "sgtsne_ctx_repulsive_exact", "sgtsne_ctx_timing",
     "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
     "sgtsne_ctx_tuning",
+    "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
+    "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
+    "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
+    "sgtsne_batch_set_graph",
 ]
 
 
@@ -104,6 +111,23 @@ def lib():
         "sgtsne_ctx_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
         "sgtsne_ctx_destroy": (C.c_int, [vp]),
+        "sgtsne_batch_create": (C.c_int, [C.POINTER(vp), P, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
+        "sgtsne_batch_destroy": (C.c_int, [vp]),
+        "sgtsne_batch_add": (C.c_int, [vp, C.c_int, C.c_int64, i32p, i32p, f32p, C.c_int, vp, C.c_int, C.c_int,
+                                       C.POINTER(C.c_int)]),
+        "sgtsne_batch_add_points": (C.c_int, [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_double, vp, C.c_int,
+                                              C.c_int, C.POINTER(C.c_int)]),
+        "sgtsne_batch_step": (C.c_int, [vp, C.c_int]),
+        "sgtsne_batch_sync": (C.c_int, [vp]),
+        "sgtsne_batch_job_status": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "sgtsne_batch_get_Y": (C.c_int, [vp, C.c_int, f32p, C.c_int]),
+        "sgtsne_batch_terms": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
+        "sgtsne_batch_remove": (C.c_int, [vp, C.c_int]),
+        "sgtsne_batch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "sgtsne_batch_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]),
+        "sgtsne_batch_set_graph": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -290,3 +314,116 @@ class Context:
         it, n, d, nnz = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
         _check(lib().sgtsne_ctx_info(self._h, C.byref(it), C.byref(n), C.byref(d), C.byref(nnz)))
         return {"iter": it.value, "n": n.value, "d": d.value, "nnz": nnz.value}
+
+
+class Full(SgtsneError):
+    """No room in the batch right now (slots, arena or grid budget)."""
+
+
+class Batch:
+    """Many embedding jobs advanced together on the device (sgtsne_batch_*).
+
+    Jobs are admitted from a CSR P (add) or from their points (add_points:
+    kNN, perplexity and P + P^T on the device); step() queues iterations for
+    every admitted job and returns at once; job_status / get_Y synchronise.
+    """
+
+    def __init__(self, params, max_jobs, max_points, max_nnz, grid_budget_bytes=0):
+        self.params = params
+        self.d = params.d
+        self._h = C.c_void_p()
+        _check(lib().sgtsne_batch_create(C.byref(self._h), C.byref(params), int(max_jobs), int(max_points),
+                                         int(max_nnz), int(grid_budget_bytes)))
+        self.n = {}
+
+    def close(self):
+        if self._h:
+            lib().sgtsne_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _y_arg(self, y_in, n):
+        if y_in is None:
+            return None, None
+        shape = (self.d, n) if self.params.layout == LAYOUT_SOA else (n, self.d)
+        y = np.ascontiguousarray(y_in, dtype=np.float32).reshape(shape)
+        return y, y.ctypes.data
+
+    def _admit(self, status, job, n):
+        if status == ERR_FULL:
+            raise Full(lib().sgtsne_last_error().decode(errors="replace"))
+        _check(status)
+        self.n[job.value] = n
+        return job.value
+
+    def add(self, P, symmetrize=True, y_in=None, max_iter=0, ng_cap=0):
+        job = C.c_int(-1)
+        keep, yp = self._y_arg(y_in, P.n)
+        st = lib().sgtsne_batch_add(self._h, P.n, P.nnz, P.rowptr, P.col, P.val, int(bool(symmetrize)), yp,
+                                    int(max_iter), int(ng_cap), C.byref(job))
+        return self._admit(st, job, P.n)
+
+    def add_points(self, X, k=90, u=30.0, y_in=None, max_iter=0, ng_cap=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, dim = X.shape
+        job = C.c_int(-1)
+        keep, yp = self._y_arg(y_in, n)
+        st = lib().sgtsne_batch_add_points(self._h, X, n, dim, int(k), float(u), yp, int(max_iter), int(ng_cap),
+                                           C.byref(job))
+        return self._admit(st, job, n)
+
+    def step(self, iters=1):
+        _check(lib().sgtsne_batch_step(self._h, int(iters)))
+
+    def sync(self):
+        _check(lib().sgtsne_batch_sync(self._h))
+
+    def job_status(self, job):
+        st, it = C.c_int(), C.c_int()
+        _check(lib().sgtsne_batch_job_status(self._h, int(job), C.byref(st), C.byref(it)))
+        return st.value, it.value
+
+    def get_Y(self, job, layout=LAYOUT_AOS):
+        n = self.n[job]
+        y = np.empty((n, self.d) if layout == LAYOUT_AOS else (self.d, n), np.float32)
+        _check(lib().sgtsne_batch_get_Y(self._h, int(job), y.reshape(-1), int(layout)))
+        return y
+
+    def terms(self, job, layout=LAYOUT_AOS):
+        n = self.n[job]
+        shape = (n, self.d) if layout == LAYOUT_AOS else (self.d, n)
+        Fa = np.empty(shape, np.float32)
+        Fr = np.empty(shape, np.float32)
+        Z = C.c_double()
+        ng = C.c_int()
+        _check(lib().sgtsne_batch_terms(self._h, int(job), Fa.ctypes.data, Fr.ctypes.data, C.byref(Z),
+                                        C.byref(ng), int(layout)))
+        return Fa, Fr, Z.value, ng.value
+
+    def remove(self, job):
+        _check(lib().sgtsne_batch_remove(self._h, int(job)))
+        self.n.pop(job, None)
+
+    def info(self):
+        j, p, z, g, r = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().sgtsne_batch_info(self._h, C.byref(j), C.byref(p), C.byref(z), C.byref(g), C.byref(r)))
+        return {"jobs": j.value, "points": p.value, "nnz": z.value, "grid_bytes": g.value, "graph_replays": r.value}
+
+    def tuning(self):
+        g, u, rpb, gr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().sgtsne_batch_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), C.byref(gr)))
+        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value, "graph": bool(gr.value)}
+
+    def set_graph(self, on):
+        _check(lib().sgtsne_batch_set_graph(self._h, int(bool(on))))

@@ -29,6 +29,13 @@ extern "C" {
 #define SGTSNE_ERR_NUMERIC (-4)   /* non-finite embedding (out of grid)    */
 #define SGTSNE_ERR_STATE (-5)     /* call order (e.g. step before upload)  */
 #define SGTSNE_ERR_NODEV (-6)     /* no gfx950 device visible              */
+#define SGTSNE_ERR_FULL (-8)      /* no room: job slots, arena or grid budget
+                                     (retry after a job leaves), or a job's
+                                     grid outgrew its capacity               */
+
+/* job states reported by sgtsne_batch_job_status (errors are negative) */
+#define SGTSNE_JOB_RUNNING 0
+#define SGTSNE_JOB_DONE 1
 
 /* ---- sparse formats: the reference's numbering (CU/types.hpp:66,
  *      CU/pq.cu:25-57): 0 CSR, 1 BSR (stub in the reference, rejected
@@ -201,6 +208,69 @@ int sgtsne_ctx_errors(sgtsne_ctx *ctx, double *out, int max, int *count);
 int sgtsne_ctx_info(sgtsne_ctx *ctx, int *iter, int *n, int *d, int64_t *nnz);
 
 int sgtsne_ctx_destroy(sgtsne_ctx *ctx);
+
+/* ---- batched engine ------------------------------------------------------
+ * Many independent embedding jobs of uneven size (same params.d) advanced
+ * together: one launch per phase for all jobs, iterations replayed from a
+ * captured HIP graph, P, Y and the grids resident on the device.  Each job
+ * runs the plain method's schedule (CU/gradient_descend.cu:102-245) from its
+ * own admission; its result does not depend on which other jobs share the
+ * batch (bit-identical alone and in any mix).  The reference has no batched
+ * form: it runs one sgtsneCUDA call (CU/sgtsne.cu:26-128) per embedding. */
+typedef struct sgtsne_batch sgtsne_batch;
+
+/* Capacity: job slots (<= 1024), points and P_sym nonzeros of the arena, and
+ * the byte budget of the per-job grid buffers (<= 0: 64 GiB). */
+int sgtsne_batch_create(sgtsne_batch **batch, const sgtsne_params *params, int max_jobs, int64_t max_points,
+                        int64_t max_nnz, int64_t grid_budget_bytes);
+int sgtsne_batch_destroy(sgtsne_batch *batch);
+
+/* Admits a job given its row-stochastic kNN P (CSR, host; symmetrize: form
+ * (P + P^T) / (2N) as sgtsneCUDA does, CU/sgtsne.cu:58-76) and y_in (layout
+ * params.layout, or NULL for the seeded N(0, 1e-4^2) start).  max_iter <= 0:
+ * params.max_iter; ng_cap <= 0: automatic grid capacity.  Returns
+ * SGTSNE_ERR_FULL when the job does not fit now.  The job starts iterating
+ * with the next sgtsne_batch_step. */
+int sgtsne_batch_add(sgtsne_batch *batch, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                     const float *val, int symmetrize, const float *y_in, int max_iter, int ng_cap, int *job);
+
+/* Admits a job from its points X [n][dim]: exact kNN (k), perplexity u and
+ * P + P^T on the device (perplexityEqualization, CU/perplexity_search.cu:114-141,
+ * then as sgtsne_batch_add).  Runs on a stream of its own, so it overlaps
+ * iterations already queued by sgtsne_batch_step. */
+int sgtsne_batch_add_points(sgtsne_batch *batch, const float *X, int n, int dim, int k, double u, const float *y_in,
+                            int max_iter, int ng_cap, int *job);
+
+/* Queues iters batch iterations (asynchronous; returns before they finish).
+ * A newly admitted job spends its first batch iteration centring Y and
+ * sizing its grid, then advances one iteration per batch iteration until
+ * its max_iter. */
+int sgtsne_batch_step(sgtsne_batch *batch, int iters);
+int sgtsne_batch_sync(sgtsne_batch *batch);
+
+/* Waits for queued work, then reports SGTSNE_JOB_RUNNING / SGTSNE_JOB_DONE
+ * or a negative error of the job and its iteration counter. */
+int sgtsne_batch_job_status(sgtsne_batch *batch, int job, int *status, int *iter);
+int sgtsne_batch_get_Y(sgtsne_batch *batch, int job, float *y, int layout);
+
+/* The job's attractive and (unnormalised) repulsive terms and Z at its
+ * current Y, without stepping (same meaning as sgtsne_ctx_terms). */
+int sgtsne_batch_terms(sgtsne_batch *batch, int job, float *Fattr, float *Frep, double *Z, int *ng, int layout);
+
+/* Frees the job's slot and grid buffers; its rows are reclaimed by a later
+ * compaction of the arena. */
+int sgtsne_batch_remove(sgtsne_batch *batch, int job);
+
+/* Live jobs, arena points and nonzeros in use, grid bytes in use, graph replays. */
+int sgtsne_batch_info(sgtsne_batch *batch, int *jobs, int64_t *points, int64_t *nnz, int64_t *grid_bytes,
+                      int64_t *graph_replays);
+
+/* Attractive-kernel shape chosen by the batch autotuner and whether
+ * iterations are graph-replayed. */
+int sgtsne_batch_tuning(sgtsne_batch *batch, int *group, int *unroll, int *rows_per_block, int *use_graph);
+
+/* Graph replay on (default) or off (iterations launched one by one). */
+int sgtsne_batch_set_graph(sgtsne_batch *batch, int on);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,207 @@
+"""The batched engine (sgtsne_batch_*), through the C-ABI.
+
+Bars: each job's per-iteration terms within 1e-4 relative of the FP64 oracle
+(same bar as the single-job path); a job's trajectory bit-identical whether
+it runs alone or among other jobs (admitted earlier or later, removed on the
+way), with or without graph replay; final KL of a 1000-iteration job within
+1% of the FP64 loop."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _mixture(n, dim, clusters, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, 4, size=(clusters, dim))
+    return (c[rng.integers(0, clusters, n)] + rng.standard_normal((n, dim))).astype(np.float32)
+
+
+def _job(sg, O, n, seed, k=45, u=15.0, dim=20):
+    X = _mixture(n, dim, 6, seed)
+    I, D = O.knn_exact(X, k)
+    So = O.symmetrize(O.perplexity_equalize(I, D, u))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    return S, So
+
+
+def _clustered(n, d, rng, spread=30.0, clusters=12):
+    centres = rng.uniform(-spread, spread, size=(clusters, d))
+    lab = rng.integers(0, clusters, n)
+    return (centres[lab] + rng.standard_normal((n, d)) * rng.uniform(0.3, 2.0, clusters)[lab, None]).astype(np.float32)
+
+
+def _batch(sg, d=2, max_jobs=16, points=200_000, nnz=20_000_000, **kw):
+    return sg.Batch(sg.tsneparams(d=d, **kw), max_jobs, points, nnz)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_batch_terms_vs_oracle(sg, O, d):
+    # three jobs of uneven size in one batch; the terms of each at a formed
+    # (clustered) embedding and at an early one, against the FP64 oracle
+    sizes = (700, 3100, 5200)
+    rng = np.random.default_rng(20 + d)
+    h = {1: 0.5, 2: 0.7, 3: 1.2}[d]
+    with _batch(sg, d=d) as B:
+        jobs = []
+        for t, n in enumerate(sizes):
+            S, So = _job(sg, O, n, 100 * d + t)
+            Y = _clustered(n, d, rng) if t != 1 else (rng.standard_normal((n, d)) * 2.0).astype(np.float32)
+            jobs.append((B.add(S, symmetrize=False, y_in=Y), So))
+        B.step(1)  # admission: centring and the first grid
+        for j, So in jobs:
+            Y = B.get_Y(j).astype(np.float64)
+            Fa, Fr, Z, ng = B.terms(j)
+            Fao = O.attractive(So, Y)
+            Fro, Zo, ngo = O.repulsive_interp(Y.copy(), h, O.ZETA_DIRECT)
+            assert ng == ngo
+            assert rel_l2(Fa, Fao) < TOL
+            assert rel_l2(Fr, Fro) < TOL
+            assert abs(Z - Zo) / Zo < TOL
+
+
+def test_batch_algebraic_zeta(sg, O):
+    S, So = _job(sg, O, 2000, 5)
+    Y = _clustered(2000, 2, np.random.default_rng(3))
+    with _batch(sg, zeta_mode=sg.ZETA_ALGEBRAIC) as B:
+        j = B.add(S, symmetrize=False, y_in=Y)
+        B.step(1)
+        Yc = B.get_Y(j).astype(np.float64)
+        _, Fr, Z, _ = B.terms(j)
+    Fro, Zo, _ = O.repulsive_interp(Yc.copy(), 0.7, O.ZETA_ALGEBRAIC)
+    assert rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL
+
+
+def test_batch_iterations_track_oracle(sg, O):
+    # ten iterations of a batched job against the FP64 loop from the same start
+    n = 2500
+    S, So = _job(sg, O, n, 7)
+    y0 = (np.random.default_rng(4).standard_normal((n, 2)) * 1e-2).astype(np.float32)
+    with _batch(sg) as B:
+        other = B.add(*_job(sg, O, 1800, 8)[:1], symmetrize=False)
+        j = B.add(S, symmetrize=False, y_in=y0)
+        B.step(1)
+        Yc = B.get_Y(j).astype(np.float64)
+        B.step(10)
+        Y10 = B.get_Y(j)
+        st, it = B.job_status(j)
+        assert st == sg.JOB_RUNNING and it == 10
+    Yo = Yc.copy()
+    O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, 10, zmode=O.ZETA_DIRECT)
+    assert rel_l2(Y10, Yo) < 1e-3
+
+
+def _run_alone(sg, S, y0, iters, graph=True):
+    with _batch(sg) as B:
+        B.set_graph(graph)
+        j = B.add(S, symmetrize=False, y_in=y0)
+        B.step(iters + 1)
+        return B.get_Y(j)
+
+
+def test_batch_alone_vs_mix_bitexact(sg, O):
+    # job A's trajectory (through the re-orders at iterations 50 and 150) is
+    # the same bits alone and in a mix where other jobs arrive before and after
+    # it and one of them leaves (forcing a compaction that moves A)
+    SA, _ = _job(sg, O, 3000, 31)
+    SB, _ = _job(sg, O, 5000, 32)
+    SC, _ = _job(sg, O, 1200, 33)
+    y0 = (np.random.default_rng(5).standard_normal((3000, 2)) * 1e-4).astype(np.float32)
+    iters = 170
+    Y_alone = _run_alone(sg, SA, y0, iters)
+    with _batch(sg, max_jobs=4) as B:
+        b = B.add(SB, symmetrize=False)
+        B.step(37)
+        a = B.add(SA, symmetrize=False, y_in=y0)
+        c = B.add(SC, symmetrize=False)
+        B.step(60)
+        B.remove(b)  # A moves down over B's rows
+        B.step(iters + 1 - 60)
+        st, it = B.job_status(a)
+        assert st == sg.JOB_RUNNING and it == iters
+        Y_mix = B.get_Y(a)
+        assert B.info()["jobs"] == 2
+    assert np.array_equal(Y_alone, Y_mix)
+
+
+def test_batch_graph_replay_equals_launches(sg, O):
+    S, _ = _job(sg, O, 2200, 41)
+    y0 = (np.random.default_rng(6).standard_normal((2200, 2)) * 1e-4).astype(np.float32)
+    Yg = _run_alone(sg, S, y0, 80, graph=True)
+    Yl = _run_alone(sg, S, y0, 80, graph=False)
+    assert np.array_equal(Yg, Yl)
+
+
+def test_batch_final_kl(sg, O):
+    # 1000 iterations of a batched job (alongside two others): final KL within
+    # 1% of the FP64 loop from the same start
+    n = 2500
+    X = _mixture(n, 50, 10, 12)
+    I, D = sg.knn_exact(X, 90)
+    P = sg.perplexityEqualization(I, D, n, 90, 30.0)
+    So = O.symmetrize(O.CSR(n, P.rowptr.astype(np.int64), P.col, P.val.astype(np.float64)))
+    y0 = (1e-4 * np.random.default_rng(1).standard_normal((n, 2))).astype(np.float32)
+    with _batch(sg) as B:
+        B.add(_job(sg, O, 900, 51)[0], symmetrize=False)
+        j = B.add(P, symmetrize=True, y_in=y0)
+        B.add(_job(sg, O, 4000, 52)[0], symmetrize=False, max_iter=400)
+        B.step(1001)
+        st, it = B.job_status(j)
+        assert st == sg.JOB_DONE and it == 1000
+        Y = B.get_Y(j).astype(np.float64)
+    Yo = y0.astype(np.float64).copy()
+    O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, 1000, zmode=O.ZETA_DIRECT)
+    _, Zg, _ = O.repulsive_interp(Y.copy(), 0.7, O.ZETA_DIRECT)
+    _, Zo, _ = O.repulsive_interp(Yo.copy(), 0.7, O.ZETA_DIRECT)
+    klg = O.kl(So, Y, Zg)
+    klo = O.kl(So, Yo, Zo)
+    assert abs(klg - klo) / klo < 0.01, (klg, klo)
+
+
+def test_batch_add_points_front_end(sg, O):
+    # the device front end (exact kNN, perplexity, P + P^T) gives the same
+    # P_sym as the host-visible path: the attractive terms agree
+    n = 1500
+    X = _mixture(n, 30, 5, 61)
+    I, D = O.knn_exact(X, 60)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 20.0))
+    Y = _clustered(n, 2, np.random.default_rng(7))
+    with _batch(sg) as B:
+        j = B.add_points(X, k=60, u=20.0, y_in=Y)
+        B.step(1)
+        Yc = B.get_Y(j).astype(np.float64)
+        Fa, _, _, _ = B.terms(j)
+    assert rel_l2(Fa, O.attractive(So, Yc)) < TOL
+
+
+def test_batch_admission_limits(sg, O):
+    S, _ = _job(sg, O, 1000, 71)
+    with sg.Batch(sg.tsneparams(), 2, 2500, 10_000_000) as B:
+        a = B.add(S, symmetrize=False)
+        B.add(S, symmetrize=False)
+        with pytest.raises(sg.Full):
+            B.add(S, symmetrize=False)  # no slot
+        B.remove(a)
+        with pytest.raises(sg.Full):
+            B.add(_job(sg, O, 1600, 72)[0], symmetrize=False)  # 1000 + 1600 > 2500 points
+        B.add(S, symmetrize=False)
+        B.step(3)
+        assert B.info()["jobs"] == 2
+
+
+def test_batch_numeric_error_is_reported(sg, O):
+    S, _ = _job(sg, O, 800, 81)
+    Y = np.zeros((800, 2), np.float32)
+    Y[3, 1] = np.nan
+    with _batch(sg) as B:
+        bad = B.add(S, symmetrize=False, y_in=Y)
+        good = B.add(S, symmetrize=False)
+        B.step(5)
+        assert B.job_status(bad)[0] == -4
+        st, it = B.job_status(good)
+        assert st == sg.JOB_RUNNING and it == 4
+        assert np.all(np.isfinite(B.get_Y(good)))
