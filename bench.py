@@ -68,6 +68,102 @@ def cpu_baseline(P, Y, d, it0, target_s, max_iters):
     return ms, nthreads, iters + 1
 
 
+def plain_job(sg, X, d, k, u, y0, max_iter=1000):
+    """The plain method on one job: exact kNN, perplexity, sgtsneCUDA-style run."""
+    I, D = sg.knn_exact(X, k)
+    P = sg.perplexityEqualization(I, D, X.shape[0], k, u)
+    Y = sg.sgtsne(P, sg.tsneparams(d=d, maxIter=max_iter, layout=sg.LAYOUT_AOS), y_in=y0)
+    return P, Y
+
+
+def job_kl(sg, P, Y, d):
+    with sg.Context(sg.tsneparams(d=d, reorder=0)) as ctx:
+        ctx.upload_P(P, symmetrize=True)
+        ctx.set_Y(Y)
+        return ctx.kl(1.0)
+
+
+def batched_leg(sg, jobs, sample, k=90, u=30.0):
+    """256 jobs (N log-uniform in [500, 50,000], 50-D 5..20-cluster mixtures)
+    in one batched engine against the plain method one job at a time."""
+    from sgtsne_amd import synth
+
+    sizes = synth.batch_sizes(jobs=jobs)
+    seeds = [synth.BASE_SEED + 3000 + i for i in range(jobs)]
+    total = int(sizes.sum())
+    log(f"[bench] batched: {jobs} jobs, {total} points")
+    B = sg.Batch(sg.tsneparams(d=2), jobs, total, 2 * k * total)
+    t_front = 0.0
+    ids = []
+    for i in range(jobs):
+        X, _ = synth.gaussian_job(int(sizes[i]), seed=seeds[i])
+        y0 = synth.random_init(int(sizes[i]), 2, seed=seeds[i])
+        t0 = time.perf_counter()
+        ids.append(B.add_points(X, k=k, u=u, y_in=y0, max_iter=1000))
+        t_front += time.perf_counter() - t0
+    t0 = time.perf_counter()
+    B.step(1001)
+    B.sync()
+    t_iter = time.perf_counter() - t0
+    done = sum(1 for j in ids if B.job_status(j)[0] == sg.JOB_DONE)
+    info, tune = B.info(), B.tuning()
+    # plain method on a size-stratified sample; KL of the same jobs both ways
+    order = np.argsort(sizes)
+    pick = order[np.linspace(0, jobs - 1, sample).round().astype(int)]
+    t_plain = 0.0
+    ratios = []
+    for i in pick:
+        X, _ = synth.gaussian_job(int(sizes[i]), seed=seeds[i])
+        y0 = synth.random_init(int(sizes[i]), 2, seed=seeds[i])
+        t0 = time.perf_counter()
+        P, Yp = plain_job(sg, X, 2, k, u, y0)
+        t_plain += time.perf_counter() - t0
+        ratios.append(job_kl(sg, P, B.get_Y(int(ids[i])), 2) / job_kl(sg, P, Yp, 2))
+    B.close()
+    t_plain_all = t_plain * jobs / sample
+    out = {"jobs": jobs, "points": total, "nnz_sym": info["nnz"], "jobs_done": done,
+           "front_end_s": round(t_front, 3), "iterations_s": round(t_iter, 3),
+           "ms_per_batch_iteration": round(t_iter * 1e3 / 1001, 4),
+           "embeds_per_s": round(jobs / (t_front + t_iter), 3),
+           "plain_embeds_per_s": round(jobs / t_plain_all, 3),
+           "speedup_vs_plain": round(t_plain_all / (t_front + t_iter), 2),
+           "plain_sample": f"{sample} jobs stratified by N, extrapolated to {jobs}",
+           "kl_batched_over_plain": {"mean": round(float(np.mean(ratios)), 5), "max": round(float(np.max(ratios)), 5)},
+           "graph_replays": info["graph_replays"], "attractive_tuning": tune}
+    log(f"[bench] batched: {out}")
+    return out
+
+
+def served_leg(sg, requests, rate, tick, k=90, u=30.0):
+    """Poisson arrivals (N log-uniform in [1,000, 200,000], d in {2, 3}) through
+    the continuous-batching server, then one at a time through the plain method."""
+    from sgtsne_amd import serving, synth
+
+    arrivals = synth.serving_requests(requests, rate)
+    data = {seed: synth.gaussian_job(n, seed=seed)[0] for _, n, _, seed in arrivals}  # the requests' payloads
+
+    def make_X(n, seed):
+        return data[seed]
+
+    log(f"[bench] served: {requests} requests at {rate}/s, N total {sum(a[1] for a in arrivals)}")
+    with serving.Server(tick=tick, k=k, u=u) as srv:
+        t0 = time.perf_counter()
+        got = srv.run(arrivals, make_X)
+        t_srv = time.perf_counter() - t0
+    ok = sum(1 for r in got if r.Y is not None and np.all(np.isfinite(r.Y)))
+    s_stats = serving.latency_stats(got)
+    t0 = time.perf_counter()
+    base = serving.one_at_a_time(arrivals, make_X, k=k, u=u)
+    t_base = time.perf_counter() - t0
+    b_stats = serving.latency_stats(base)
+    out = {"requests": requests, "rate_per_s": rate, "tick": tick, "completed_finite": ok,
+           "served": s_stats, "one_at_a_time": b_stats,
+           "embeds_per_s_ratio": round(s_stats["embeds_per_s"] / b_stats["embeds_per_s"], 2),
+           "wall_s": {"served": round(t_srv, 2), "one_at_a_time": round(t_base, 2)}}
+    log(f"[bench] served: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +177,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-iters", type=int, default=30)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-sample", type=int, default=16)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-tick", type=int, default=25)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -177,6 +280,9 @@ def main():
         log(f"[bench] CPU {ms_cpu:.1f} ms/iter on {cores} threads -> GPU speed-up {ms_cpu / ms_per_step:.1f}x")
     ctx.close()
 
+    batched = None if args.no_batched else batched_leg(sg, args.batch_jobs, args.batch_sample)
+    served = None if args.no_served else served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick)
+
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 4),
@@ -200,6 +306,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "term_error_vs_fp64": term_err,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

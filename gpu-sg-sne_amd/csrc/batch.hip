@@ -236,7 +236,7 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
 int tune(sgtsne_batch *b) {
   if (b->used_points < 4096) return SGTSNE_OK;
   struct Cand { int unroll, rpb; };
-  const Cand cands[] = {{1, 0}, {2, 0}, {4, 0}, {2, 32}, {2, 64}, {1, 64}, {2, 128}, {4, 128}, {2, 256}};
+  const Cand cands[] = {{1, 0}, {2, 0}, {4, 0}, {2, 32}, {2, 64}, {1, 64}, {2, 128}, {1, 128}, {2, 256}};
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));
@@ -541,7 +541,7 @@ void destroy_all(sgtsne_batch *b) {
 }
 
 // Reads one job's rows of an arena array back in the caller's order.
-int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int layout) {
+int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int layout, double scale = 1.0) {
   const HostJob &h = b->hj[job];
   const int n = h.n, d = b->d, ds = b->ds;
   std::vector<float> v((size_t)n * ds);
@@ -553,7 +553,7 @@ int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int lay
   for (int r = 0; r < n; ++r) {
     const int i = pm[r];
     for (int a = 0; a < d; ++a) {
-      const float x = v[(size_t)r * ds + a];
+      const float x = scale == 1.0 ? v[(size_t)r * ds + a] : (float)(v[(size_t)r * ds + a] * scale);
       if (layout == SGTSNE_LAYOUT_SOA) out[(size_t)a * n + i] = x;
       else out[(size_t)i * d + a] = x;
     }
@@ -802,7 +802,7 @@ int sgtsne_batch_terms(sgtsne_batch *b, int job, float *Fattr, float *Frep, doub
   if (Z) *Z = z;
   if (ng) *ng = g.ng;
   if (Fattr) SGT_TRY(fetch_points(b, job, b->Fattr, Fattr, layout));
-  if (Frep) SGT_TRY(fetch_points(b, job, b->Frep, Frep, layout));
+  if (Frep) SGT_TRY(fetch_points(b, job, b->Frep, Frep, layout, 1.0 / z));
   return SGTSNE_OK;
 }
 

@@ -253,7 +253,7 @@ int sgtsne_batch_sync(sgtsne_batch *batch);
 int sgtsne_batch_job_status(sgtsne_batch *batch, int job, int *status, int *iter);
 int sgtsne_batch_get_Y(sgtsne_batch *batch, int job, float *y, int layout);
 
-/* The job's attractive and (unnormalised) repulsive terms and Z at its
+/* The job's attractive term, repulsive term (normalised by Z) and Z at its
  * current Y, without stepping (same meaning as sgtsne_ctx_terms). */
 int sgtsne_batch_terms(sgtsne_batch *batch, int job, float *Fattr, float *Frep, double *Z, int *ng, int layout);
 
