@@ -177,6 +177,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-iters", type=int, default=30)
+    ap.add_argument("--calibrate", action="store_true", help="also time a plain 256 MiB device copy")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-sample", type=int, default=16)
@@ -226,6 +227,8 @@ def main():
         ctx.step(args.warmup)
     t_warm = time.perf_counter() - t0
     log(f"[bench] warmup {args.warmup} iterations: {t_warm * 1e3 / max(args.warmup, 1):.3f} ms/iter")
+    tune = ctx.tuning()  # re-chosen after the first Morton re-order
+    log(f"[bench] attractive kernel in the timed region: {tune}")
     Y_snap = ctx.get_Y()
     it0 = ctx.info()["iter"]
 
@@ -239,6 +242,7 @@ def main():
               "g2s_forces": ph[3] / iters, "update_center": ph[4] / iters, "device_total": ph[5] / iters}
     log(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iter; phases {phases}")
     Y_end = ctx.get_Y()
+    reorders = ctx.reorders()
     kl = ctx.kl(1.0)
     Fa_g, Fr_g, Z_g, grid = ctx.terms()
     log(f"[bench] final KL {kl:.4f}, grid side {grid}")
@@ -250,7 +254,8 @@ def main():
     achieved = bytes_b / t_spmv / 1e9 if t_spmv > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": (f"attr_csr_blk_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
+                "kernel": (f"attr_csr_win_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["window"]
+                           else f"attr_csr_blk_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
                            else f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
                            else f"attr_csr_seg_kernel<{args.d},{tune['group']}>"),
                 "bytes_per_launch": bytes_b,
@@ -280,6 +285,12 @@ def main():
         log(f"[bench] CPU {ms_cpu:.1f} ms/iter on {cores} threads -> GPU speed-up {ms_cpu / ms_per_step:.1f}x")
     ctx.close()
 
+    copy = None
+    if args.calibrate:
+        nb = 256 << 20
+        ms = sg.copy_probe(nb, 20)
+        copy = {"bytes": nb, "ms": round(ms, 5), "GB_per_s": round(2 * nb / ms / 1e6, 1)}
+        log(f"[bench] plain copy: {copy}")
     batched = None if args.no_batched else batched_leg(sg, args.batch_jobs, args.batch_sample)
     served = None if args.no_served else served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick)
 
@@ -297,7 +308,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
-                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder, "timed_iterations": [it0, it0 + args.steps - 1],
+                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder, "morton_reorders": reorders, "timed_iterations": [it0, it0 + args.steps - 1],
                    "final_grid_side": grid, "final_kl": round(kl, 5),
                    "attractive_tuning": tune},
         "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
@@ -306,6 +317,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "term_error_vs_fp64": term_err,
+        "copy_probe": copy,
         "batched": batched,
         "served": served,
     }

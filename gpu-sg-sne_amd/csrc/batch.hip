@@ -82,7 +82,7 @@ struct sgtsne_batch {
 
   // launch shapes (autotuned for the attractive term; the others are
   // persistent work-group counts)
-  int spmv_group = 16, spmv_unroll = 2, spmv_rpb = 0;
+  int spmv_group = 16, spmv_unroll = 2, spmv_rpb = 0, spmv_win = 0;
   int64_t tuned_points = 0;
   std::vector<double> tune_ms;
   int s2g_blocks = 1024, s2g_lds = 32768, conv_blocks = 1024, pt_blocks = 2048;
@@ -164,6 +164,9 @@ int rebuild_items(sgtsne_batch *b) {
 int attractive(sgtsne_batch *b) {
   const int n = (int)b->used_points;
   if (n == 0) return SGTSNE_OK;
+  if (b->spmv_win > 0)
+    return launch_attr_csr_win(b->stream, b->d, b->spmv_group, b->spmv_unroll, b->spmv_rpb, b->spmv_win, n,
+                               b->rowptr, b->col, b->val, b->Y, b->Fattr);
   if (b->spmv_rpb > 0)
     return launch_attr_csr_blk(b->stream, b->d, b->spmv_group, b->spmv_unroll, b->spmv_rpb, n, b->rowptr, b->col,
                                b->val, b->Y, b->Fattr);
@@ -235,19 +238,23 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
 // choice never changes a result; only rows per work-group and unroll vary.
 int tune(sgtsne_batch *b) {
   if (b->used_points < 4096) return SGTSNE_OK;
-  struct Cand { int unroll, rpb; };
-  const Cand cands[] = {{1, 0}, {2, 0}, {4, 0}, {2, 32}, {2, 64}, {1, 64}, {2, 128}, {1, 128}, {2, 256}};
+  struct Cand { int unroll, rpb, win; };
+  const Cand cands[] = {{1, 0, 0},       {2, 0, 0},       {4, 0, 0},       {2, 32, 0},     {2, 64, 0},
+                        {1, 64, 0},      {2, 128, 0},     {1, 128, 0},     {2, 256, 0},    {1, 256, 4096},
+                        {2, 512, 4096},  {1, 512, 8192},  {2, 1024, 8192}};
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));
   b->tune_ms.clear();
   float best = 1e30f;
-  Cand pick{b->spmv_unroll, b->spmv_rpb};
+  Cand pick{b->spmv_unroll, b->spmv_rpb, b->spmv_win};
   const Cand keep = pick;
   int st = SGTSNE_OK;
   for (const Cand &c : cands) {
+    if (c.win > 0 && (size_t)c.win * b->ds * sizeof(float) > 65536) continue;  // window must fit LDS
     b->spmv_unroll = c.unroll;
     b->spmv_rpb = c.rpb;
+    b->spmv_win = c.win;
     st = attractive(b);  // warm
     if (st) break;
     (void)hipEventRecord(e0, b->stream);
@@ -269,10 +276,12 @@ int tune(sgtsne_batch *b) {
   if (st) {
     b->spmv_unroll = keep.unroll;
     b->spmv_rpb = keep.rpb;
+    b->spmv_win = keep.win;
     return st;
   }
   b->spmv_unroll = pick.unroll;
   b->spmv_rpb = pick.rpb;
+  b->spmv_win = pick.win;
   b->tuned_points = b->used_points;
   invalidate_graph(b);
   return SGTSNE_OK;
@@ -842,8 +851,9 @@ int sgtsne_batch_info(sgtsne_batch *b, int *jobs, int64_t *points, int64_t *nnz,
   return SGTSNE_OK;
 }
 
-int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_block, int *use_graph) {
+int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_block, int *window, int *use_graph) {
   SGT_TRY(check_batch(b));
+  if (window) *window = b->spmv_win;
   if (group) *group = b->spmv_group;
   if (unroll) *unroll = b->spmv_unroll;
   if (rows_per_block) *rows_per_block = b->spmv_rpb;

@@ -15,6 +15,7 @@
 #include "../../include/sgtsne.h"
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cfloat>
 #include <chrono>
@@ -89,7 +90,10 @@ struct sgtsne_ctx {
   int seg_group = 16;
   int seg_unroll = 0;  // 0: plain segmented kernel; else the unrolled form
   int seg_rpb = 0;     // > 0: blocked form with this many rows per work-group
+  int seg_win = 0;     // > 0: windowed blocked form, this many points of Y staged in LDS
   std::vector<double> tune_ms;  // autotuner timings (ms per candidate)
+  std::vector<std::array<int, 4>> tune_cands;
+  bool tune_pinned = false;  // the caller fixed the attractive kernel: no re-tuning
 
   // per-point state, AoS with stride ds
   float *Y = nullptr, *uY = nullptr, *gains = nullptr, *Fattr = nullptr, *Frep = nullptr;
@@ -121,6 +125,7 @@ struct sgtsne_ctx {
   // locality order: device row r holds the caller's point perm[r]
   bool permuted = false;
   int reorders = 0;
+  int reorders_declined = 0;  // Morton orders measured slower than the current one
   std::vector<int32_t> perm, iperm;
 };
 
@@ -241,6 +246,9 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
 int attractive(sgtsne_ctx *c) {
   switch (c->p.format) {
     case SGTSNE_FORMAT_CSR_SEG:
+      if (c->seg_win > 0)
+        return launch_attr_csr_win(c->stream, c->d, c->seg_group, c->seg_unroll, c->seg_rpb, c->seg_win, c->n,
+                                   c->rowptr, c->col, c->val, c->Y, c->Fattr);
       if (c->seg_rpb > 0)
         return launch_attr_csr_blk(c->stream, c->d, c->seg_group, c->seg_unroll, c->seg_rpb, c->n, c->rowptr,
                                    c->col, c->val, c->Y, c->Fattr);
@@ -312,11 +320,44 @@ void harvest_timing(sgtsne_ctx *c, PhaseEvents &pe, int it) {
   pe.valid = false;
 }
 
-// Iterations at which the points are re-ordered along a Morton curve of the
-// embedding (reorder >= 2, CSR formats): by then the clusters have formed and
-// the neighbours of a point in P are its neighbours in Y, so the gathers of
-// the attractive term and the spread / gather touch small windows of memory.
+// Iterations at which a re-order of the points along a Morton curve of the
+// embedding is tried (reorder >= 2, CSR formats).  Once the clusters have
+// formed, the neighbours of a point in P are its neighbours in Y, and the
+// gathers of the attractive term touch small windows of memory; before that
+// (large N converges slowly) the breadth-first order of P is better.  So the
+// new order is kept only when the attractive kernel measures faster on it.
 constexpr int kSpatialReorderIters[] = {50, 150, 300, 600};
+
+int tune_attractive(sgtsne_ctx *c);
+
+// Mean ms of the current attractive kernel over 3 launches (after a warm one)
+// on the given P and Y; the context's own arrays are restored afterwards.
+int time_attractive(sgtsne_ctx *c, int32_t *rp, int32_t *col, float *val, float *Y, float *ms) {
+  int32_t *k_rp = c->rowptr, *k_col = c->col;
+  float *k_val = c->val, *k_Y = c->Y;
+  c->rowptr = rp;
+  c->col = col;
+  c->val = val;
+  c->Y = Y;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess ? SGTSNE_OK : SGTSNE_ERR_HIP;
+  if (!st) st = attractive(c);
+  if (!st) (void)hipEventRecord(e0, c->stream);
+  for (int r = 0; r < 3 && !st; r++) st = attractive(c);
+  if (!st) (void)hipEventRecord(e1, c->stream);
+  if (!st && hipEventSynchronize(e1) != hipSuccess) st = SGTSNE_ERR_HIP;
+  float t = 0.f;
+  if (!st) (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / 3.0f;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  c->rowptr = k_rp;
+  c->col = k_col;
+  c->val = k_val;
+  c->Y = k_Y;
+  if (st == SGTSNE_ERR_HIP) set_error("timing of the attractive kernel failed");
+  return st;
+}
 
 int spatial_reorder(sgtsne_ctx *c) {
   const int n = c->n, d = c->d, ds = c->ds;
@@ -338,8 +379,25 @@ int spatial_reorder(sgtsne_ctx *c) {
   SGT_TRY(alloc_dev((void **)&tmp, sizeof(float) * (size_t)n * ds));
   int st = spatial_order(c->stream, n, d, ds, c->Y, lo, extent, order, inv);
   if (!st) st = launch_permute_csr(c->stream, n, c->nnz, order, inv, c->rowptr, c->col, c->val, rp2, col2, val2);
-  float **state[3] = {&c->Y, &c->uY, &c->gains};
-  for (int k = 0; k < 3 && !st; k++) {
+  if (!st) st = launch_gather_points(c->stream, n, ds, order, c->Y, tmp);
+  // keep the new order only if the attractive term runs faster on it
+  float t_old = 0.f, t_new = 0.f;
+  if (!st) st = time_attractive(c, c->rowptr, c->col, c->val, c->Y, &t_old);
+  if (!st) st = time_attractive(c, rp2, col2, val2, tmp, &t_new);
+  if (!st && !(t_new < 0.98f * t_old)) {
+    (void)hipStreamSynchronize(c->stream);
+    free_dev(order);
+    free_dev(inv);
+    free_dev(tmp);
+    free_dev(rp2);
+    free_dev(col2);
+    free_dev(val2);
+    c->reorders_declined++;
+    return SGTSNE_OK;
+  }
+  if (!st) std::swap(c->Y, tmp);
+  float **state[2] = {&c->uY, &c->gains};
+  for (int k = 0; k < 2 && !st; k++) {
     st = launch_gather_points(c->stream, n, ds, order, *state[k], tmp);
     std::swap(*state[k], tmp);
   }
@@ -379,6 +437,8 @@ int spatial_reorder(sgtsne_ctx *c) {
   for (int r = 0; r < n; r++) c->iperm[c->perm[r]] = r;
   c->permuted = true;
   c->reorders++;
+  // the first re-order changes the gather pattern most: choose the kernel again
+  if (c->reorders == 1 && c->p.format == SGTSNE_FORMAT_CSR_SEG && !c->tune_pinned) SGT_TRY(tune_attractive(c));
   return SGTSNE_OK;
 }
 
@@ -542,24 +602,35 @@ int alloc_points(sgtsne_ctx *c) {
 // candidate on the uploaded matrix and keeps the fastest.  Runs once per
 // upload; the result is reported by sgtsne_ctx_tuning.
 int tune_attractive(sgtsne_ctx *c) {
-  struct Cand { int g, u, rpb; };  // lanes per row, unroll (0 = plain), rows per work-group (0 = grid-stride)
-  const Cand cands[] = {{16, 0, 0},  {32, 0, 0},  {16, 4, 0},  {32, 2, 0},  {64, 2, 0},
-                        {16, 1, 64}, {32, 1, 64}, {32, 2, 64}, {16, 2, 128}, {32, 2, 256}, {64, 1, 256}};
+  // lanes per row, unroll (0 = plain), rows per work-group (0 = grid-stride),
+  // points of Y staged in LDS (0 = none)
+  struct Cand { int g, u, rpb, win; };
+  const Cand cands[] = {{16, 0, 0, 0},     {32, 0, 0, 0},      {16, 4, 0, 0},      {32, 2, 0, 0},
+                        {64, 2, 0, 0},     {16, 1, 64, 0},     {32, 1, 64, 0},     {32, 2, 64, 0},
+                        {16, 2, 128, 0},   {32, 2, 256, 0},    {64, 1, 256, 0},    {16, 1, 256, 4096},
+                        {16, 2, 512, 4096}, {32, 1, 512, 4096}, {16, 1, 512, 8192}, {16, 2, 1024, 8192},
+                        {32, 2, 1024, 8192}};
   if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  int bg = c->seg_group, bu = c->seg_unroll, brpb = c->seg_rpb;
+  int bg = c->seg_group, bu = c->seg_unroll, brpb = c->seg_rpb, bw = c->seg_win;
   c->tune_ms.clear();
+  c->tune_cands.clear();
   int st = SGTSNE_OK;
   for (const Cand &k : cands) {
+    if (k.win > 0 && (size_t)k.win * c->ds * sizeof(float) > 65536) continue;  // window must fit LDS
     float ms = 0.f, tot = 0.f;
     for (int rep = 0; rep < 3 && st == SGTSNE_OK; rep++) {
       (void)hipEventRecord(e0, c->stream);
-      st = k.rpb > 0 ? launch_attr_csr_blk(c->stream, c->d, k.g, k.u, k.rpb, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
-           : k.u > 0 ? launch_attr_csr_seg2(c->stream, c->d, k.g, k.u, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr)
-                     : launch_attr_csr_seg(c->stream, c->d, k.g, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
+      st = k.win > 0   ? launch_attr_csr_win(c->stream, c->d, k.g, k.u, k.rpb, k.win, c->n, c->rowptr, c->col,
+                                             c->val, c->Y, c->Fattr)
+           : k.rpb > 0 ? launch_attr_csr_blk(c->stream, c->d, k.g, k.u, k.rpb, c->n, c->rowptr, c->col, c->val,
+                                             c->Y, c->Fattr)
+           : k.u > 0   ? launch_attr_csr_seg2(c->stream, c->d, k.g, k.u, c->n, c->rowptr, c->col, c->val, c->Y,
+                                              c->Fattr)
+                       : launch_attr_csr_seg(c->stream, c->d, k.g, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
       (void)hipEventRecord(e1, c->stream);
       (void)hipEventSynchronize(e1);
       (void)hipEventElapsedTime(&ms, e0, e1);
@@ -567,7 +638,8 @@ int tune_attractive(sgtsne_ctx *c) {
     }
     if (st != SGTSNE_OK) break;
     c->tune_ms.push_back(tot / 2);
-    if (tot / 2 < best) { best = tot / 2; bg = k.g; bu = k.u; brpb = k.rpb; }
+    c->tune_cands.push_back({k.g, k.u, k.rpb, k.win});
+    if (tot / 2 < best) { best = tot / 2; bg = k.g; bu = k.u; brpb = k.rpb; bw = k.win; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -575,6 +647,7 @@ int tune_attractive(sgtsne_ctx *c) {
   c->seg_group = bg;
   c->seg_unroll = bu;
   c->seg_rpb = brpb;
+  c->seg_win = bw;
   return SGTSNE_OK;
 }
 
@@ -964,15 +1037,51 @@ int sgtsne_ctx_info(sgtsne_ctx *c, int *iter, int *n, int *d, int64_t *nnz) {
   return SGTSNE_OK;
 }
 
-int sgtsne_ctx_tuning(sgtsne_ctx *c, int *group, int *unroll, int *rows_per_block, double *ms, int max,
-                      int *count) {
+int sgtsne_ctx_tuning(sgtsne_ctx *c, int *group, int *unroll, int *rows_per_block, int *window, double *ms,
+                      int max, int *count) {
   SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
   if (rows_per_block) *rows_per_block = c->seg_rpb;
+  if (window) *window = c->seg_win;
   if (group) *group = c->seg_group;
   if (unroll) *unroll = c->seg_unroll;
   if (count) *count = (int)c->tune_ms.size();
   if (ms)
     for (int k = 0; k < std::min(max, (int)c->tune_ms.size()); k++) ms[k] = c->tune_ms[k];
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_set_attractive(sgtsne_ctx *c, int group, int unroll, int rows_per_block, int window) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  SGT_CHECK(c->have_P && c->p.format == SGTSNE_FORMAT_CSR_SEG, SGTSNE_ERR_STATE,
+            "pinning the attractive kernel needs P uploaded in the CSR_SEG format");
+  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 && window >= 0 && (window == 0 || rows_per_block > 0),
+            SGTSNE_ERR_ARG, "bad attractive kernel shape");
+  const int keep[4] = {c->seg_group, c->seg_unroll, c->seg_rpb, c->seg_win};
+  c->seg_group = group;
+  c->seg_unroll = unroll;
+  c->seg_rpb = rows_per_block;
+  c->seg_win = window;
+  // one launch checks that the shape exists (it only rewrites Fattr)
+  int st = attractive(c);
+  if (st == SGTSNE_OK && hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_error("attractive kernel failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (st != SGTSNE_OK) {
+    c->seg_group = keep[0];
+    c->seg_unroll = keep[1];
+    c->seg_rpb = keep[2];
+    c->seg_win = keep[3];
+    return st;
+  }
+  c->tune_pinned = true;
+  return SGTSNE_OK;
+}
+
+int sgtsne_ctx_reorders(sgtsne_ctx *c, int *applied, int *declined) {
+  SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
+  if (applied) *applied = c->reorders;
+  if (declined) *declined = c->reorders_declined;
   return SGTSNE_OK;
 }
 
@@ -1019,6 +1128,39 @@ int sgtsne_run(int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, co
   }
   if (st == SGTSNE_OK) st = sgtsne_ctx_get_Y(c, y_out, params->layout);
   sgtsne_ctx_destroy(c);
+  return st;
+}
+
+int sgtsne_copy_probe(int64_t bytes, int reps, double *ms_per_copy) {
+  SGT_CHECK(bytes >= 16 && bytes % 16 == 0 && reps >= 1 && ms_per_copy, SGTSNE_ERR_ARG, "bad probe arguments");
+  hipStream_t s = nullptr;
+  SGT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = alloc_dev((void **)&a, (size_t)bytes);
+  if (!st) st = alloc_dev((void **)&b, (size_t)bytes);
+  if (!st && (hipMemsetAsync(a, 0, (size_t)bytes, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+              hipEventCreate(&e1) != hipSuccess)) {
+    set_error("probe set-up failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  const int64_t cnt = bytes / 16;
+  if (!st) st = launch_stream_copy(s, a, b, cnt);  // warm
+  if (!st) (void)hipEventRecord(e0, s);
+  for (int r = 0; r < reps && !st; r++) st = launch_stream_copy(s, a, b, cnt);
+  if (!st) (void)hipEventRecord(e1, s);
+  if (hipStreamSynchronize(s) != hipSuccess && !st) {
+    set_error("probe copy failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  float t = 0.f;
+  if (!st) (void)hipEventElapsedTime(&t, e0, e1);
+  *ms_per_copy = t / reps;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  free_dev(a);
+  free_dev(b);
+  (void)hipStreamDestroy(s);
   return st;
 }
 

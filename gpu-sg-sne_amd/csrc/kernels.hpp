@@ -47,6 +47,9 @@ int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, con
                          const int32_t *col, const float *val, const float *Y, float *F);
 int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_per_block, int n,
                         const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
+// windowed blocked form: win points of Y around each row range staged in LDS
+int launch_attr_csr_win(hipStream_t s, int d, int group, int unroll, int rows_per_block, int win, int n,
+                        const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
 int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
                         const float *val, const float *Y, float *F);
 int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,
@@ -59,6 +62,7 @@ int launch_kl(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t 
 
 // ---- repulsion (kernels_grid.hip) ----
 int launch_fill(hipStream_t s, float *p, int64_t count, float v);
+int launch_stream_copy(hipStream_t s, const float4 *in, float4 *out, int64_t count);
 int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, RedWork rw,
                          DevScalars *sc);
 // S2G into V [d+1][ng^d] (V is overwritten).  scratch: float workspace for the

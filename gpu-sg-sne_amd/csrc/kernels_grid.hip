@@ -26,6 +26,14 @@ constexpr int kBlock = 256;
 
 enum FinMode { FIN_MINMAX = 0, FIN_ZNEU = 1, FIN_MEAN = 2, FIN_ZEXACT = 3, FIN_KL = 4 };
 
+// ---- plain streaming copy (bandwidth / counter calibration) ----------------------
+__global__ __launch_bounds__(kBlock) void stream_copy_kernel(const float4 *__restrict__ in, float4 *__restrict__ out,
+                                                             int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) out[i] = in[i];
+}
+
 // ---- fill ------------------------------------------------------------------
 __global__ void fill_kernel(float *__restrict__ p, int64_t count, float v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,6 +459,13 @@ int finalize(hipStream_t s, int mode, int nblocks, int d, int n, int apply_mean,
 }
 
 }  // namespace
+
+int launch_stream_copy(hipStream_t s, const float4 *in, float4 *out, int64_t count) {
+  if (count <= 0) return SGTSNE_OK;
+  stream_copy_kernel<<<2048, kBlock, 0, s>>>(in, out, count);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
 
 int launch_fill(hipStream_t s, float *p, int64_t count, float v) {
   if (count <= 0) return SGTSNE_OK;

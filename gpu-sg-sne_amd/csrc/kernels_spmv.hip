@@ -12,6 +12,7 @@
 #include "kernels.hpp"
 #include "../../include/sgtsne.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <type_traits>
 
@@ -159,6 +160,105 @@ __global__ __launch_bounds__(kBlock) void attr_csr_blk_kernel(int n, int rows_pe
     for (; t < e; t += G) {
       float yj[D];
       load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
+// Format 4, windowed form: as the blocked form, but the work-group first
+// copies the window of Y centred on its rows into LDS.  In the locality order
+// most neighbours of a row range lie in that window, so most gathers become
+// LDS reads and only the rest reach L2.  Each lane sums the same entries in
+// the same order as the other forms (only the source of y_j changes), so the
+// result is the same.
+template <int D, int G, int U>
+__global__ __launch_bounds__(kBlock) void attr_csr_win_kernel(int n, int rows_per_block, int nranges, int win,
+                                                              const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ col,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ Y,
+                                                              float *__restrict__ F) {
+  using V = typename VecT<D>::T;
+  extern __shared__ float4 wraw[];
+  V *wy = reinterpret_cast<V *>(wraw);
+  const V *__restrict__ Yv = reinterpret_cast<const V *>(Y);
+  const int per_xcd = (gridDim.x + kXcds - 1) / kXcds;
+  const int range = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (range >= nranges) return;  // the whole work-group leaves together
+  const int r0 = range * rows_per_block;
+  const int r1 = min(n, r0 + rows_per_block);
+  const int wn = min(win, n);
+  const int c0 = max(0, min(((r0 + r1) >> 1) - (wn >> 1), n - wn));
+  {
+    // eight loads in flight per thread before the LDS writes
+    constexpr int SU = 8;
+    for (int i0 = 0; i0 < wn; i0 += kBlock * SU) {
+      V tmp[SU];
+#pragma unroll
+      for (int k = 0; k < SU; ++k) {
+        const int i = i0 + k * kBlock + (int)threadIdx.x;
+        if (i < wn) tmp[k] = Yv[c0 + i];
+      }
+#pragma unroll
+      for (int k = 0; k < SU; ++k) {
+        const int i = i0 + k * kBlock + (int)threadIdx.x;
+        if (i < wn) wy[i] = tmp[k];
+      }
+    }
+  }
+  __syncthreads();
+  // one generic-address load per neighbour: the LDS copy when it is in the
+  // window, Y otherwise; no branch, so the loads of an unrolled step overlap
+  const V *wg = wy;
+  auto src = [&](int c) -> const V * {
+    const unsigned u = (unsigned)(c - c0);
+    return u < (unsigned)wn ? wg + u : Yv + c;
+  };
+  auto unpack = [&](const V &v, float (&yj)[D]) {
+    if constexpr (D == 1) {
+      yj[0] = v;
+    } else if constexpr (D == 2) {
+      yj[0] = v.x; yj[1] = v.y;
+    } else {
+      yj[0] = v.x; yj[1] = v.y; yj[2] = v.z;
+    }
+  };
+  auto gather = [&](int c, float (&yj)[D]) { unpack(*src(c), yj); };
+  const int grp = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kBlock / G;
+  for (int row = r0 + grp; row < r1; row += NG) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    const int e = rowptr[row + 1];
+    int t = rowptr[row] + lane;
+    for (; t + (U - 1) * G < e; t += U * G) {
+      int c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
+      V w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = *src(c[u]);
+      float yj[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u) unpack(w[u], yj[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
+    }
+    for (; t < e; t += G) {
+      float yj[D];
+      gather(__builtin_nontemporal_load(col + t), yj);
       accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
     }
 #pragma unroll
@@ -357,6 +457,39 @@ int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_pe
   per_d(I2{});
   per_d(I3{});
   SGT_CHECK(ok, SGTSNE_ERR_ARG, "no blocked attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_win(hipStream_t s, int d, int group, int unroll, int rows_per_block, int win, int n,
+                        const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  const int ds = d == 3 ? 4 : d;
+  const int wn = std::min(win, n);
+  const size_t lds = (size_t)wn * ds * sizeof(float);
+  SGT_CHECK(win >= 64 && lds <= 65536, SGTSNE_ERR_ARG, "window of %d points does not fit LDS", win);
+  const int nranges = div_up(n, rows_per_block);
+  const int blocks = div_up(nranges, kXcds) * kXcds;
+  bool ok = false;
+  auto go = [&](auto dd, auto gg, auto uu) {
+    constexpr int DD = decltype(dd)::value, GG = decltype(gg)::value, UU = decltype(uu)::value;
+    if (d == DD && group == GG && unroll == UU) {
+      attr_csr_win_kernel<DD, GG, UU><<<blocks, kBlock, lds, s>>>(n, rows_per_block, nranges, win, rowptr, col, val,
+                                                                  Y, F);
+      ok = true;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  auto per_g = [&](auto dd, auto uu) { go(dd, I16{}, uu); go(dd, I32{}, uu); };
+  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); };
+  per_d(I1{});
+  per_d(I2{});
+  per_d(I3{});
+  SGT_CHECK(ok, SGTSNE_ERR_ARG, "no windowed attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }

@@ -37,11 +37,11 @@ EXPORTED = [
     "sgtsne_ctx_get_P", "sgtsne_ctx_set_Y", "sgtsne_ctx_get_Y", "sgtsne_ctx_step",
     "sgtsne_ctx_terms", "sgtsne_ctx_kl", "sgtsne_ctx_repulsive_exact", "sgtsne_ctx_timing",
     "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
-    "sgtsne_ctx_tuning",
+    "sgtsne_ctx_tuning", "sgtsne_ctx_set_attractive", "sgtsne_ctx_reorders",
     "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
     "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
-    "sgtsne_batch_set_graph",
+    "sgtsne_batch_set_graph", "sgtsne_copy_probe",
 ]
 
 
@@ -105,9 +105,11 @@ def lib():
         "sgtsne_ctx_repulsive_exact": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
         "sgtsne_ctx_timing": (C.c_int, [vp, f64p]),
         "sgtsne_ctx_set_timing": (C.c_int, [vp, C.c_int]),
-        "sgtsne_ctx_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp,
-                                        C.c_int, C.POINTER(C.c_int)]),
+        "sgtsne_ctx_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), vp, C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_ctx_errors": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
+        "sgtsne_ctx_set_attractive": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "sgtsne_ctx_reorders": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_ctx_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
         "sgtsne_ctx_destroy": (C.c_int, [vp]),
@@ -126,8 +128,9 @@ def lib():
         "sgtsne_batch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "sgtsne_batch_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                          C.POINTER(C.c_int)]),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_batch_set_graph": (C.c_int, [vp, C.c_int]),
+        "sgtsne_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -141,6 +144,13 @@ def _check(status):
     if status != 0:
         msg = lib().sgtsne_last_error().decode(errors="replace")
         raise SgtsneError(f"sgtsne error {status}: {msg}")
+
+
+def copy_probe(nbytes, reps=10):
+    """ms of one device-to-device streaming copy of nbytes (plain kernel)."""
+    ms = C.c_double()
+    _check(lib().sgtsne_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def version():
@@ -293,15 +303,24 @@ class Context:
         return out
 
     def tuning(self):
-        g, u, rpb, cnt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        g, u, rpb, win, cnt = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         ms = np.zeros(32)
-        _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), ms.ctypes.data, 32,
-                                       C.byref(cnt)))
-        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value,
-                "candidates_ms": [round(x, 4) for x in ms[:cnt.value]]}
+        _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), C.byref(win), ms.ctypes.data,
+                                       32, C.byref(cnt)))
+        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value, "window": win.value,
+                "candidates_ms": [round(float(x), 4) for x in ms[:cnt.value]]}
 
     def set_timing(self, on):
         _check(lib().sgtsne_ctx_set_timing(self._h, int(bool(on))))
+
+    def reorders(self):
+        a, dcl = C.c_int(), C.c_int()
+        _check(lib().sgtsne_ctx_reorders(self._h, C.byref(a), C.byref(dcl)))
+        return {"applied": a.value, "declined": dcl.value}
+
+    def set_attractive(self, group, unroll=0, rows_per_block=0, window=0):
+        """Pins the attractive kernel shape (CSR_SEG format)."""
+        _check(lib().sgtsne_ctx_set_attractive(self._h, int(group), int(unroll), int(rows_per_block), int(window)))
 
     def errors(self):
         cnt = C.c_int()
@@ -421,9 +440,10 @@ class Batch:
         return {"jobs": j.value, "points": p.value, "nnz": z.value, "grid_bytes": g.value, "graph_replays": r.value}
 
     def tuning(self):
-        g, u, rpb, gr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        _check(lib().sgtsne_batch_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), C.byref(gr)))
-        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value, "graph": bool(gr.value)}
+        g, u, rpb, win, gr = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().sgtsne_batch_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), C.byref(win), C.byref(gr)))
+        return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value, "window": win.value,
+                "graph": bool(gr.value)}
 
     def set_graph(self, on):
         _check(lib().sgtsne_batch_set_graph(self._h, int(bool(on))))

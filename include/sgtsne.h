@@ -191,11 +191,25 @@ int sgtsne_ctx_repulsive_exact(sgtsne_ctx *ctx, float *Frep, double *Z, int layo
 int sgtsne_ctx_timing(sgtsne_ctx *ctx, double *out7);
 
 /* Choice of the attractive-kernel autotuner (run at upload for the default
- * format): lanes per row, unroll (0 = plain form), rows per work-group (0 =
- * grid-stride form), and the measured ms of each candidate (up to max;
- * *count receives how many were timed). */
-int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, int *rows_per_block, double *ms, int max,
-                      int *count);
+ * format, and again after the first Morton re-order): lanes per row, unroll
+ * (0 = plain form), rows per work-group (0 = grid-stride form), points of Y
+ * staged in LDS per work-group (0 = none), and the measured ms of each
+ * candidate (up to max; *count receives how many were timed). */
+int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, int *rows_per_block, int *window, double *ms,
+                      int max, int *count);
+
+/* Pins the attractive kernel of the CSR_SEG format to one shape (same meaning
+ * as sgtsne_ctx_tuning's outputs) and stops re-tuning: shapes with the same
+ * lanes per row sum every row in the same order, so pinning the lanes makes
+ * runs bit-reproducible across machines whose autotuner would choose
+ * differently.  Call after upload_P; an unsupported shape returns
+ * SGTSNE_ERR_ARG and leaves the current one.  No reference counterpart (the
+ * reference's kernel is fixed by its format, CU/pq.cu:25-57). */
+int sgtsne_ctx_set_attractive(sgtsne_ctx *ctx, int group, int unroll, int rows_per_block, int window);
+
+/* Morton re-orders of the points applied so far, and those tried but
+ * declined because the attractive kernel measured slower on them. */
+int sgtsne_ctx_reorders(sgtsne_ctx *ctx, int *applied, int *declined);
 
 /* Enables (default) or disables the per-phase event timing of ctx_step. */
 int sgtsne_ctx_set_timing(sgtsne_ctx *ctx, int on);
@@ -218,6 +232,12 @@ int sgtsne_ctx_destroy(sgtsne_ctx *ctx);
  * batch (bit-identical alone and in any mix).  The reference has no batched
  * form: it runs one sgtsneCUDA call (CU/sgtsne.cu:26-128) per embedding. */
 typedef struct sgtsne_batch sgtsne_batch;
+
+/* Device bandwidth probe: copies `bytes` (a multiple of 16) between two
+ * device buffers with a plain streaming kernel `reps` times and returns the
+ * ms of one copy.  Calibrates HBM counters and the roofline; no reference
+ * counterpart. */
+int sgtsne_copy_probe(int64_t bytes, int reps, double *ms_per_copy);
 
 /* Capacity: job slots (<= 1024), points and P_sym nonzeros of the arena, and
  * the byte budget of the per-job grid buffers (<= 0: 64 GiB). */
@@ -265,9 +285,11 @@ int sgtsne_batch_remove(sgtsne_batch *batch, int job);
 int sgtsne_batch_info(sgtsne_batch *batch, int *jobs, int64_t *points, int64_t *nnz, int64_t *grid_bytes,
                       int64_t *graph_replays);
 
-/* Attractive-kernel shape chosen by the batch autotuner and whether
- * iterations are graph-replayed. */
-int sgtsne_batch_tuning(sgtsne_batch *batch, int *group, int *unroll, int *rows_per_block, int *use_graph);
+/* Attractive-kernel shape chosen by the batch autotuner (lanes per row fixed
+ * at 16, so the choice never changes a result) and whether iterations are
+ * graph-replayed. */
+int sgtsne_batch_tuning(sgtsne_batch *batch, int *group, int *unroll, int *rows_per_block, int *window,
+                        int *use_graph);
 
 /* Graph replay on (default) or off (iterations launched one by one). */
 int sgtsne_batch_set_graph(sgtsne_batch *batch, int on);

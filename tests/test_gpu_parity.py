@@ -389,3 +389,40 @@ def test_reorder_modes_agree(sg, O):
             kls[mode] = ctx.kl(1.0)
     for mode in (1, 2):
         assert abs(kls[mode] - kls[0]) / kls[0] < 0.02, kls
+
+
+# every shape of the default CSR kernel: same result as the oracle, and the
+# same bits for every shape with the same lanes per row
+_SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0, 0), (16, 1, 64, 0), (16, 2, 128, 0),
+           (16, 1, 256, 4096), (16, 2, 512, 4096), (16, 2, 1024, 8192), (32, 2, 256, 0), (32, 1, 512, 4096)]
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_attractive_kernel_shapes(sg, O, d):
+    n = 12000
+    X = _mixture(n, 20, 8, 90 + d)
+    I, D = O.knn_exact(X, 30)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 10.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(d)
+    Y = _clustered_embedding(n, d, rng)
+    got = {}
+    with sg.Context(sg.tsneparams(d=d, reorder=1)) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        ctx.set_Y(Y)
+        Yc = ctx.get_Y()
+        Fao = O.attractive(So, Yc.astype(np.float64))
+        for shp in _SHAPES:
+            g, u, rpb, win = shp
+            if win * (4 if d == 3 else d) * 4 > 65536:
+                with pytest.raises(sg.SgtsneError):
+                    ctx.set_attractive(*shp)
+                continue
+            ctx.set_attractive(*shp)
+            assert ctx.tuning()["window"] == win
+            Fa, _, _, _ = ctx.terms()
+            assert rel_l2(Fa, Fao) < TOL, shp
+            got[shp] = Fa
+    for shp, Fa in got.items():
+        ref = got[(shp[0], 0, 0, 0)]
+        assert np.array_equal(Fa, ref), shp
