@@ -46,26 +46,42 @@ def workload(name, n_override=None):
     return X
 
 
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(P, Y, d, it0, target_s, max_iters):
-    """FP64 oracle (OpenMP, all host cores) on the same P and Y snapshot."""
+    """FP64 oracle (OpenMP, all host cores, then one core) on the same P and Y
+    snapshot: ms per gradient iteration."""
     from oracle import oracle as O
 
     O.lib()
     nthreads = O.get_threads()
     So = O.CSR(P.n, P.rowptr.astype(np.int64), P.col, P.val.astype(np.float64))
-    Yd = Y.astype(np.float64).copy()
-    uY = np.zeros_like(Yd)
-    g = np.ones_like(Yd)
-    t0 = time.perf_counter()
-    O.run(So, Yd, uY, g, it0, it0 + 1)
-    t1 = time.perf_counter()
-    per = t1 - t0
-    iters = int(max(1, min(max_iters, target_s / max(per, 1e-6)) - 1))
-    t0 = time.perf_counter()
-    O.run(So, Yd, uY, g, it0 + 1, it0 + 1 + iters)
-    t1 = time.perf_counter()
-    ms = (t1 - t0) / iters * 1e3
-    return ms, nthreads, iters + 1
+
+    def timed(iters_cap, seconds):
+        Yd = Y.astype(np.float64).copy()
+        uY = np.zeros_like(Yd)
+        g = np.ones_like(Yd)
+        t0 = time.perf_counter()
+        O.run(So, Yd, uY, g, it0, it0 + 1)
+        per = time.perf_counter() - t0
+        iters = int(max(1, min(iters_cap, seconds / max(per, 1e-6)) - 1))
+        t0 = time.perf_counter()
+        O.run(So, Yd, uY, g, it0 + 1, it0 + 1 + iters)
+        return (time.perf_counter() - t0) / iters * 1e3, iters + 1
+
+    ms, iters = timed(max_iters, target_s)
+    O.set_threads(1)
+    ms1, iters1 = timed(3, 4.0)
+    O.set_threads(nthreads)
+    return ms, nthreads, iters, ms1, iters1
 
 
 def plain_job(sg, X, d, k, u, y0, max_iter=1000):
@@ -83,7 +99,7 @@ def job_kl(sg, P, Y, d):
         return ctx.kl(1.0)
 
 
-def batched_leg(sg, jobs, sample, k=90, u=30.0):
+def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
     """256 jobs (N log-uniform in [500, 50,000], 50-D 5..20-cluster mixtures)
     in one batched engine against the plain method one job at a time."""
     from sgtsne_amd import synth
@@ -93,6 +109,7 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0):
     total = int(sizes.sum())
     log(f"[bench] batched: {jobs} jobs, {total} points")
     B = sg.Batch(sg.tsneparams(d=2), jobs, total, 2 * k * total)
+    B.set_graph(graph)
     t_front = 0.0
     ids = []
     for i in range(jobs):
@@ -134,7 +151,7 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0):
     return out
 
 
-def served_leg(sg, requests, rate, tick, k=90, u=30.0):
+def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True):
     """Poisson arrivals (N log-uniform in [1,000, 200,000], d in {2, 3}) through
     the continuous-batching server, then one at a time through the plain method."""
     from sgtsne_amd import serving, synth
@@ -147,6 +164,8 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0):
 
     log(f"[bench] served: {requests} requests at {rate}/s, N total {sum(a[1] for a in arrivals)}")
     with serving.Server(tick=tick, k=k, u=u) as srv:
+        for e in srv.engines.values():
+            e.set_graph(graph)
         t0 = time.perf_counter()
         got = srv.run(arrivals, make_X)
         t_srv = time.perf_counter() - t0
@@ -164,36 +183,8 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="powerlaw40_50d")
-    ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--d", type=int, default=2)
-    ap.add_argument("--format", default="CSR_SEG")
-    ap.add_argument("--reorder", type=int, default=2, help="device point order: 0 given, 1 BFS of P, 2 BFS + Morton of Y")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-max-iters", type=int, default=30)
-    ap.add_argument("--calibrate", action="store_true", help="also time a plain 256 MiB device copy")
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--batch-jobs", type=int, default=256)
-    ap.add_argument("--batch-sample", type=int, default=16)
-    ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=40.0)
-    ap.add_argument("--serve-tick", type=int, default=25)
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
-        sys.exit(2)
-
-    import sgtsne_amd as sg
-
+def main_leg(sg, args):
+    """N=1.3M (by default) single-job ms/iter with roofline, CPU baseline and KL check."""
     k, u = 90, 30.0
     t_setup = time.perf_counter()
     log(f"[bench] generating {args.workload} ...")
@@ -213,6 +204,8 @@ def main():
     t0 = time.perf_counter()
     ctx.upload_P(P, symmetrize=True)
     t_sym = time.perf_counter() - t0
+    if args.pin_attractive:
+        ctx.set_attractive(*[int(x) for x in args.pin_attractive.split(",")])
     info = ctx.info()
     nnz = info["nnz"]
     tune = ctx.tuning()
@@ -221,6 +214,7 @@ def main():
 
     y0 = sg.synth.random_init(n, args.d, seed=1)
     ctx.set_Y(y0)
+    y0c = ctx.get_Y() if args.kl_check else None  # the centred start both runs use
     ctx.set_timing(True)
     t0 = time.perf_counter()
     if args.warmup > 0:
@@ -261,6 +255,15 @@ def main():
                 "bytes_per_launch": bytes_b,
                 "avg_launch_ms": round(phases["attractive"], 5)}
 
+    # HBM traffic per launch of the same kernel on the same workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain copy)
+    prof = os.path.join(ROOT, "profiles", "attractive_spmv_powerlaw1.3M_d2.json")
+    if os.path.exists(prof) and args.workload == "powerlaw40_50d" and args.d == 2 and n == 1_300_000:
+        pj = json.load(open(prof))
+        if pj["kernel"].replace(" ", "") == roofline["kernel"].replace(" ", ""):
+            roofline["traffic"] = pj["traffic_bytes_per_launch"]
+            roofline["traffic_source"] = "profiles/attractive_spmv_powerlaw1.3M_d2.json (rocprofv3 --pmc)"
+
     cpu = None
     term_err = None
     if not args.no_cpu:
@@ -278,12 +281,98 @@ def main():
         log(f"[bench] FP32 vs FP64 oracle terms: {term_err}")
         del So
         log(f"[bench] CPU baseline: FP64 oracle from the iteration-{it0} Y ...")
-        ms_cpu, cores, cpu_iters = cpu_baseline(P_sym, Y_snap, args.d, it0, args.cpu_seconds, args.cpu_max_iters)
+        ms_cpu, cores, cpu_iters, ms1, it1 = cpu_baseline(P_sym, Y_snap, args.d, it0, args.cpu_seconds,
+                                                          args.cpu_max_iters)
         cpu = {"value": round(ms_cpu, 2), "unit": "ms/iter", "cores": cores, "kind": "port",
                "sample": f"{cpu_iters} iterations (from iteration {it0}) of the FP64 C/OpenMP oracle on the "
-                         f"same P_sym (nnz={nnz}) and the GPU's iteration-{it0} Y, own FFT (no FFTW on host)"}
+                         f"same P_sym (nnz={nnz}) and the GPU's iteration-{it0} Y, own FFT (no FFTW on host)",
+               "host_cpu": host_cpu_model(), "one_thread_ms": round(ms1, 1), "one_thread_iters": it1}
         log(f"[bench] CPU {ms_cpu:.1f} ms/iter on {cores} threads -> GPU speed-up {ms_cpu / ms_per_step:.1f}x")
+    kl_check = None
+    if args.kl_check:
+        from oracle import oracle as O
+
+        P_sym = ctx.get_P()
+        So = O.CSR(P_sym.n, P_sym.rowptr.astype(np.int64), P_sym.col, P_sym.val.astype(np.float64))
+        iters = it0 + args.steps
+        log(f"[bench] KL check: FP64 oracle, {iters} iterations on the host ...")
+        Yo = y0c.astype(np.float64)
+        t0 = time.perf_counter()
+        O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, iters, zmode=O.ZETA_DIRECT)
+        t_o = time.perf_counter() - t0
+        _, Zo, _ = O.repulsive_interp(Yo.copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d], O.ZETA_DIRECT)
+        _, Zg, _ = O.repulsive_interp(Y_end.astype(np.float64).copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d],
+                                      O.ZETA_DIRECT)
+        klo = O.kl(So, Yo, Zo)
+        klg = O.kl(So, Y_end.astype(np.float64), Zg)
+        kl_check = {"iterations": iters, "kl_gpu_fp32": round(klg, 6), "kl_oracle_fp64": round(klo, 6),
+                    "rel_diff": round(abs(klg - klo) / klo, 6), "oracle_s": round(t_o, 1),
+                    "oracle_threads": O.get_threads()}
+        log(f"[bench] KL check: {kl_check}")
+        del So
     ctx.close()
+
+    return {
+        "value": round(ms_per_step, 4),
+        "ms_per_step": round(ms_per_step, 4),
+        "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
+                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder,
+                   "morton_reorders": reorders, "timed_iterations": [it0, it0 + args.steps - 1],
+                   "final_grid_side": grid, "final_kl": round(kl, 5), "attractive_tuning": tune},
+        "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
+        "full_run_ms": round((t_warm + t_run) * 1e3, 1),
+        "setup_s": {"knn": round(t_knn, 2), "perplexity": round(t_perp, 2), "symmetrize_upload": round(t_sym, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "term_error_vs_fp64": term_err,
+        "kl_check": kl_check,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--workload", default="powerlaw40_50d")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--d", type=int, default=2)
+    ap.add_argument("--format", default="CSR_SEG")
+    ap.add_argument("--reorder", type=int, default=2, help="device point order: 0 given, 1 BFS of P, 2 BFS + Morton of Y")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-iters", type=int, default=30)
+    ap.add_argument("--calibrate", action="store_true", help="also time a plain 256 MiB device copy")
+    ap.add_argument("--pin-attractive", default=None,
+                    help="G,U,RPB,WIN: fix the attractive kernel shape instead of autotuning (profiling)")
+    ap.add_argument("--kl-check", action="store_true",
+                    help="run the FP64 oracle for all iterations from the same P_sym and start; compare final KL")
+    ap.add_argument("--legs", default="main,batched,served",
+                    help="comma list of main (N=1.3M ms/iter), batched (256 jobs), served (request stream)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-sample", type=int, default=16)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-tick", type=int, default=25)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched and served engines launch each kernel instead of replaying a graph")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import sgtsne_amd as sg
+
+    legs = set(args.legs.split(","))
+    if args.no_batched:
+        legs.discard("batched")
+    if args.no_served:
+        legs.discard("served")
+    m = main_leg(sg, args) if "main" in legs else {}
 
     copy = None
     if args.calibrate:
@@ -291,33 +380,33 @@ def main():
         ms = sg.copy_probe(nb, 20)
         copy = {"bytes": nb, "ms": round(ms, 5), "GB_per_s": round(2 * nb / ms / 1e6, 1)}
         log(f"[bench] plain copy: {copy}")
-    batched = None if args.no_batched else batched_leg(sg, args.batch_jobs, args.batch_sample)
-    served = None if args.no_served else served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick)
+    g = not args.no_graph
+    batched = batched_leg(sg, args.batch_jobs, args.batch_sample, graph=g) if "batched" in legs else None
+    served = served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g) if "served" in legs else None
 
+    value = m.get("value")
     out = {
         "metric": METRIC,
-        "value": round(ms_per_step, 4),
+        "value": value,
         "unit": "ms/iter",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": m.get("ms_per_step"),
         "higher_is_better": False,
         "scaling": "strong",
-        "vs_baseline": round(ms_per_step / PUBLISHED_MS_PER_ITER, 6),
+        "vs_baseline": round(value / PUBLISHED_MS_PER_ITER, 6) if value else None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
-                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder, "morton_reorders": reorders, "timed_iterations": [it0, it0 + args.steps - 1],
-                   "final_grid_side": grid, "final_kl": round(kl, 5),
-                   "attractive_tuning": tune},
-        "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
-        "full_run_ms": round((t_warm + t_run) * 1e3, 1),
-        "setup_s": {"knn": round(t_knn, 2), "perplexity": round(t_perp, 2), "symmetrize_upload": round(t_sym, 2)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "term_error_vs_fp64": term_err,
+        "config": m.get("config", {"workload": "none (main leg skipped)"}),
+        "phases_ms": m.get("phases_ms"),
+        "full_run_ms": m.get("full_run_ms"),
+        "setup_s": m.get("setup_s"),
+        "roofline": m.get("roofline"),
+        "cpu_baseline": m.get("cpu_baseline"),
+        "term_error_vs_fp64": m.get("term_error_vs_fp64"),
         "copy_probe": copy,
+        "kl_check": m.get("kl_check"),
         "batched": batched,
         "served": served,
     }
