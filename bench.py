@@ -248,7 +248,8 @@ def main_leg(sg, args):
     achieved = bytes_b / t_spmv / 1e9 if t_spmv > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": (f"attr_csr_win_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["window"]
+                "kernel": (f"attr_csr_loc_kernel<{args.d},{tune['group']}>" if tune["window"] < 0
+                           else f"attr_csr_win_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["window"]
                            else f"attr_csr_blk_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
                            else f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
                            else f"attr_csr_seg_kernel<{args.d},{tune['group']}>"),

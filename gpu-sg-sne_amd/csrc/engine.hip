@@ -90,7 +90,13 @@ struct sgtsne_ctx {
   int seg_group = 16;
   int seg_unroll = 0;  // 0: plain segmented kernel; else the unrolled form
   int seg_rpb = 0;     // > 0: blocked form with this many rows per work-group
-  int seg_win = 0;     // > 0: windowed blocked form, this many points of Y staged in LDS
+  int seg_win = 0;     // > 0: windowed blocked form, this many points of Y staged in LDS;
+                       // -1: column-localised blocks (loc_* below)
+  // column-localised blocks: row ranges, their distinct columns, 16-bit local indices
+  int loc_nblk = 0, loc_cap = 0;
+  int32_t *loc_brow = nullptr, *loc_bu = nullptr, *loc_ucol = nullptr;
+  uint16_t *loc_lidx = nullptr;
+  double loc_reuse = 0.0;  // nonzeros per staged column
   std::vector<double> tune_ms;  // autotuner timings (ms per candidate)
   std::vector<std::array<int, 4>> tune_cands;
   bool tune_pinned = false;  // the caller fixed the attractive kernel: no re-tuning
@@ -146,7 +152,17 @@ void free_dev(T *&p) {
   p = nullptr;
 }
 
+void free_loc(sgtsne_ctx *c) {
+  free_dev(c->loc_brow);
+  free_dev(c->loc_bu);
+  free_dev(c->loc_ucol);
+  free_dev(c->loc_lidx);
+  c->loc_nblk = 0;
+  if (c->seg_win < 0) c->seg_win = 0;
+}
+
 void free_P(sgtsne_ctx *c) {
+  free_loc(c);
   free_dev(c->rowptr);
   free_dev(c->col);
   free_dev(c->val);
@@ -246,6 +262,9 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
 int attractive(sgtsne_ctx *c) {
   switch (c->p.format) {
     case SGTSNE_FORMAT_CSR_SEG:
+      if (c->seg_win < 0)
+        return launch_attr_csr_loc(c->stream, c->d, c->seg_group, c->loc_nblk, c->loc_cap, c->loc_brow, c->loc_bu,
+                                   c->loc_ucol, c->rowptr, c->loc_lidx, c->val, c->Y, c->Fattr);
       if (c->seg_win > 0)
         return launch_attr_csr_win(c->stream, c->d, c->seg_group, c->seg_unroll, c->seg_rpb, c->seg_win, c->n,
                                    c->rowptr, c->col, c->val, c->Y, c->Fattr);
@@ -329,12 +348,21 @@ void harvest_timing(sgtsne_ctx *c, PhaseEvents &pe, int it) {
 constexpr int kSpatialReorderIters[] = {50, 150, 300, 600};
 
 int tune_attractive(sgtsne_ctx *c);
+int build_localised(sgtsne_ctx *c);
 
 // Mean ms of the current attractive kernel over 3 launches (after a warm one)
 // on the given P and Y; the context's own arrays are restored afterwards.
 int time_attractive(sgtsne_ctx *c, int32_t *rp, int32_t *col, float *val, float *Y, float *ms) {
   int32_t *k_rp = c->rowptr, *k_col = c->col;
   float *k_val = c->val, *k_Y = c->Y;
+  // the column-localised blocks belong to one order: compare orders with the
+  // plain kernel of the same lanes per row
+  const int k_win = c->seg_win, k_unroll = c->seg_unroll, k_rpb = c->seg_rpb;
+  if (c->seg_win < 0) {
+    c->seg_win = 0;
+    c->seg_unroll = 0;
+    c->seg_rpb = 0;
+  }
   c->rowptr = rp;
   c->col = col;
   c->val = val;
@@ -355,6 +383,9 @@ int time_attractive(sgtsne_ctx *c, int32_t *rp, int32_t *col, float *val, float 
   c->col = k_col;
   c->val = k_val;
   c->Y = k_Y;
+  c->seg_win = k_win;
+  c->seg_unroll = k_unroll;
+  c->seg_rpb = k_rpb;
   if (st == SGTSNE_ERR_HIP) set_error("timing of the attractive kernel failed");
   return st;
 }
@@ -437,6 +468,12 @@ int spatial_reorder(sgtsne_ctx *c) {
   for (int r = 0; r < n; r++) c->iperm[c->perm[r]] = r;
   c->permuted = true;
   c->reorders++;
+  // the column-localised blocks follow the order
+  const bool had_loc = c->loc_nblk > 0, using_loc = c->seg_win < 0;
+  if (had_loc) {
+    SGT_TRY(build_localised(c));
+    if (using_loc && c->loc_nblk > 0) c->seg_win = -1;
+  }
   // the first re-order changes the gather pattern most: choose the kernel again
   if (c->reorders == 1 && c->p.format == SGTSNE_FORMAT_CSR_SEG && !c->tune_pinned) SGT_TRY(tune_attractive(c));
   return SGTSNE_OK;
@@ -523,27 +560,93 @@ void unpack_y(const std::vector<float> &in, int layout, int n, int d, int ds, co
 // sort): each component from its lowest-degree vertex, neighbours in column
 // order.  Rows that are neighbours in P end up close in memory, so the
 // gathers of the attractive term stay in cache.
-void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t> &perm) {
-  perm.assign(n, 0);
-  std::vector<uint8_t> seen(n, 0);
-  int start = 0, mind = INT32_MAX;
-  for (int i = 0; i < n; i++)
-    if (rp[i + 1] - rp[i] < mind) { mind = rp[i + 1] - rp[i]; start = i; }
-  int head = 0, tail = 0, seed = 0;
-  seen[start] = 1;
-  perm[tail++] = start;
-  while (tail < n) {
-    if (head == tail) {
-      while (seen[seed]) ++seed;
-      seen[seed] = 1;
-      perm[tail++] = seed;
+// Reverse Cuthill-McKee order of the graph of P: per connected component, a
+// breadth-first sweep from a pseudo-peripheral vertex (George-Liu: repeat
+// the sweep from a minimum-degree vertex of the last level while the depth
+// grows) that takes the unvisited neighbours of each vertex in increasing
+// degree, the whole order then reversed.  Compared with a plain breadth-first
+// order it keeps the neighbours of consecutive rows in a much narrower band
+// of indices, so the gathers of consecutive rows share cache lines.
+namespace {
+
+// Breadth-first levels from `root` over unvisited vertices (mark != stamp);
+// returns the depth and leaves the last level in [lo, hi) of `q`.
+int bfs_levels(int root, const int32_t *rp, const int32_t *col, std::vector<int32_t> &mark, int stamp,
+               std::vector<int32_t> &q, size_t &lo, size_t &hi) {
+  q.clear();
+  q.push_back(root);
+  mark[root] = stamp;
+  size_t head = 0;
+  int depth = 0;
+  lo = 0;
+  hi = 1;
+  while (head < q.size()) {
+    const size_t end = q.size();
+    lo = head;
+    hi = end;
+    for (; head < end; ++head) {
+      const int v = q[head];
+      for (int t = rp[v]; t < rp[v + 1]; t++) {
+        const int u = col[t];
+        if (mark[u] != stamp) {
+          mark[u] = stamp;
+          q.push_back(u);
+        }
+      }
     }
-    const int v = perm[head++];
-    for (int t = rp[v]; t < rp[v + 1]; t++) {
-      const int u = col[t];
-      if (!seen[u]) { seen[u] = 1; perm[tail++] = u; }
+    if (q.size() > end) ++depth;
+  }
+  return depth;
+}
+
+}  // namespace
+
+void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t> &perm) {
+  perm.clear();
+  perm.reserve(n);
+  std::vector<int32_t> deg(n), mark(n, 0), q;
+  std::vector<uint8_t> placed(n, 0);
+  for (int i = 0; i < n; i++) deg[i] = rp[i + 1] - rp[i];
+  // candidate starts in increasing degree (ties by index)
+  std::vector<int32_t> by_deg(n);
+  for (int i = 0; i < n; i++) by_deg[i] = i;
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  int stamp = 1;
+  std::vector<std::pair<int32_t, int32_t>> nb;
+  for (int cand : by_deg) {
+    if (placed[cand]) continue;
+    // pseudo-peripheral vertex of cand's component
+    int root = cand;
+    size_t lo = 0, hi = 0;
+    int depth = bfs_levels(root, rp, col, mark, ++stamp, q, lo, hi);
+    for (int sweep = 0; sweep < 4; sweep++) {
+      int best = q[lo];
+      for (size_t t = lo; t < hi; t++)
+        if (deg[q[t]] < deg[best] || (deg[q[t]] == deg[best] && q[t] < best)) best = q[t];
+      const int d2 = bfs_levels(best, rp, col, mark, ++stamp, q, lo, hi);
+      if (d2 <= depth) break;
+      depth = d2;
+      root = best;
+    }
+    // Cuthill-McKee sweep of the component from root
+    const size_t first = perm.size();
+    perm.push_back(root);
+    placed[root] = 1;
+    for (size_t head = first; head < perm.size(); ++head) {
+      const int v = perm[head];
+      nb.clear();
+      for (int t = rp[v]; t < rp[v + 1]; t++) {
+        const int u = col[t];
+        if (!placed[u]) {
+          placed[u] = 1;
+          nb.emplace_back(deg[u], u);
+        }
+      }
+      std::sort(nb.begin(), nb.end());
+      for (const auto &e : nb) perm.push_back(e.second);
     }
   }
+  std::reverse(perm.begin(), perm.end());
 }
 
 int apply_locality_order(sgtsne_ctx *c) {
@@ -601,6 +704,63 @@ int alloc_points(sgtsne_ctx *c) {
 // Autotuner of the attractive kernel: times every (lanes per row, unroll)
 // candidate on the uploaded matrix and keeps the fastest.  Runs once per
 // upload; the result is reported by sgtsne_ctx_tuning.
+// Column-localised blocks of the current P (host build, once per order):
+// consecutive rows are grouped while their distinct columns stay within
+// `cap` (the LDS copy of Y) and the rows within 1024; each block's distinct
+// columns are sorted, so the LDS staging loads of dense column runs coalesce.
+int build_localised(sgtsne_ctx *c) {
+  free_loc(c);
+  const int n = c->n;
+  const int cap = c->d == 3 ? 4096 : 8192;
+  const int rmax = 1024;
+  std::vector<int32_t> rp((size_t)n + 1), col((size_t)std::max<int64_t>(c->nnz, 1));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  SGT_HIP(hipMemcpy(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost));
+  if (c->nnz > 0) SGT_HIP(hipMemcpy(col.data(), c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++)
+    if (rp[i + 1] - rp[i] > cap) return SGTSNE_OK;  // a row wider than the LDS copy: form unavailable
+  std::vector<int32_t> brow, bu, ucol, mark(n, -1), pos(n, 0);
+  std::vector<uint16_t> lidx((size_t)std::max<int64_t>(c->nnz, 1));
+  brow.push_back(0);
+  bu.push_back(0);
+  int r = 0, blk = 0;
+  std::vector<int32_t> cur;
+  while (r < n) {
+    cur.clear();
+    const int rstart = r;
+    while (r < n && r - rstart < rmax) {
+      int fresh = 0;
+      for (int t = rp[r]; t < rp[r + 1]; t++) fresh += mark[col[t]] != blk;
+      if ((int)cur.size() + fresh > cap && r > rstart) break;
+      for (int t = rp[r]; t < rp[r + 1]; t++)
+        if (mark[col[t]] != blk) {
+          mark[col[t]] = blk;
+          cur.push_back(col[t]);
+        }
+      ++r;
+    }
+    std::sort(cur.begin(), cur.end());
+    for (size_t k = 0; k < cur.size(); k++) pos[cur[k]] = (int32_t)k;
+    for (int t = rp[rstart]; t < rp[r]; t++) lidx[t] = (uint16_t)pos[col[t]];
+    ucol.insert(ucol.end(), cur.begin(), cur.end());
+    brow.push_back(r);
+    bu.push_back((int32_t)ucol.size());
+    ++blk;
+  }
+  c->loc_nblk = blk;
+  c->loc_cap = cap;
+  SGT_TRY(alloc_dev((void **)&c->loc_brow, sizeof(int32_t) * brow.size()));
+  SGT_TRY(alloc_dev((void **)&c->loc_bu, sizeof(int32_t) * bu.size()));
+  SGT_TRY(alloc_dev((void **)&c->loc_ucol, sizeof(int32_t) * std::max<size_t>(ucol.size(), 1)));
+  SGT_TRY(alloc_dev((void **)&c->loc_lidx, sizeof(uint16_t) * lidx.size()));
+  SGT_HIP(hipMemcpy(c->loc_brow, brow.data(), sizeof(int32_t) * brow.size(), hipMemcpyHostToDevice));
+  SGT_HIP(hipMemcpy(c->loc_bu, bu.data(), sizeof(int32_t) * bu.size(), hipMemcpyHostToDevice));
+  if (!ucol.empty()) SGT_HIP(hipMemcpy(c->loc_ucol, ucol.data(), sizeof(int32_t) * ucol.size(), hipMemcpyHostToDevice));
+  SGT_HIP(hipMemcpy(c->loc_lidx, lidx.data(), sizeof(uint16_t) * lidx.size(), hipMemcpyHostToDevice));
+  c->loc_reuse = ucol.empty() ? 0.0 : (double)c->nnz / (double)ucol.size();
+  return SGTSNE_OK;
+}
+
 int tune_attractive(sgtsne_ctx *c) {
   // lanes per row, unroll (0 = plain), rows per work-group (0 = grid-stride),
   // points of Y staged in LDS (0 = none)
@@ -609,8 +769,11 @@ int tune_attractive(sgtsne_ctx *c) {
                         {64, 2, 0, 0},     {16, 1, 64, 0},     {32, 1, 64, 0},     {32, 2, 64, 0},
                         {16, 2, 128, 0},   {32, 2, 256, 0},    {64, 1, 256, 0},    {16, 1, 256, 4096},
                         {16, 2, 512, 4096}, {32, 1, 512, 4096}, {16, 1, 512, 8192}, {16, 2, 1024, 8192},
-                        {32, 2, 1024, 8192}};
+                        {32, 2, 1024, 8192}, {16, 0, 0, -1},    {32, 0, 0, -1},    {64, 0, 0, -1}};
   if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
+  // the column-localised candidates run only when their blocks exist (built
+  // when the caller pins that form): on kNN graphs of high-dimensional data
+  // no order gives their blocks enough column reuse to pay for the build
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));
@@ -621,10 +784,13 @@ int tune_attractive(sgtsne_ctx *c) {
   int st = SGTSNE_OK;
   for (const Cand &k : cands) {
     if (k.win > 0 && (size_t)k.win * c->ds * sizeof(float) > 65536) continue;  // window must fit LDS
+    if (k.win < 0 && c->loc_nblk == 0) continue;                                // no localised blocks
     float ms = 0.f, tot = 0.f;
     for (int rep = 0; rep < 3 && st == SGTSNE_OK; rep++) {
       (void)hipEventRecord(e0, c->stream);
-      st = k.win > 0   ? launch_attr_csr_win(c->stream, c->d, k.g, k.u, k.rpb, k.win, c->n, c->rowptr, c->col,
+      st = k.win < 0   ? launch_attr_csr_loc(c->stream, c->d, k.g, c->loc_nblk, c->loc_cap, c->loc_brow, c->loc_bu,
+                                             c->loc_ucol, c->rowptr, c->loc_lidx, c->val, c->Y, c->Fattr)
+           : k.win > 0 ? launch_attr_csr_win(c->stream, c->d, k.g, k.u, k.rpb, k.win, c->n, c->rowptr, c->col,
                                              c->val, c->Y, c->Fattr)
            : k.rpb > 0 ? launch_attr_csr_blk(c->stream, c->d, k.g, k.u, k.rpb, c->n, c->rowptr, c->col, c->val,
                                              c->Y, c->Fattr)
@@ -1054,8 +1220,13 @@ int sgtsne_ctx_set_attractive(sgtsne_ctx *c, int group, int unroll, int rows_per
   SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
   SGT_CHECK(c->have_P && c->p.format == SGTSNE_FORMAT_CSR_SEG, SGTSNE_ERR_STATE,
             "pinning the attractive kernel needs P uploaded in the CSR_SEG format");
-  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 && window >= 0 && (window == 0 || rows_per_block > 0),
+  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 && window >= -1 && (window <= 0 || rows_per_block > 0) &&
+                (window >= 0 || (unroll == 0 && rows_per_block == 0)),
             SGTSNE_ERR_ARG, "bad attractive kernel shape");
+  if (window < 0 && c->loc_nblk == 0) {
+    SGT_TRY(build_localised(c));
+    SGT_CHECK(c->loc_nblk > 0, SGTSNE_ERR_ARG, "a row is wider than the column-localised blocks");
+  }
   const int keep[4] = {c->seg_group, c->seg_unroll, c->seg_rpb, c->seg_win};
   c->seg_group = group;
   c->seg_unroll = unroll;
@@ -1129,6 +1300,16 @@ int sgtsne_run(int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, co
   if (st == SGTSNE_OK) st = sgtsne_ctx_get_Y(c, y_out, params->layout);
   sgtsne_ctx_destroy(c);
   return st;
+}
+
+int sgtsne_locality_order(int n, const int32_t *rowptr, const int32_t *col, int32_t *perm) {
+  SGT_CHECK(n > 0 && rowptr && perm && (rowptr[n] == 0 || col), SGTSNE_ERR_ARG, "bad arguments");
+  for (int i = 0; i < n; i++) SGT_CHECK(rowptr[i] <= rowptr[i + 1], SGTSNE_ERR_ARG, "rowptr decreases at row %d", i);
+  for (int64_t t = 0; t < rowptr[n]; t++) SGT_CHECK(col[t] >= 0 && col[t] < n, SGTSNE_ERR_ARG, "column out of range");
+  std::vector<int32_t> p;
+  bfs_order(n, rowptr, col, p);
+  std::copy(p.begin(), p.end(), perm);
+  return SGTSNE_OK;
 }
 
 int sgtsne_copy_probe(int64_t bytes, int reps, double *ms_per_copy) {

@@ -50,6 +50,10 @@ int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_pe
 // windowed blocked form: win points of Y around each row range staged in LDS
 int launch_attr_csr_win(hipStream_t s, int d, int group, int unroll, int rows_per_block, int win, int n,
                         const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
+// column-localised form: per row block, Y at its distinct columns staged in LDS
+int launch_attr_csr_loc(hipStream_t s, int d, int group, int nblk, int cap, const int32_t *brow, const int32_t *bu,
+                        const int32_t *ucol, const int32_t *rowptr, const uint16_t *lidx, const float *val,
+                        const float *Y, float *F);
 int launch_attr_csr_row(hipStream_t s, int d, int n, const int32_t *rowptr, const int32_t *col,
                         const float *val, const float *Y, float *F);
 int launch_attr_coo(hipStream_t s, int d, int64_t nnz, const int32_t *row, const int32_t *col,

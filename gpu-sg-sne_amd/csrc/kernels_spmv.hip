@@ -269,6 +269,81 @@ __global__ __launch_bounds__(kBlock) void attr_csr_win_kernel(int n, int rows_pe
   }
 }
 
+// Format 4, column-localised form.  Rows are cut into blocks whose distinct
+// columns number at most `cap`; each block keeps its sorted distinct columns
+// (ucol) and every nonzero a 16-bit index into them (lidx), both built once at
+// upload.  A work-group first gathers Y at its block's distinct columns into
+// LDS (each neighbour of the block read once, with coalesced loads where the
+// columns are dense), then every nonzero reads its y_j from LDS.  Each lane
+// sums the same entries in the same order as attr_csr_seg_kernel<D,G>, so the
+// result is bit-identical to it.
+template <int D, int G>
+__global__ __launch_bounds__(kBlock) void attr_csr_loc_kernel(int nblk, const int32_t *__restrict__ brow,
+                                                              const int32_t *__restrict__ bu,
+                                                              const int32_t *__restrict__ ucol,
+                                                              const int32_t *__restrict__ rowptr,
+                                                              const uint16_t *__restrict__ lidx,
+                                                              const float *__restrict__ val,
+                                                              const float *__restrict__ Y,
+                                                              float *__restrict__ F) {
+  using V = typename VecT<D>::T;
+  extern __shared__ float4 lraw[];
+  V *ly = reinterpret_cast<V *>(lraw);
+  const V *__restrict__ Yv = reinterpret_cast<const V *>(Y);
+  const int per_xcd = (gridDim.x + kXcds - 1) / kXcds;
+  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (b >= nblk) return;  // the whole work-group leaves together
+  const int r0 = brow[b], r1 = brow[b + 1];
+  const int u0 = bu[b], nu = bu[b + 1] - u0;
+  {
+    constexpr int SU = 8;
+    for (int i0 = 0; i0 < nu; i0 += kBlock * SU) {
+      int c[SU];
+#pragma unroll
+      for (int k = 0; k < SU; ++k) {
+        const int i = i0 + k * kBlock + (int)threadIdx.x;
+        c[k] = i < nu ? __builtin_nontemporal_load(ucol + u0 + i) : -1;
+      }
+      V tmp[SU];
+#pragma unroll
+      for (int k = 0; k < SU; ++k)
+        if (c[k] >= 0) tmp[k] = Yv[c[k]];
+#pragma unroll
+      for (int k = 0; k < SU; ++k)
+        if (c[k] >= 0) ly[i0 + k * kBlock + threadIdx.x] = tmp[k];
+    }
+  }
+  __syncthreads();
+  const int grp = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kBlock / G;
+  for (int row = r0 + grp; row < r1; row += NG) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    const int e = rowptr[row + 1];
+    for (int t = rowptr[row] + lane; t < e; t += G) {
+      const V v = ly[__builtin_nontemporal_load(lidx + t)];
+      float yj[D];
+      if constexpr (D == 1) {
+        yj[0] = v;
+      } else if constexpr (D == 2) {
+        yj[0] = v.x; yj[1] = v.y;
+      } else {
+        yj[0] = v.x; yj[1] = v.y; yj[2] = v.z;
+      }
+      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
 // Format 0: one thread per row (CU/pq.cuh:16-75).
 template <int D>
 __global__ __launch_bounds__(kBlock) void attr_csr_row_kernel(int n, const int32_t *__restrict__ rowptr,
@@ -490,6 +565,38 @@ int launch_attr_csr_win(hipStream_t s, int d, int group, int unroll, int rows_pe
   per_d(I2{});
   per_d(I3{});
   SGT_CHECK(ok, SGTSNE_ERR_ARG, "no windowed attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_loc(hipStream_t s, int d, int group, int nblk, int cap, const int32_t *brow, const int32_t *bu,
+                        const int32_t *ucol, const int32_t *rowptr, const uint16_t *lidx, const float *val,
+                        const float *Y, float *F) {
+  if (nblk <= 0) return SGTSNE_OK;
+  const int ds = d == 3 ? 4 : d;
+  const size_t lds = (size_t)cap * ds * sizeof(float);
+  SGT_CHECK(cap >= 1 && cap <= 65536 && lds <= 65536, SGTSNE_ERR_ARG, "column block of %d points does not fit LDS",
+            cap);
+  const int blocks = div_up(nblk, kXcds) * kXcds;
+  bool ok = false;
+  auto go = [&](auto dd, auto gg) {
+    constexpr int DD = decltype(dd)::value, GG = decltype(gg)::value;
+    if (d == DD && group == GG) {
+      attr_csr_loc_kernel<DD, GG><<<blocks, kBlock, lds, s>>>(nblk, brow, bu, ucol, rowptr, lidx, val, Y, F);
+      ok = true;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  auto per_d = [&](auto dd) { go(dd, I16{}); go(dd, I32{}); go(dd, I64{}); };
+  per_d(I1{});
+  per_d(I2{});
+  per_d(I3{});
+  SGT_CHECK(ok, SGTSNE_ERR_ARG, "no column-localised attractive kernel for d=%d group=%d", d, group);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }

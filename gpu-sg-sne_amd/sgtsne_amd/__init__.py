@@ -41,7 +41,7 @@ EXPORTED = [
     "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
     "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
-    "sgtsne_batch_set_graph", "sgtsne_copy_probe",
+    "sgtsne_batch_set_graph", "sgtsne_copy_probe", "sgtsne_locality_order",
 ]
 
 
@@ -131,6 +131,7 @@ def lib():
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_batch_set_graph": (C.c_int, [vp, C.c_int]),
         "sgtsne_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "sgtsne_locality_order": (C.c_int, [C.c_int, i32p, i32p, i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -144,6 +145,13 @@ def _check(status):
     if status != 0:
         msg = lib().sgtsne_last_error().decode(errors="replace")
         raise SgtsneError(f"sgtsne error {status}: {msg}")
+
+
+def locality_order(P):
+    """Reverse Cuthill-McKee order of P's graph (host): perm[r] = vertex at row r."""
+    perm = np.empty(P.n, np.int32)
+    _check(lib().sgtsne_locality_order(P.n, P.rowptr, P.col if P.nnz else np.zeros(1, np.int32), perm))
+    return perm
 
 
 def copy_probe(nbytes, reps=10):

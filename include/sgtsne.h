@@ -86,10 +86,11 @@ typedef struct sgtsne_params {
   int ell_width;      /* hybrid ELL width; <= 0 = automatic                   */
   int zeta_mode;      /* SGTSNE_ZETA_*                          (DIRECT)      */
   int reorder;        /* device order of the points (the API always reads and
-                         writes the caller's order): 0 as given; 1 breadth-
-                         first order of P at upload; 2 also re-ordered along
-                         a Morton curve of Y at iterations 50, 150, 300, 600
-                         (CSR formats)                                  (2)   */
+                         writes the caller's order): 0 as given; 1 reverse
+                         Cuthill-McKee order of P at upload; 2 also tries a
+                         Morton order of Y at iterations 50, 150, 300, 600
+                         and keeps it when the attractive kernel measures
+                         faster (CSR formats)                            (2)   */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of
@@ -193,7 +194,8 @@ int sgtsne_ctx_timing(sgtsne_ctx *ctx, double *out7);
 /* Choice of the attractive-kernel autotuner (run at upload for the default
  * format, and again after the first Morton re-order): lanes per row, unroll
  * (0 = plain form), rows per work-group (0 = grid-stride form), points of Y
- * staged in LDS per work-group (0 = none), and the measured ms of each
+ * staged in LDS per work-group (0 = none; -1 = column-localised blocks: Y at
+ * each row block's distinct columns staged in LDS), and the measured ms of each
  * candidate (up to max; *count receives how many were timed). */
 int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, int *rows_per_block, int *window, double *ms,
                       int max, int *count);
@@ -232,6 +234,13 @@ int sgtsne_ctx_destroy(sgtsne_ctx *ctx);
  * batch (bit-identical alone and in any mix).  The reference has no batched
  * form: it runs one sgtsneCUDA call (CU/sgtsne.cu:26-128) per embedding. */
 typedef struct sgtsne_batch sgtsne_batch;
+
+/* The locality order the engine applies at upload (reorder >= 1): reverse
+ * Cuthill-McKee of the graph of P (rowptr [n+1], col [nnz], host),
+ * perm[r] = the vertex placed at row r.  Host only; no device work.  The
+ * reference reorders with METIS nested dissection for its CSB blocks
+ * (HY/sgtsne.cpp:90-112); this replaces that step. */
+int sgtsne_locality_order(int n, const int32_t *rowptr, const int32_t *col, int32_t *perm);
 
 /* Device bandwidth probe: copies `bytes` (a multiple of 16) between two
  * device buffers with a plain streaming kernel `reps` times and returns the

@@ -95,3 +95,42 @@ def test_cpp_mirror_header_compiles_and_links(sg, tmp_path):
                     "-L", libdir, "-lsgtsne_hip", "-o", str(exe)], check=True)
     env = dict(os.environ, LD_LIBRARY_PATH=libdir)
     assert subprocess.run([str(exe)], env=env).returncode == 0
+
+
+def _bandwidth(rp, col, perm):
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    return int(np.abs(inv[rows] - inv[col]).max())
+
+
+def test_locality_order_is_rcm(sg):
+    # host-only entry: a permutation whose bandwidth matches scipy's reverse
+    # Cuthill-McKee on a kNN graph of points in the plane given in random order
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+    rng = np.random.default_rng(0)
+    n, k = 3000, 8
+    pts = rng.uniform(0, 1, size=(n, 2))
+    d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    nbr = np.argsort(d2, axis=1)[:, 1:k + 1]
+    A = csr_matrix((np.ones(n * k), nbr.ravel(), np.arange(0, n * k + 1, k)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    A.sort_indices()
+    rp, col = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    P = sg.CSR(n, rp, col, np.ones(col.size, np.float32))
+    perm = sg.locality_order(P)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    ours = _bandwidth(rp, col, perm)
+    ref = _bandwidth(rp, col, reverse_cuthill_mckee(A, symmetric_mode=True).astype(np.int64))
+    rand = _bandwidth(rp, col, np.arange(n))
+    assert ours <= 1.25 * ref and ours < rand / 5, (ours, ref, rand)
+
+
+def test_locality_order_components_and_isolated(sg):
+    # two components and isolated vertices are all placed exactly once
+    rp = np.array([0, 1, 2, 2, 3, 4, 4], np.int32)
+    col = np.array([1, 0, 4, 3], np.int32)
+    perm = sg.locality_order(sg.CSR(6, rp, col, np.ones(4, np.float32)))
+    assert np.array_equal(np.sort(perm), np.arange(6))

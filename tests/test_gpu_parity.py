@@ -393,8 +393,9 @@ def test_reorder_modes_agree(sg, O):
 
 # every shape of the default CSR kernel: same result as the oracle, and the
 # same bits for every shape with the same lanes per row
-_SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0, 0), (16, 1, 64, 0), (16, 2, 128, 0),
-           (16, 1, 256, 4096), (16, 2, 512, 4096), (16, 2, 1024, 8192), (32, 2, 256, 0), (32, 1, 512, 4096)]
+_SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (64, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0, 0), (16, 1, 64, 0),
+           (16, 2, 128, 0), (16, 1, 256, 4096), (16, 2, 512, 4096), (16, 2, 1024, 8192), (32, 2, 256, 0),
+           (32, 1, 512, 4096), (16, 0, 0, -1), (32, 0, 0, -1), (64, 0, 0, -1)]
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
