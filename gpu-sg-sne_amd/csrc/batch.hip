@@ -178,8 +178,8 @@ int attractive(sgtsne_batch *b) {
 int enqueue_terms(sgtsne_batch *b) {
   const int zd = b->p.zeta_mode == SGTSNE_ZETA_DIRECT;
   SGT_TRY(attractive(b));
-  SGT_TRY(launch_bs2g(b->stream, b->d, b->jobs_d, b->items_d, b->nitems, b->Y, std::min(b->nitems, b->s2g_blocks),
-                      b->s2g_lds));
+  SGT_TRY(launch_bs2g(b->stream, b->d, b->jobs_d, b->items_d, b->nitems, b->Y,
+                      std::min(b->nitems * (b->d == 3 ? 4 : 1), b->s2g_blocks), b->d == 3 ? 64512 : b->s2g_lds));
   SGT_TRY(launch_bconv(b->stream, b->d, zd, b->args_d, b->prefix_d, b->max_jobs, b->conv_blocks));
   SGT_TRY(launch_bg2s(b->stream, b->d, zd, b->jobs_d, b->items_d, b->nitems, b->Y, b->Frep, b->zpart,
                       std::min(b->nitems, b->pt_blocks)));
@@ -524,8 +524,13 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
 void destroy_all(sgtsne_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   invalidate_graph(b);
+  // per-job grids come from the stream-ordered pool: release them the same way
   for (auto &h : b->hj)
-    if (h.gmem) (void)hipFree(h.gmem);
+    if (h.gmem) {
+      (void)hipFreeAsync(h.gmem, b->stream);
+      h.gmem = nullptr;
+    }
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   bfree(b->rowptr);
   bfree(b->col);
   bfree(b->val);
@@ -559,6 +564,8 @@ int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int lay
                          b->stream));
   SGT_HIP(hipMemcpyAsync(pm.data(), b->perm + h.off, sizeof(int32_t) * n, hipMemcpyDeviceToHost, b->stream));
   SGT_HIP(hipStreamSynchronize(b->stream));
+  for (int r = 0; r < n; ++r)
+    SGT_CHECK(pm[r] >= 0 && pm[r] < n, SGTSNE_ERR_STATE, "job %d: point order corrupt at row %d", job, r);
   for (int r = 0; r < n; ++r) {
     const int i = pm[r];
     for (int a = 0; a < d; ++a) {
@@ -628,7 +635,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   if (!st) st = balloc((void **)&b->mpart, sizeof(double) * 8 * b->items_cap);
   if (!st) st = balloc(&b->args_d, batch_pass_args_bytes() * np * max_jobs);
   if (!st) st = balloc((void **)&b->prefix_d, sizeof(int) * np * (max_jobs + 1));
-  if (!st) st = balloc((void **)&b->tw, 512 * sizeof(float2));
+  if (!st) st = balloc((void **)&b->tw, twiddle_count() * sizeof(float2));
   if (!st) st = balloc((void **)&b->scal_d, 8 * sizeof(double));
   if (!st) chk(hipMemsetAsync(b->rowptr, 0, sizeof(int32_t) * (P + 1), b->stream), "memset");
   if (!st) chk(hipMemsetAsync(b->col, 0, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1), b->stream), "memset");

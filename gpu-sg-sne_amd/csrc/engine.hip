@@ -652,8 +652,20 @@ void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t
 int apply_locality_order(sgtsne_ctx *c) {
   const int n = c->n;
   std::vector<int32_t> rp((size_t)n + 1), col((size_t)std::max<int64_t>(c->nnz, 1));
-  SGT_HIP(hipMemcpy(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost));
-  if (c->nnz > 0) SGT_HIP(hipMemcpy(col.data(), c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+  // P was written on the context's (non-blocking) stream, which a plain
+  // hipMemcpy on the null stream does not wait for: read it back in order
+  SGT_HIP(hipMemcpyAsync(rp.data(), c->rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, c->stream));
+  if (c->nnz > 0)
+    SGT_HIP(hipMemcpyAsync(col.data(), c->col, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost, c->stream));
+  SGT_HIP(hipStreamSynchronize(c->stream));
+  // the sweep indexes host arrays with these: check them first
+  SGT_CHECK(rp[0] == 0 && rp[n] == c->nnz, SGTSNE_ERR_STATE, "device P inconsistent (rowptr ends %d, %d)", rp[0],
+            rp[n]);
+  for (int i = 0; i < n; i++)
+    SGT_CHECK(rp[i] <= rp[i + 1], SGTSNE_ERR_STATE, "device P inconsistent: rowptr decreases at row %d", i);
+  for (int64_t t = 0; t < c->nnz; t++)
+    SGT_CHECK(col[t] >= 0 && col[t] < n, SGTSNE_ERR_STATE, "device P inconsistent: column %d at %lld", col[t],
+              (long long)t);
   bfs_order(n, rp.data(), col.data(), c->perm);
   c->iperm.assign(n, 0);
   for (int r = 0; r < n; r++) c->iperm[c->perm[r]] = r;
@@ -933,7 +945,7 @@ int sgtsne_ctx_create(sgtsne_ctx **out, const sgtsne_params *params) {
         set_error("hipEventCreate failed");
         return fail(SGTSNE_ERR_HIP);
       }
-  if ((st = alloc_dev((void **)&c->cb.tw, sizeof(float2) * 512))) return fail(st);
+  if ((st = alloc_dev((void **)&c->cb.tw, sizeof(float2) * twiddle_count()))) return fail(st);
   if ((st = init_twiddles(c->stream, c->cb.tw))) return fail(st);
   *out = c;
   return SGTSNE_OK;
@@ -1388,10 +1400,8 @@ int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float
   g.d = d;
   g.ng = ng;
   g.nGrid = ng - 2;
-  int L = 1, lg = 0;
-  while (L < 2 * ng - 1) { L <<= 1; ++lg; }
+  const int L = fft_length_for(ng);
   g.L = L;
-  g.logL = lg;
   g.hgrid = h;
   const size_t G = d == 1 ? (size_t)ng : d == 2 ? (size_t)ng * ng : (size_t)ng * ng * ng;
   size_t wb = 0, kb = 0;
@@ -1406,7 +1416,7 @@ int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float
   if (!st) st = alloc_dev((void **)&cb.Kt1, kb);
   if (!st) st = alloc_dev((void **)&cb.Kt2, kb);
   if (!st) st = alloc_dev((void **)&cb.Kh, kb);
-  if (!st) st = alloc_dev((void **)&cb.tw, 512 * sizeof(float2));
+  if (!st) st = alloc_dev((void **)&cb.tw, twiddle_count() * sizeof(float2));
   if (!st) st = init_twiddles(s, cb.tw);
   if (!st && hipMemcpyAsync(Vd, V, G * m * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
     set_error("copy of V failed");

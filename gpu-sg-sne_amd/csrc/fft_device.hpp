@@ -1,4 +1,4 @@
-// fft_device.hpp -- the LDS radix-2 FFT passes of the non-periodic grid
+// fft_device.hpp -- the LDS FFT passes of the non-periodic grid
 // convolution, shared by the single-job launcher (kernels_fft.hip) and the
 // batched engine (kernels_batch.hip).
 //
@@ -7,9 +7,12 @@
 // (HY/non_periodic_conv.cpp:26-708, CU/non_periodic_convF.cu:162-517).  For
 // an FFT length L >= 2 ng - 1 a single circulant of size L with the symmetric
 // embedding K_circ[i] = K(min(i, L - i)) gives exactly the same product on the
-// ng outputs, so here L is the next power of two and every line transform is a
-// radix-2 Stockham FFT held in LDS.  K_circ is even in every axis, so its
-// spectrum is real and even: only its (L/2+1)^d half-spectrum is kept.  Two
+// ng outputs.  Here L is the smallest even 2^a 3^b 5^c >= 2 ng - 1 and every
+// line transform is a Stockham autosort FFT held in LDS, in radix-4/2/3/5
+// stages (the formulation of Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC 2008).  K_circ is even in
+// every axis, so its spectrum is real and even: only its (L/2+1)^d
+// half-spectrum is kept.  Two
 // real grid channels share one complex line (their results are the real and
 // imaginary parts of the inverse transform).  The two kernels K2 = (1+h^2r^2)^-2
 // and K1 = (1+h^2r^2)^-1 are transformed together as K2 + i K1 (both spectra
@@ -30,9 +33,45 @@ namespace fft {
 
 constexpr int kFftThreads = 256;
 constexpr int kLdsElems = 4096;  // complex elements per ping-pong buffer (32 KiB)
-constexpr int kTwLog = 10;       // twiddle table built for length 1024
+constexpr int kMaxL = 1024;      // longest line transform
 constexpr int kMaxPasses = 8;
+constexpr int kMaxStages = 10;
 constexpr size_t kLdsBytes = 2 * (size_t)kLdsElems * sizeof(float2);
+
+// Even lengths whose only prime factors are 2, 3 and 5.
+__host__ __device__ inline bool fft_smooth(int L) {
+  if (L < 2 || (L & 1)) return false;
+  while (L % 2 == 0) L /= 2;
+  while (L % 3 == 0) L /= 3;
+  while (L % 5 == 0) L /= 5;
+  return L == 1;
+}
+
+// Smallest supported FFT length >= n.
+__host__ __device__ inline int fft_length(int n) {
+  int L = n < 2 ? 2 : n + (n & 1);
+  while (!fft_smooth(L)) L += 2;
+  return L;
+}
+
+// Twiddle tables, one per supported length in increasing order, each holding
+// exp(-2 pi i t / L) for t < L; the table of L starts at fft_table_offset(L).
+__host__ __device__ inline int fft_table_offset(int L) {
+  int off = 0;
+  for (int m = 2; m < L; m += 2)
+    if (fft_smooth(m)) off += m;
+  return off;
+}
+__host__ __device__ inline int fft_table_size() { return fft_table_offset(kMaxL + 2); }
+
+// q = x / d for 0 <= x < 2^22, through a float reciprocal and one correction.
+__device__ __forceinline__ int fdiv(int x, int d, float inv) {
+  int q = (int)((float)x * inv);
+  const int r = x - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
 
 // IN_VFIX: the spread grid in 64-bit fixed point (batched engine); the pass
 // converts it and clears every element it reads, so the grid is zero again
@@ -54,7 +93,8 @@ enum PassKind {
 };
 
 struct PassArgs {
-  int L, logL, lpw_log;  // FFT length, lines per work-group = 1 << lpw_log
+  int L, nl;             // FFT length, lines per work-group
+  int nst, rad[kMaxStages];  // Stockham stages and their radices
   int nlines, ninner;    // line = outer * ninner + inner
   int64_t i_os, i_is, i_es, pair_in;
   int n_in;
@@ -70,7 +110,7 @@ struct PassArgs {
   const void *in;
   void *out;
   const float2 *Kh;
-  const float2 *tw;
+  const float2 *tw;  // twiddle table of length L
 };
 
 struct ConvPtrs {
@@ -82,32 +122,102 @@ struct ConvPtrs {
 
 __host__ __device__ inline int mirror(int e, int L) { return e <= (L >> 1) ? e : L - e; }
 
-// Radix-2 Stockham autosort on nl lines of length L (natural order in and
-// out).  Returns the buffer holding the result.
-__device__ inline float2 *stockham(float2 *x, float2 *y, int L, int logL, int nl, bool inv,
-                                   const float2 *__restrict__ tw) {
-  const int half = L >> 1;
-  const int total = nl * half;
-  for (int s = 0; s < logL; ++s) {
-    const int Ns = 1 << s;
-    for (int b = threadIdx.x; b < total; b += blockDim.x) {
-      const int l = b >> (logL - 1);
-      const int j = b & (half - 1);
-      const float2 *src = x + l * L;
-      float2 *dst = y + l * L;
-      float2 a0 = src[j], a1 = src[j + half];
-      const int k = j & (Ns - 1);
-      float2 w = tw[k << (kTwLog - 1 - s)];
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// i * sg * z
+__device__ __forceinline__ float2 mul_isg(float2 z, float sg) { return make_float2(-sg * z.y, sg * z.x); }
+
+// R-point DFT in place; sg = -1 forward, +1 inverse (unnormalised).
+template <int R>
+__device__ __forceinline__ void butterfly(float2 (&v)[R], float sg) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const float2 t0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 t1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 t2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 t3 = mul_isg(make_float2(v[1].x - v[3].x, v[1].y - v[3].y), sg);
+    v[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+    v[1] = make_float2(t1.x + t3.x, t1.y + t3.y);
+    v[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+    v[3] = make_float2(t1.x - t3.x, t1.y - t3.y);
+  } else if constexpr (R == 3) {
+    const float c = -0.5f, s3 = 0.86602540378443864676f;  // cos, sin of 2 pi / 3
+    const float2 a = make_float2(v[1].x + v[2].x, v[1].y + v[2].y);
+    const float2 d = mul_isg(make_float2(v[1].x - v[2].x, v[1].y - v[2].y), sg * s3);
+    const float2 m = make_float2(v[0].x + c * a.x, v[0].y + c * a.y);
+    v[0] = make_float2(v[0].x + a.x, v[0].y + a.y);
+    v[1] = make_float2(m.x + d.x, m.y + d.y);
+    v[2] = make_float2(m.x - d.x, m.y - d.y);
+  } else {  // R == 5
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;  // cos 2pi/5, cos 4pi/5
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;   // sin 2pi/5, sin 4pi/5
+    const float2 a1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y);
+    const float2 b1 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
+    const float2 a2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y);
+    const float2 b2 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+    const float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    const float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    const float2 n1 = mul_isg(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), sg);
+    const float2 n2 = mul_isg(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), sg);
+    v[0] = make_float2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+    v[1] = make_float2(m1.x + n1.x, m1.y + n1.y);
+    v[4] = make_float2(m1.x - n1.x, m1.y - n1.y);
+    v[2] = make_float2(m2.x + n2.x, m2.y + n2.y);
+    v[3] = make_float2(m2.x - n2.x, m2.y - n2.y);
+  }
+}
+
+// One radix-R Stockham stage over nl lines of length L; Ns is the length of
+// the sub-transforms completed by the previous stages.
+template <int R>
+__device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L, int nl, int Ns,
+                                               const float2 *__restrict__ tw, bool inv) {
+  const int LR = L / R;
+  const int stride = L / (Ns * R);
+  const float invLR = 1.0f / (float)LR, invNs = 1.0f / (float)Ns;
+  const float sg = inv ? 1.0f : -1.0f;
+  for (int b = threadIdx.x; b < nl * LR; b += blockDim.x) {
+    const int l = fdiv(b, LR, invLR);
+    const int j = b - l * LR;
+    const int q = fdiv(j, Ns, invNs);
+    const int k = j - q * Ns;
+    const float2 *src = x + l * L;
+    float2 *dst = y + l * L;
+    float2 v[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[t] = src[j + t * LR];
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+      float2 w = tw[t * k * stride];
       if (inv) w.y = -w.y;
-      float2 t = make_float2(a1.x * w.x - a1.y * w.y, a1.x * w.y + a1.y * w.x);
-      const int o = ((j - k) << 1) + k;
-      dst[o] = make_float2(a0.x + t.x, a0.y + t.y);
-      dst[o + Ns] = make_float2(a0.x - t.x, a0.y - t.y);
+      v[t] = cmul(v[t], w);
     }
+    butterfly<R>(v, sg);
+    const int base = q * Ns * R + k;
+#pragma unroll
+    for (int t = 0; t < R; ++t) dst[base + t * Ns] = v[t];
+  }
+}
+
+// Stockham autosort on nl lines (natural order in and out).  Returns the
+// buffer holding the result.
+__device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int nl, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < a.nst; ++s) {
+    const int R = a.rad[s];
+    if (R == 4) stockham_stage<4>(x, y, a.L, nl, Ns, a.tw, inv);
+    else if (R == 2) stockham_stage<2>(x, y, a.L, nl, Ns, a.tw, inv);
+    else if (R == 3) stockham_stage<3>(x, y, a.L, nl, Ns, a.tw, inv);
+    else stockham_stage<5>(x, y, a.L, nl, Ns, a.tw, inv);
     __syncthreads();
     float2 *tmp = x;
     x = y;
     y = tmp;
+    Ns *= R;
   }
   return x;
 }
@@ -116,8 +226,9 @@ __device__ inline float2 *stockham(float2 *x, float2 *y, int L, int logL, int nl
 // channel pair p.  Every thread of the work-group must call it (barriers).
 template <int IN, int OUT, int MID, bool ECONTIG>
 __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds) {
-  const int L = a.L, logL = a.logL;
-  const int nl = 1 << a.lpw_log;
+  const int L = a.L;
+  const int nl = a.nl;
+  const float invL = 1.0f / (float)L, invnl = 1.0f / (float)nl;
   float2 *b0 = lds;
   float2 *b1 = lds + nl * L;
   const int line0 = blk * nl;
@@ -126,8 +237,8 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
   // ---- load (zero padding beyond n_in) ----
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     int l, e;
-    if (ECONTIG) { l = idx >> logL; e = idx & (L - 1); }
-    else { l = idx & (nl - 1); e = idx >> a.lpw_log; }
+    if (ECONTIG) { l = fdiv(idx, L, invL); e = idx - l * L; }
+    else { e = fdiv(idx, nl, invnl); l = idx - e * nl; }
     const int line = line0 + l;
     float2 v = make_float2(0.f, 0.f);
     if (line < a.nlines && e < a.n_in) {
@@ -168,16 +279,11 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
   }
   __syncthreads();
 
-  float2 *r;
-  if (MID == MID_INV) {
-    r = stockham(b0, b1, L, logL, nl, true, a.tw);
-  } else {
-    r = stockham(b0, b1, L, logL, nl, false, a.tw);
-  }
+  float2 *r = stockham(b0, b1, a, nl, MID == MID_INV);
   if (MID == MID_CONV) {
     // multiply by the real kernel spectrum, then inverse transform
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int l = idx >> logL, e = idx & (L - 1);
+      const int l = fdiv(idx, L, invL), e = idx - l * L;
       const int line = line0 + l;
       float kv = 0.f, kvi = 0.f;
       if (line < a.nlines) {
@@ -197,14 +303,14 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
     }
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
-    r = stockham(r, other, L, logL, nl, true, a.tw);
+    r = stockham(r, other, a, nl, true);
   }
 
   // ---- store ----
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     int l, e;
-    if (ECONTIG) { l = idx >> logL; e = idx & (L - 1); }
-    else { l = idx & (nl - 1); e = idx >> a.lpw_log; }
+    if (ECONTIG) { l = fdiv(idx, L, invL); e = idx - l * L; }
+    else { e = fdiv(idx, nl, invnl); l = idx - e * nl; }
     const int line = line0 + l;
     if (line >= a.nlines || e >= a.n_out) continue;
     const int outer = line / a.ninner;
@@ -227,10 +333,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
 }
 
 // Work-groups of one pass (per channel pair).
-__host__ __device__ inline int pass_blocks(const PassArgs &a) {
-  const int nl = 1 << a.lpw_log;
-  return (a.nlines + nl - 1) / nl;
-}
+__host__ __device__ inline int pass_blocks(const PassArgs &a) { return (a.nlines + a.nl - 1) / a.nl; }
 
 // Pass shapes of a d-dimensional convolution, in launch order.
 __host__ __device__ inline int pass_kinds(int d, int *kinds) {
@@ -284,16 +387,20 @@ __host__ __device__ inline Plan make_plan(int d, bool zdirect) {
 __host__ __device__ inline PassArgs base_args(const GridParams &g, const Plan &pl, const ConvPtrs &b) {
   PassArgs a{};
   a.L = g.L;
-  a.logL = g.logL;
-  int lpw = kLdsElems >> g.logL;
-  if (lpw < 1) lpw = 1;
-  if (lpw > 64) lpw = 64;
-  int lg = 0;
-  while ((1 << (lg + 1)) <= lpw) ++lg;
-  a.lpw_log = lg;
+  int nl = kLdsElems / g.L;
+  if (nl < 1) nl = 1;
+  if (nl > 64) nl = 64;
+  a.nl = nl;
+  // radix-4 stages first, then 2, 3, 5
+  int m = g.L, ns = 0;
+  while (m % 4 == 0 && ns < kMaxStages) { a.rad[ns++] = 4; m /= 4; }
+  while (m % 2 == 0 && ns < kMaxStages) { a.rad[ns++] = 2; m /= 2; }
+  while (m % 3 == 0 && ns < kMaxStages) { a.rad[ns++] = 3; m /= 3; }
+  while (m % 5 == 0 && ns < kMaxStages) { a.rad[ns++] = 5; m /= 5; }
+  a.nst = ns;
   a.H = g.L / 2 + 1;
   a.hsq = (float)(g.hgrid * g.hgrid);
-  a.tw = b.tw;
+  a.tw = b.tw + fft_table_offset(g.L);
   a.Kh = b.Kh;
   a.scale = 1.0f;
   a.fix_inv = 1.0;

@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "fft_device.hpp"
 
 namespace sgt {
 
@@ -35,13 +36,7 @@ __host__ __device__ inline void grid_from_box(int d, int nGrid, const double *mi
   g.d = d;
   g.nGrid = nGrid;
   g.ng = nGrid + 2;
-  int L = 1, lg = 0;
-  while (L < 2 * g.ng - 1) {
-    L <<= 1;
-    ++lg;
-  }
-  g.L = L;
-  g.logL = lg;
+  g.L = fft::fft_length(2 * g.ng - 1);
   for (int a = 0; a < 3; a++) {
     g.minv[a] = a < d ? (float)minv[a] : 0.0f;
     g.center[a] = a < d ? (float)(0.5 * (minv[a] + maxv[a])) : 0.0f;

@@ -31,8 +31,7 @@ struct GridParams {
   int d;
   int nGrid;        // interpolation intervals + 1 (reference nGridDim)
   int ng;           // grid side = nGrid + 2
-  int L;            // FFT length (power of two, >= 2 ng - 1)
-  int logL;
+  int L;            // FFT length (even 2^a 3^b 5^c >= 2 ng - 1)
   float minv[3];    // min of Y per dimension (grid origin)
   float scale;      // (nGrid - 1) / maxy
   float gmax;       // largest float below nGrid - 1 (clamp)
@@ -90,13 +89,17 @@ struct ConvBuffers {
   float2 *Kt1;    // (K2, K1) half-spectrum scratch
   float2 *Kt2;
   float2 *Kh;     // (K2^, K1^) half-spectra of the kernels [(L/2+1)^d]
-  float2 *tw;     // twiddles exp(-2 pi i t / 1024), t < 512
+  float2 *tw;     // twiddle tables of every supported length (twiddle_count())
 };
 int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes);
 // V [d+1][ng^d] -> Phi [d+1 (+1 with zdirect: K1 * V0 in channel d+1)][ng^d]
 int launch_conv(hipStream_t s, const GridParams &g, int zdirect, const float *V, float *Phi,
                 const ConvBuffers &b);
+// twiddle tables of every supported FFT length (twiddle_count() float2)
+size_t twiddle_count();
 int init_twiddles(hipStream_t s, float2 *tw);
+// FFT length of a grid of side ng: smallest even 2^a 3^b 5^c >= 2 ng - 1
+int fft_length_for(int ng);
 
 // ---- P construction (kernels_prep.hip) ----
 int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D);

@@ -51,10 +51,20 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int node = lane % NODES, slot = lane / NODES;
   const int a0 = node & 3, a1 = (node >> 2) & 3, a2 = node >> 4;
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const BatchItem itm = items[it];
+  // work unit = one sub-chunk of an item: 3-D stencil windows grow fast with
+  // the point count, so 3-D items are spread in quarters, each window its own
+  constexpr int SUB = D == 3 ? 4 : 1;
+  for (int wu = blockIdx.x; wu < nitems * SUB; wu += gridDim.x) {
+    const int it = wu / SUB, part = wu - it * SUB;
+    BatchItem itm = items[it];
     const BatchJob *J = jobs + itm.job;
     if (J->status != BJ_RUN) continue;  // the same value for every thread of the block
+    if (SUB > 1) {
+      const int len = itm.end - itm.begin, step = (len + SUB - 1) / SUB;
+      itm.begin += part * step;
+      itm.end = min(itm.end, itm.begin + step);
+      if (itm.begin >= itm.end) continue;
+    }
     const GridParams g = J->g;
     const double fs = J->fix_scale;
     unsigned long long *V = reinterpret_cast<unsigned long long *>(J->vfix);
@@ -614,8 +624,7 @@ int bpass(hipStream_t s, const PassArgs *args, const int *prefix, int njobs, int
 }  // namespace
 
 size_t batch_grid_bytes(int d, int ng_cap, size_t *vfix_b, size_t *phi_b, size_t *w_b, size_t *k_b) {
-  int L = 1;
-  while (L < 2 * ng_cap - 1) L <<= 1;
+  const int L = fft_length(2 * ng_cap - 1);  // FFT lengths grow with ng: the largest one
   const size_t G = d == 1 ? (size_t)ng_cap : d == 2 ? (size_t)ng_cap * ng_cap : (size_t)ng_cap * ng_cap * ng_cap;
   *vfix_b = G * (size_t)(d + 1) * sizeof(long long);
   *phi_b = G * (size_t)(d + 2) * sizeof(float);

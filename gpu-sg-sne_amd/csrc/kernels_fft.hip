@@ -21,19 +21,28 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(PassArgs a) {
   pass_body<IN, OUT, MID, EC>(a, blockIdx.x, blockIdx.y, lds);
 }
 
-__global__ void twiddle_kernel(float2 *tw) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (1 << (kTwLog - 1))) {
-    double ang = -2.0 * M_PI * (double)t / (double)(1 << kTwLog);
-    tw[t] = make_float2((float)cos(ang), (float)sin(ang));
+// Fills every twiddle table: entry t of the table of length L is
+// exp(-2 pi i t / L), evaluated in FP64 and rounded once.
+__global__ void twiddle_kernel(float2 *tw, int total) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int off = 0, L = 2;
+  for (; L <= kMaxL; L += 2) {
+    if (!fft_smooth(L)) continue;
+    if (g < off + L) break;
+    off += L;
   }
+  const int t = g - off;
+  double sn, cs;
+  sincospi(-2.0 * (double)t / (double)L, &sn, &cs);
+  tw[g] = make_float2((float)cs, (float)sn);
 }
 
 template <int IN, int OUT, int MID, bool EC>
 int run_pass(hipStream_t s, const PassArgs &a, int pairs) {
   const int blocks = pass_blocks(a);
   if (blocks == 0) return SGTSNE_OK;
-  const size_t lds = 2 * (size_t)(1 << a.lpw_log) * a.L * sizeof(float2);
+  const size_t lds = 2 * (size_t)a.nl * a.L * sizeof(float2);
   dim3 grid(blocks, pairs);
   hipLaunchKernelGGL((fft_pass_kernel<IN, OUT, MID, EC>), grid, dim3(kFftThreads), lds, s, a);
   SGT_LAUNCH_CHECK();
@@ -56,8 +65,13 @@ int run_kind(hipStream_t s, int kind, const PassArgs &a, int pairs) {
 
 }  // namespace
 
+size_t twiddle_count() { return (size_t)fft_table_size(); }
+
+int fft_length_for(int ng) { return fft_length(2 * ng - 1); }
+
 int init_twiddles(hipStream_t s, float2 *tw) {
-  twiddle_kernel<<<2, 256, 0, s>>>(tw);
+  const int total = fft_table_size();
+  twiddle_kernel<<<div_up(total, 256), 256, 0, s>>>(tw, total);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
@@ -74,8 +88,8 @@ int conv_workspace_bytes(int d, int ng, int L, size_t *w_bytes, size_t *k_bytes)
 
 int launch_conv(hipStream_t s, const GridParams &g, int zdirect, const float *V, float *Phi,
                 const ConvBuffers &b) {
-  SGT_CHECK(g.L <= (1 << kTwLog) && g.L >= 2 * g.ng - 1, SGTSNE_ERR_ARG, "FFT length %d invalid for ng %d", g.L,
-            g.ng);
+  SGT_CHECK(g.L <= kMaxL && g.L >= 2 * g.ng - 1 && fft_smooth(g.L), SGTSNE_ERR_ARG,
+            "FFT length %d invalid for ng %d", g.L, g.ng);
   ConvPtrs ptr{V, Phi, b.W, b.Kt1, b.Kt2, b.Kh, b.tw};
   PassArgs args[kMaxPasses];
   int pairs[kMaxPasses], kinds[kMaxPasses];
