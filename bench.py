@@ -145,7 +145,8 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
            "plain_embeds_per_s": round(jobs / t_plain_all, 3),
            "speedup_vs_plain": round(t_plain_all / (t_front + t_iter), 2),
            "plain_sample": f"{sample} jobs stratified by N, extrapolated to {jobs}",
-           "kl_batched_over_plain": {"mean": round(float(np.mean(ratios)), 5), "max": round(float(np.max(ratios)), 5)},
+           "kl_batched_over_plain": {"min": round(float(np.min(ratios)), 5), "mean": round(float(np.mean(ratios)), 5),
+                                     "max": round(float(np.max(ratios)), 5)},
            "graph_replays": info["graph_replays"], "attractive_tuning": tune}
     log(f"[bench] batched: {out}")
     return out

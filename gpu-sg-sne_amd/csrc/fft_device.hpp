@@ -384,8 +384,10 @@ __host__ __device__ inline Plan make_plan(int d, bool zdirect) {
   return pl;
 }
 
-__host__ __device__ inline PassArgs base_args(const GridParams &g, const Plan &pl, const ConvPtrs &b) {
-  PassArgs a{};
+// The fields every pass shares, written in place (a may live in global
+// memory: no private copy of the argument block is made).
+__host__ __device__ inline void base_args(PassArgs &a, const GridParams &g, const Plan &pl, const ConvPtrs &b) {
+  a = PassArgs{};
   a.L = g.L;
   int nl = kLdsElems / g.L;
   if (nl < 1) nl = 1;
@@ -413,106 +415,117 @@ __host__ __device__ inline PassArgs base_args(const GridParams &g, const Plan &p
     a.out_c[p][1] = pl.out_c[p][1];
     a.kcplx[p] = pl.kcplx[p];
   }
-  return a;
+}
+
+// Passes of a d-dimensional convolution.
+__host__ __device__ inline int conv_npasses(int d) { return d == 1 ? 2 : d == 2 ? 5 : 8; }
+
+// Arguments of pass i (in the order of pass_kinds) of one convolution,
+// written in place; returns the channel pairs the pass runs over.
+__host__ __device__ inline int conv_pass(const GridParams &g, bool zdirect, const ConvPtrs &b, int i, PassArgs &a) {
+  const int d = g.d, ng = g.ng, L = g.L, H = L / 2 + 1;
+  const Plan pl = make_plan(d, zdirect);
+  const int64_t G = d == 1 ? ng : d == 2 ? (int64_t)ng * ng : (int64_t)ng * ng * ng;
+  const int kpow = zdirect ? 0 : 2;
+  base_args(a, g, pl, b);
+
+  if (d == 1) {
+    if (i == 0) {  // kernel spectra
+      a.nlines = 1; a.ninner = 1; a.n_in = L; a.n_out = H; a.o_es = 1; a.kpow = kpow;
+      a.out = b.Kh;
+      return 1;
+    }
+    a.nlines = 1; a.ninner = 1;
+    a.in = b.V; a.i_es = 1; a.n_in = ng; a.pair_in = G;
+    a.out = b.Phi; a.o_es = 1; a.n_out = ng; a.pair_out = G;
+    a.kdim = 1; a.scale = 1.0f / (float)L;
+    return pl.pairs;
+  }
+
+  if (d == 2) {
+    const int64_t wp = (int64_t)ng * L;
+    switch (i) {
+      case 0:
+        a.nlines = H; a.ninner = H; a.n_in = L; a.o_is = H; a.o_es = 1; a.n_out = H; a.kpow = kpow;
+        a.out = b.Kt1;
+        return 1;
+      case 1:
+        a.nlines = H; a.ninner = H; a.in = b.Kt1; a.i_is = 1; a.i_es = H; a.n_in = L;
+        a.out = b.Kh; a.o_is = 1; a.o_es = H; a.n_out = H;
+        return 1;
+      case 2:  // rows: V -> W
+        a.nlines = ng; a.ninner = ng;
+        a.in = b.V; a.i_is = ng; a.i_es = 1; a.n_in = ng; a.pair_in = G;
+        a.out = b.W; a.o_is = L; a.o_es = 1; a.n_out = L; a.pair_out = wp;
+        return pl.pairs;
+      case 3:  // columns: fwd, x spectrum, inv
+        a.nlines = L; a.ninner = L;
+        a.in = b.W; a.i_is = 1; a.i_es = L; a.n_in = ng; a.pair_in = wp;
+        a.out = b.W; a.o_is = 1; a.o_es = L; a.n_out = ng; a.pair_out = wp;
+        a.kdim = 2;
+        return pl.pairs;
+      default:  // rows: inverse -> Phi
+        a.nlines = ng; a.ninner = ng;
+        a.in = b.W; a.i_is = L; a.i_es = 1; a.n_in = L; a.pair_in = wp;
+        a.out = b.Phi; a.o_is = ng; a.o_es = 1; a.n_out = ng; a.pair_out = G;
+        a.scale = 1.0f / ((float)L * (float)L);
+        return pl.pairs;
+    }
+  }
+
+  // d == 3
+  const int64_t H2 = (int64_t)H * H, L2 = (int64_t)L * L;
+  const int64_t wp = (int64_t)ng * L2;
+  switch (i) {
+    case 0:
+      a.nlines = (int)H2; a.ninner = H; a.n_in = L; a.kpow = kpow;
+      a.out = b.Kt1; a.o_os = H2; a.o_is = H; a.o_es = 1; a.n_out = H;
+      return 1;
+    case 1:
+      a.nlines = (int)H2; a.ninner = H; a.in = b.Kt1; a.i_os = H2; a.i_is = 1; a.i_es = H; a.n_in = L;
+      a.out = b.Kt2; a.o_os = H2; a.o_is = 1; a.o_es = H; a.n_out = H;
+      return 1;
+    case 2:
+      a.nlines = (int)H2; a.ninner = H; a.in = b.Kt2; a.i_os = H; a.i_is = 1; a.i_es = H2; a.n_in = L;
+      a.out = b.Kh; a.o_os = H; a.o_is = 1; a.o_es = H2; a.n_out = H;
+      return 1;
+    case 3:  // x: V -> W
+      a.nlines = ng * ng; a.ninner = ng;
+      a.in = b.V; a.i_os = (int64_t)ng * ng; a.i_is = ng; a.i_es = 1; a.n_in = ng; a.pair_in = G;
+      a.out = b.W; a.o_os = L2; a.o_is = L; a.o_es = 1; a.n_out = L; a.pair_out = wp;
+      return pl.pairs;
+    case 4:  // y forward (all ky kept)
+      a.nlines = ng * L; a.ninner = L;
+      a.in = b.W; a.i_os = L2; a.i_is = 1; a.i_es = L; a.n_in = ng; a.pair_in = wp;
+      a.out = b.W; a.o_os = L2; a.o_is = 1; a.o_es = L; a.n_out = L; a.pair_out = wp;
+      return pl.pairs;
+    case 5:  // z: fwd, x spectrum, inv (z < ng kept)
+      a.nlines = (int)L2; a.ninner = L;
+      a.in = b.W; a.i_os = L; a.i_is = 1; a.i_es = L2; a.n_in = ng; a.pair_in = wp;
+      a.out = b.W; a.o_os = L; a.o_is = 1; a.o_es = L2; a.n_out = ng; a.pair_out = wp;
+      a.kdim = 3;
+      return pl.pairs;
+    case 6:  // y inverse (y < ng kept)
+      a.nlines = ng * L; a.ninner = L;
+      a.in = b.W; a.i_os = L2; a.i_is = 1; a.i_es = L; a.n_in = L; a.pair_in = wp;
+      a.out = b.W; a.o_os = L2; a.o_is = 1; a.o_es = L; a.n_out = ng; a.pair_out = wp;
+      return pl.pairs;
+    default:  // x inverse -> Phi
+      a.nlines = ng * ng; a.ninner = ng;
+      a.in = b.W; a.i_os = L2; a.i_is = L; a.i_es = 1; a.n_in = L; a.pair_in = wp;
+      a.out = b.Phi; a.o_os = (int64_t)ng * ng; a.o_is = ng; a.o_es = 1; a.n_out = ng; a.pair_out = G;
+      a.scale = 1.0f / ((float)L * (float)L * (float)L);
+      return pl.pairs;
+  }
 }
 
 // Arguments of every pass of one convolution (in the order of pass_kinds),
 // and the channel pairs each pass runs over.  Returns the pass count.
 __host__ __device__ inline int conv_setup(const GridParams &g, bool zdirect, const ConvPtrs &b, PassArgs *out,
                                           int *pairs_out) {
-  const int d = g.d, ng = g.ng, L = g.L, H = L / 2 + 1;
-  const Plan pl = make_plan(d, zdirect);
-  const int pairs = pl.pairs;
-  const int64_t G = d == 1 ? ng : d == 2 ? (int64_t)ng * ng : (int64_t)ng * ng * ng;
-  const int kpow = zdirect ? 0 : 2;
-
-  if (d == 1) {
-    PassArgs k = base_args(g, pl, b);  // kernel spectra
-    k.nlines = 1; k.ninner = 1; k.n_in = L; k.n_out = H; k.o_es = 1; k.kpow = kpow;
-    k.out = b.Kh;
-    PassArgs c = base_args(g, pl, b);
-    c.nlines = 1; c.ninner = 1;
-    c.in = b.V; c.i_es = 1; c.n_in = ng; c.pair_in = G;
-    c.out = b.Phi; c.o_es = 1; c.n_out = ng; c.pair_out = G;
-    c.kdim = 1; c.scale = 1.0f / (float)L;
-    out[0] = k; pairs_out[0] = 1;
-    out[1] = c; pairs_out[1] = pairs;
-    return 2;
-  }
-
-  if (d == 2) {
-    PassArgs ka = base_args(g, pl, b);
-    ka.nlines = H; ka.ninner = H; ka.n_in = L; ka.o_is = H; ka.o_es = 1; ka.n_out = H; ka.kpow = kpow;
-    ka.out = b.Kt1;
-    PassArgs kb = base_args(g, pl, b);
-    kb.nlines = H; kb.ninner = H; kb.in = b.Kt1; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
-    kb.out = b.Kh; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
-    const int64_t wp = (int64_t)ng * L;
-    PassArgs pa = base_args(g, pl, b);  // rows: V -> W
-    pa.nlines = ng; pa.ninner = ng;
-    pa.in = b.V; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
-    pa.out = b.W; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
-    PassArgs pb = base_args(g, pl, b);  // columns: fwd, x spectrum, inv
-    pb.nlines = L; pb.ninner = L;
-    pb.in = b.W; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
-    pb.out = b.W; pb.o_is = 1; pb.o_es = L; pb.n_out = ng; pb.pair_out = wp;
-    pb.kdim = 2;
-    PassArgs pc = base_args(g, pl, b);  // rows: inverse -> Phi
-    pc.nlines = ng; pc.ninner = ng;
-    pc.in = b.W; pc.i_is = L; pc.i_es = 1; pc.n_in = L; pc.pair_in = wp;
-    pc.out = b.Phi; pc.o_is = ng; pc.o_es = 1; pc.n_out = ng; pc.pair_out = G;
-    pc.scale = 1.0f / ((float)L * (float)L);
-    out[0] = ka; pairs_out[0] = 1;
-    out[1] = kb; pairs_out[1] = 1;
-    out[2] = pa; pairs_out[2] = pairs;
-    out[3] = pb; pairs_out[3] = pairs;
-    out[4] = pc; pairs_out[4] = pairs;
-    return 5;
-  }
-
-  // d == 3
-  const int64_t H2 = (int64_t)H * H, L2 = (int64_t)L * L;
-  PassArgs ka = base_args(g, pl, b);
-  ka.nlines = (int)H2; ka.ninner = H; ka.n_in = L; ka.kpow = kpow;
-  ka.out = b.Kt1; ka.o_os = H2; ka.o_is = H; ka.o_es = 1; ka.n_out = H;
-  PassArgs kb = base_args(g, pl, b);
-  kb.nlines = (int)H2; kb.ninner = H; kb.in = b.Kt1; kb.i_os = H2; kb.i_is = 1; kb.i_es = H; kb.n_in = L;
-  kb.out = b.Kt2; kb.o_os = H2; kb.o_is = 1; kb.o_es = H; kb.n_out = H;
-  PassArgs kc = base_args(g, pl, b);
-  kc.nlines = (int)H2; kc.ninner = H; kc.in = b.Kt2; kc.i_os = H; kc.i_is = 1; kc.i_es = H2; kc.n_in = L;
-  kc.out = b.Kh; kc.o_os = H; kc.o_is = 1; kc.o_es = H2; kc.n_out = H;
-  const int64_t wp = (int64_t)ng * L2;
-  PassArgs pa = base_args(g, pl, b);  // x: V -> W
-  pa.nlines = ng * ng; pa.ninner = ng;
-  pa.in = b.V; pa.i_os = (int64_t)ng * ng; pa.i_is = ng; pa.i_es = 1; pa.n_in = ng; pa.pair_in = G;
-  pa.out = b.W; pa.o_os = L2; pa.o_is = L; pa.o_es = 1; pa.n_out = L; pa.pair_out = wp;
-  PassArgs pb = base_args(g, pl, b);  // y forward (all ky kept)
-  pb.nlines = ng * L; pb.ninner = L;
-  pb.in = b.W; pb.i_os = L2; pb.i_is = 1; pb.i_es = L; pb.n_in = ng; pb.pair_in = wp;
-  pb.out = b.W; pb.o_os = L2; pb.o_is = 1; pb.o_es = L; pb.n_out = L; pb.pair_out = wp;
-  PassArgs pc = base_args(g, pl, b);  // z: fwd, x spectrum, inv (z < ng kept)
-  pc.nlines = (int)L2; pc.ninner = L;
-  pc.in = b.W; pc.i_os = L; pc.i_is = 1; pc.i_es = L2; pc.n_in = ng; pc.pair_in = wp;
-  pc.out = b.W; pc.o_os = L; pc.o_is = 1; pc.o_es = L2; pc.n_out = ng; pc.pair_out = wp;
-  pc.kdim = 3;
-  PassArgs pd = base_args(g, pl, b);  // y inverse (y < ng kept)
-  pd.nlines = ng * L; pd.ninner = L;
-  pd.in = b.W; pd.i_os = L2; pd.i_is = 1; pd.i_es = L; pd.n_in = L; pd.pair_in = wp;
-  pd.out = b.W; pd.o_os = L2; pd.o_is = 1; pd.o_es = L; pd.n_out = ng; pd.pair_out = wp;
-  PassArgs pe = base_args(g, pl, b);  // x inverse -> Phi
-  pe.nlines = ng * ng; pe.ninner = ng;
-  pe.in = b.W; pe.i_os = L2; pe.i_is = L; pe.i_es = 1; pe.n_in = L; pe.pair_in = wp;
-  pe.out = b.Phi; pe.o_os = (int64_t)ng * ng; pe.o_is = ng; pe.o_es = 1; pe.n_out = ng; pe.pair_out = G;
-  pe.scale = 1.0f / ((float)L * (float)L * (float)L);
-  out[0] = ka; pairs_out[0] = 1;
-  out[1] = kb; pairs_out[1] = 1;
-  out[2] = kc; pairs_out[2] = 1;
-  out[3] = pa; pairs_out[3] = pairs;
-  out[4] = pb; pairs_out[4] = pairs;
-  out[5] = pc; pairs_out[5] = pairs;
-  out[6] = pd; pairs_out[6] = pairs;
-  out[7] = pe; pairs_out[7] = pairs;
-  return 8;
+  const int np = conv_npasses(g.d);
+  for (int i = 0; i < np; ++i) pairs_out[i] = conv_pass(g, zdirect, b, i, out[i]);
+  return np;
 }
 
 }  // namespace fft

@@ -191,7 +191,9 @@ __global__ __launch_bounds__(kFftThreads) void bpass_kernel(const PassArgs *__re
       if (pf[mid] <= u) lo = mid;
       else hi = mid;
     }
-    const PassArgs a = args[(size_t)lo * npasses + pass];
+    // read in place: the block is uniform, so its fields come in through
+    // scalar loads instead of a per-lane private copy
+    const PassArgs &a = args[(size_t)lo * npasses + pass];
     pass_body<IN, OUT, MID, EC>(a, u - pf[lo], blockIdx.y, lds);
   }
 }
@@ -447,14 +449,17 @@ __global__ __launch_bounds__(256) void bplan_kernel(int d, BatchJob *__restrict_
     mx[a] = wave_max(mx[a]);
   }
   bad = wave_sum(bad);
-  if (lane != 0) return;
+  // the butterflies leave the same values in every lane, so every lane takes
+  // the same decisions below: lane 0 writes the job record, lane p the
+  // arguments of pass p (straight into the table, no private copy)
   if (st == BJ_RUN) {
-    J->iter += 1;
-    if (J->iter >= J->max_iter) {
-      J->status = BJ_DONE;
+    const int it = J->iter + 1;
+    if (lane == 0) J->iter = it;
+    if (it >= J->max_iter) {
+      if (lane == 0) J->status = BJ_DONE;
       return;
     }
-  } else {
+  } else if (lane == 0) {
     J->status = BJ_RUN;
   }
   double maxy = 0.0;
@@ -462,11 +467,13 @@ __global__ __launch_bounds__(256) void bplan_kernel(int d, BatchJob *__restrict_
   for (int a = 0; a < d; ++a) {
     finite = finite && isfinite(mn[a]) && isfinite(mx[a]);
     maxy = fmax(maxy, mx[a] - mn[a]);
-    J->minv[a] = mn[a];
-    J->maxv[a] = mx[a];
+    if (lane == 0) {
+      J->minv[a] = mn[a];
+      J->maxv[a] = mx[a];
+    }
   }
   if (!finite || !(maxy > 0.0)) {
-    J->status = BJ_ERR_NUMERIC;
+    if (lane == 0) J->status = BJ_ERR_NUMERIC;
     return;
   }
   int nGrid;
@@ -477,25 +484,28 @@ __global__ __launch_bounds__(256) void bplan_kernel(int d, BatchJob *__restrict_
     nGrid = best_grid_size_in(kGridSizesDev, kGridSizesCount, nGrid);
   }
   if (nGrid + 2 > J->ng_cap) {
-    J->status = BJ_ERR_GRID;
+    if (lane == 0) J->status = BJ_ERR_GRID;
     return;
   }
   GridParams g;
   grid_from_box(d, nGrid, mn, mx, maxy, g);
-  J->g = g;
   // fixed point: |charge| <= maxy / 2 + 1 per point, |weight| <= 1, n points per cell
   const double bound = 2.0 * (double)J->n * (0.5 * maxy + 1.0);
   const int e = 61 - ilogb(bound);
-  J->fix_scale = ldexp(1.0, e);
-  J->fix_inv = ldexp(1.0, -e);
-  ConvPtrs ptr{J->vfix, J->phi, J->W, J->Kt1, J->Kt2, J->Kh, tw};
-  PassArgs pa[kMaxPasses];
-  int pairs[kMaxPasses], kinds[kMaxPasses];
-  const int np = conv_setup(g, J->zdirect != 0, ptr, pa, pairs);
-  pass_kinds(d, kinds);
-  for (int p = 0; p < np; ++p) {
-    if (pass_reads_grid(kinds[p])) pa[p].fix_inv = J->fix_inv;
-    args[(size_t)j * npasses + p] = pa[p];
+  const double fix_inv = ldexp(1.0, -e);
+  if (lane == 0) {
+    J->g = g;
+    J->fix_scale = ldexp(1.0, e);
+    J->fix_inv = fix_inv;
+  }
+  const int np = conv_npasses(d);
+  if (lane < np) {
+    int kinds[kMaxPasses];
+    pass_kinds(d, kinds);
+    ConvPtrs ptr{J->vfix, J->phi, J->W, J->Kt1, J->Kt2, J->Kh, tw};
+    PassArgs &a = args[(size_t)j * npasses + lane];
+    conv_pass(g, J->zdirect != 0, ptr, lane, a);
+    if (pass_reads_grid(kinds[lane])) a.fix_inv = fix_inv;
   }
 }
 
