@@ -32,11 +32,15 @@ namespace sgt {
 namespace fft {
 
 constexpr int kFftThreads = 256;
-constexpr int kLdsElems = 4096;  // complex elements per ping-pong buffer (32 KiB)
+constexpr int kLdsElems = 2048;  // complex elements per ping-pong buffer (16 KiB)
 constexpr int kMaxL = 1024;      // longest line transform
 constexpr int kMaxPasses = 8;
 constexpr int kMaxStages = 10;
-constexpr size_t kLdsBytes = 2 * (size_t)kLdsElems * sizeof(float2);
+constexpr int kPerThread = kLdsElems / kFftThreads;  // elements of a work-group's share per thread
+// two ping-pong buffers and the twiddle table of the line length
+constexpr size_t kLdsBytes = (2 * (size_t)kLdsElems + kMaxL) * sizeof(float2);
+// LDS of one pass's work-group (nl lines of length L)
+__host__ __device__ inline size_t pass_lds_bytes(int nl, int L) { return (2 * (size_t)nl * L + L) * sizeof(float2); }
 
 // Even lengths whose only prime factors are 2, 3 and 5.
 __host__ __device__ inline bool fft_smooth(int L) {
@@ -172,7 +176,9 @@ __device__ __forceinline__ void butterfly(float2 (&v)[R], float sg) {
 }
 
 // One radix-R Stockham stage over nl lines of length L; Ns is the length of
-// the sub-transforms completed by the previous stages.
+// the sub-transforms completed by the previous stages.  A thread owns one
+// butterfly position j and applies it to every G-th line, so the position's
+// index arithmetic and twiddles are computed once per stage, not per line.
 template <int R>
 __device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L, int nl, int Ns,
                                                const float2 *__restrict__ tw, bool inv) {
@@ -180,39 +186,44 @@ __device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L
   const int stride = L / (Ns * R);
   const float invLR = 1.0f / (float)LR, invNs = 1.0f / (float)Ns;
   const float sg = inv ? 1.0f : -1.0f;
-  for (int b = threadIdx.x; b < nl * LR; b += blockDim.x) {
-    const int l = fdiv(b, LR, invLR);
-    const int j = b - l * LR;
+  const int G = LR >= kFftThreads ? 1 : kFftThreads / LR;  // line groups
+  for (int s = threadIdx.x; s < G * LR; s += kFftThreads) {
+    const int g = fdiv(s, LR, invLR);
+    const int j = s - g * LR;
     const int q = fdiv(j, Ns, invNs);
     const int k = j - q * Ns;
-    const float2 *src = x + l * L;
-    float2 *dst = y + l * L;
-    float2 v[R];
-#pragma unroll
-    for (int t = 0; t < R; ++t) v[t] = src[j + t * LR];
+    float2 w[R];
 #pragma unroll
     for (int t = 1; t < R; ++t) {
-      float2 w = tw[t * k * stride];
-      if (inv) w.y = -w.y;
-      v[t] = cmul(v[t], w);
+      w[t] = tw[t * k * stride];
+      if (inv) w[t].y = -w[t].y;
     }
-    butterfly<R>(v, sg);
     const int base = q * Ns * R + k;
+    for (int l = g; l < nl; l += G) {
+      const float2 *src = x + l * L;
+      float2 *dst = y + l * L;
+      float2 v[R];
 #pragma unroll
-    for (int t = 0; t < R; ++t) dst[base + t * Ns] = v[t];
+      for (int t = 0; t < R; ++t) v[t] = src[j + t * LR];
+#pragma unroll
+      for (int t = 1; t < R; ++t) v[t] = cmul(v[t], w[t]);
+      butterfly<R>(v, sg);
+#pragma unroll
+      for (int t = 0; t < R; ++t) dst[base + t * Ns] = v[t];
+    }
   }
 }
 
-// Stockham autosort on nl lines (natural order in and out).  Returns the
-// buffer holding the result.
-__device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int nl, bool inv) {
+// Stockham autosort on nl lines (natural order in and out) with the
+// twiddle table tw (LDS).  Returns the buffer holding the result.
+__device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int nl, bool inv, const float2 *tw) {
   int Ns = 1;
   for (int s = 0; s < a.nst; ++s) {
     const int R = a.rad[s];
-    if (R == 4) stockham_stage<4>(x, y, a.L, nl, Ns, a.tw, inv);
-    else if (R == 2) stockham_stage<2>(x, y, a.L, nl, Ns, a.tw, inv);
-    else if (R == 3) stockham_stage<3>(x, y, a.L, nl, Ns, a.tw, inv);
-    else stockham_stage<5>(x, y, a.L, nl, Ns, a.tw, inv);
+    if (R == 4) stockham_stage<4>(x, y, a.L, nl, Ns, tw, inv);
+    else if (R == 2) stockham_stage<2>(x, y, a.L, nl, Ns, tw, inv);
+    else if (R == 3) stockham_stage<3>(x, y, a.L, nl, Ns, tw, inv);
+    else stockham_stage<5>(x, y, a.L, nl, Ns, tw, inv);
     __syncthreads();
     float2 *tmp = x;
     x = y;
@@ -222,28 +233,96 @@ __device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int 
   return x;
 }
 
+// Visits every element (l, e) of a work-group's nl lines of length L once,
+// calling f(l, e, line, outer, inner) with line = line0 + l split as
+// outer * ninner + inner.  ECONTIG: adjacent threads take adjacent e (the
+// pass axis is contiguous in memory) and a thread walks its lines, stepping
+// (outer, inner) by addition; otherwise adjacent threads take adjacent lines
+// and a thread walks e.  The split is computed once per thread, not per element.
+template <bool ECONTIG, class F>
+__device__ __forceinline__ void for_elements(int nl, int L, int line0, int ninner, float invni, F &&f) {
+  if (ECONTIG) {
+    const int G = L >= kFftThreads ? 1 : kFftThreads / L;
+    const float invL = 1.0f / (float)L;
+    for (int s = threadIdx.x; s < G * L; s += kFftThreads) {
+      const int g = fdiv(s, L, invL);
+      const int e = s - g * L;
+      int line = line0 + g;
+      int outer = fdiv(line, ninner, invni);
+      int inner = line - outer * ninner;
+      for (int l = g; l < nl; l += G) {
+        f(l, e, line, outer, inner);
+        line += G;
+        inner += G;
+        while (inner >= ninner) {
+          inner -= ninner;
+          ++outer;
+        }
+      }
+    }
+  } else {
+    const int G = kFftThreads / nl;
+    const float invnl = 1.0f / (float)nl;
+    for (int s = threadIdx.x; s < G * nl; s += kFftThreads) {
+      const int g = fdiv(s, nl, invnl);
+      const int l = s - g * nl;
+      const int line = line0 + l;
+      const int outer = fdiv(line, ninner, invni);
+      const int inner = line - outer * ninner;
+      for (int e = g; e < L; e += G) f(l, e, line, outer, inner);
+    }
+  }
+}
+
 // One work-group's share of one pass: lines [blk * nl, blk * nl + nl) of
-// channel pair p.  Every thread of the work-group must call it (barriers).
+// channel pair p.  Every thread of the work-group must call it (barriers);
+// the work-group has kFftThreads threads and nl * L <= kLdsElems.
+// The global reads a work-group needs (its lines, the twiddle table and,
+// for a convolution, the kernel spectrum) are all issued before the first
+// barrier, so their latencies overlap instead of adding up stage by stage.
 template <int IN, int OUT, int MID, bool ECONTIG>
 __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds) {
   const int L = a.L;
   const int nl = a.nl;
-  const float invL = 1.0f / (float)L, invnl = 1.0f / (float)nl;
+  // line indices stay below 2^22 (at most 1024^2 lines), where fdiv is exact
+  const float invL = 1.0f / (float)L, invni = 1.0f / (float)a.ninner;
   float2 *b0 = lds;
   float2 *b1 = lds + nl * L;
+  float2 *twl = lds + 2 * nl * L;
   const int line0 = blk * nl;
   const int total = nl * L;
 
+  for (int t = threadIdx.x; t < L; t += kFftThreads) twl[t] = a.tw[t];
+  // kernel spectrum of each element this thread multiplies (real part, and
+  // the imaginary part for the complex K2^ + i K1^ spectrum)
+  float2 kreg[MID == MID_CONV ? kPerThread : 1];
+  if (MID == MID_CONV) {
+#pragma unroll
+    for (int t = 0; t < kPerThread; ++t) {
+      const int idx = threadIdx.x + t * kFftThreads;
+      kreg[t] = make_float2(0.f, 0.f);
+      if (idx < total) {
+        const int l = fdiv(idx, L, invL), e = idx - l * L;
+        const int line = line0 + l;
+        if (line < a.nlines) {
+          const int outer = fdiv(line, a.ninner, invni);
+          const int inner = line - outer * a.ninner;
+          const int me = mirror(e, L);
+          int64_t ki;
+          if (a.kdim == 1) ki = me;
+          else if (a.kdim == 2) ki = (int64_t)me * a.H + mirror(inner, L);
+          else ki = ((int64_t)me * a.H + mirror(outer, L)) * a.H + mirror(inner, L);
+          const float2 k = a.Kh[ki];
+          kreg[t] = make_float2(k.x, a.kcplx[p] ? k.y : 0.0f);
+        }
+      }
+    }
+  }
+
   // ---- load (zero padding beyond n_in) ----
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int l, e;
-    if (ECONTIG) { l = fdiv(idx, L, invL); e = idx - l * L; }
-    else { e = fdiv(idx, nl, invnl); l = idx - e * nl; }
-    const int line = line0 + l;
+  for_elements<ECONTIG>(nl, L, line0, a.ninner, invni, [&](int l, int e, int line, int outer, int inner) {
     float2 v = make_float2(0.f, 0.f);
     if (line < a.nlines && e < a.n_in) {
-      const int outer = line / a.ninner;
-      const int inner = line - outer * a.ninner;
       if (IN == IN_VPAIR) {
         const float *V = static_cast<const float *>(a.in);
         const int64_t off = outer * a.i_os + inner * a.i_is + e * a.i_es;
@@ -276,45 +355,29 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
       }
     }
     b0[l * L + e] = v;
-  }
+  });
   __syncthreads();
 
-  float2 *r = stockham(b0, b1, a, nl, MID == MID_INV);
+  float2 *r = stockham(b0, b1, a, nl, MID == MID_INV, twl);
   if (MID == MID_CONV) {
-    // multiply by the real kernel spectrum, then inverse transform
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int l = fdiv(idx, L, invL), e = idx - l * L;
-      const int line = line0 + l;
-      float kv = 0.f, kvi = 0.f;
-      if (line < a.nlines) {
-        const int outer = line / a.ninner;
-        const int inner = line - outer * a.ninner;
-        const int me = mirror(e, L);
-        int64_t ki;
-        if (a.kdim == 1) ki = me;
-        else if (a.kdim == 2) ki = (int64_t)me * a.H + mirror(inner, L);
-        else ki = ((int64_t)me * a.H + mirror(outer, L)) * a.H + mirror(inner, L);
-        const float2 k = a.Kh[ki];
-        kv = k.x;
-        kvi = a.kcplx[p] ? k.y : 0.0f;
+    // multiply by the kernel spectrum, then inverse transform
+#pragma unroll
+    for (int t = 0; t < kPerThread; ++t) {
+      const int idx = threadIdx.x + t * kFftThreads;
+      if (idx < total) {
+        const float kv = kreg[t].x, kvi = kreg[t].y;
+        const float2 v = r[idx];
+        r[idx] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
       }
-      const float2 v = r[idx];
-      r[idx] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
     }
     __syncthreads();
     float2 *other = (r == b0) ? b1 : b0;
-    r = stockham(r, other, a, nl, true);
+    r = stockham(r, other, a, nl, true, twl);
   }
 
   // ---- store ----
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    int l, e;
-    if (ECONTIG) { l = fdiv(idx, L, invL); e = idx - l * L; }
-    else { e = fdiv(idx, nl, invnl); l = idx - e * nl; }
-    const int line = line0 + l;
-    if (line >= a.nlines || e >= a.n_out) continue;
-    const int outer = line / a.ninner;
-    const int inner = line - outer * a.ninner;
+  for_elements<ECONTIG>(nl, L, line0, a.ninner, invni, [&](int l, int e, int line, int outer, int inner) {
+    if (line >= a.nlines || e >= a.n_out) return;
     const int64_t off = outer * a.o_os + inner * a.o_is + e * a.o_es;
     const float2 v = r[l * L + e];
     if (OUT == OUT_W) {
@@ -328,7 +391,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
       float2 *K = static_cast<float2 *>(a.out);
       K[off] = v;
     }
-  }
+  });
   __syncthreads();  // LDS is reused by the caller's next share
 }
 
