@@ -1357,8 +1357,9 @@ int sgtsne_copy_probe(int64_t bytes, int reps, double *ms_per_copy) {
   return st;
 }
 
-int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D) {
+int sgtsne_knn(const float *X, int n, int dim, int k, int method, int32_t *I, double *D, int *fallback_rows) {
   SGT_CHECK(X && I && D, SGTSNE_ERR_ARG, "NULL argument");
+  SGT_CHECK(method >= SGTSNE_KNN_AUTO && method <= SGTSNE_KNN_FILTERED, SGTSNE_ERR_ARG, "unknown kNN method %d", method);
   SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "need n > k >= 1 and dim >= 1");
   hipStream_t s = nullptr;
   SGT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -1374,7 +1375,7 @@ int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *
     set_error("copy of X failed");
     st = SGTSNE_ERR_HIP;
   }
-  if (!st) st = launch_knn_exact(s, Xd, n, dim, k, Id, Dd);
+  if (!st) st = launch_knn(s, Xd, n, dim, k, Id, Dd, method, fallback_rows);
   if (!st && (hipMemcpyAsync(I, Id, (size_t)n * (k + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
               hipMemcpyAsync(D, Dd, (size_t)n * (k + 1) * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)) {
     set_error("copy of kNN result failed");
@@ -1389,6 +1390,10 @@ int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *
   free_dev(Id);
   (void)hipStreamDestroy(s);
   return st;
+}
+
+int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D) {
+  return sgtsne_knn(X, n, dim, k, SGTSNE_KNN_AUTO, I, D, nullptr);
 }
 
 int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float *Phi) {

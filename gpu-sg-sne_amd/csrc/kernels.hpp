@@ -103,6 +103,8 @@ int fft_length_for(int ng);
 
 // ---- P construction (kernels_prep.hip) ----
 int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D);
+int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
+               int *fallback_rows);
 int launch_perplexity(hipStream_t s, const double *D, int n, int k, double u, float *val,
                       double *beta_out);
 int launch_lambda_rescale(hipStream_t s, int n, const int32_t *rowptr, float *val, double lambda,

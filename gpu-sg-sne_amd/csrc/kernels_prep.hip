@@ -15,7 +15,9 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <climits>
 #include <vector>
 
@@ -28,7 +30,9 @@ namespace {
 // micro-tile of (query, reference) distances accumulated over the dimensions
 // in order with fma(diff, diff, acc).  Candidates below a query's current
 // worst key (distance, index) are queued in LDS and merged by one thread per
-// query into a max-heap of k entries kept in global memory.
+// query into a max-heap of k entries kept in global memory.  Queries are the
+// slots [0, nq): slot s is point qlist[s] (point s without a list); heaps are
+// indexed by slot.
 constexpr int kKnnQ = 64;
 constexpr int kKnnR = 32;
 constexpr int kKnnDC = 16;  // dimensions staged per round
@@ -40,7 +44,8 @@ __device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
 
 __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restrict__ X, int n,
                                                           int dim, int k, double *__restrict__ hd,
-                                                          int *__restrict__ hi) {
+                                                          int *__restrict__ hi, const int *__restrict__ qlist,
+                                                          int nq) {
   __shared__ double qs[kKnnDC][kKnnQ + 1];
   __shared__ double rs[kKnnDC][kKnnR + 1];
   __shared__ double qd[kKnnQ][kKnnR];
@@ -48,6 +53,7 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
   __shared__ int qcnt[kKnnQ];
   __shared__ double thr_d[kKnnQ];
   __shared__ int thr_i[kKnnQ];
+  __shared__ int qid[kKnnQ];  // point of each slot (-1: none)
 
   const int q0 = blockIdx.x * kKnnQ;
   const int tid = threadIdx.x;
@@ -57,11 +63,12 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
     qcnt[tid] = 0;
     thr_d[tid] = INFINITY;
     thr_i[tid] = INT_MAX;
-    const int q = q0 + tid;
-    if (q < n)
+    const int slot = q0 + tid;
+    qid[tid] = slot < nq ? (qlist ? qlist[slot] : slot) : -1;
+    if (slot < nq)
       for (int e = 0; e < k; ++e) {
-        hd[(size_t)q * k + e] = INFINITY;
-        hi[(size_t)q * k + e] = INT_MAX;
+        hd[(size_t)slot * k + e] = INFINITY;
+        hi[(size_t)slot * k + e] = INT_MAX;
       }
   }
   __syncthreads();
@@ -75,8 +82,8 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
     for (int l0 = 0; l0 < dim; l0 += kKnnDC) {
       for (int idx = tid; idx < kKnnDC * kKnnQ; idx += kKnnThreads) {
         const int qq = idx / kKnnDC, ll = idx % kKnnDC;
-        const int q = q0 + qq, l = l0 + ll;
-        qs[ll][qq] = (q < n && l < dim) ? X[(size_t)q * dim + l] : 0.0;
+        const int q = qid[qq], l = l0 + ll;
+        qs[ll][qq] = (q >= 0 && l < dim) ? X[(size_t)q * dim + l] : 0.0;
         if (qq < kKnnR) {
           const int r = r0 + qq;
           rs[ll][qq] = (r < n && l < dim) ? X[(size_t)r * dim + l] : 0.0;
@@ -104,11 +111,11 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const int ql = tq * 4 + a;
-      const int q = q0 + ql;
+      const int q = qid[ql];
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const int r = r0 + tr + 16 * b;
-        if (q < n && r < n && r != q && key_less(acc[a][b], r, thr_d[ql], thr_i[ql])) {
+        if (q >= 0 && r < n && r != q && key_less(acc[a][b], r, thr_d[ql], thr_i[ql])) {
           const int pos = atomicAdd(&qcnt[ql], 1);
           qd[ql][pos] = acc[a][b];
           qi[ql][pos] = r;
@@ -118,11 +125,11 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
     __syncthreads();
     // merge queued candidates into the heaps (one thread per query)
     if (tid < kKnnQ) {
-      const int q = q0 + tid;
+      const int slot = q0 + tid;
       const int cnt = qcnt[tid];
-      if (q < n && cnt > 0) {
-        double *H = hd + (size_t)q * k;
-        int *HI = hi + (size_t)q * k;
+      if (slot < nq && cnt > 0) {
+        double *H = hd + (size_t)slot * k;
+        int *HI = hi + (size_t)slot * k;
         double rd = H[0];
         int ri = HI[0];
         for (int c = 0; c < cnt; ++c) {
@@ -156,13 +163,15 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double *__restri
   }
 }
 
-// Heap -> ascending rows with the point itself first.
-__global__ void knn_finalize_kernel(int n, int k, double *__restrict__ hd, int *__restrict__ hi,
-                                    int32_t *__restrict__ I, double *__restrict__ D) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  double *H = hd + (size_t)q * k;
-  int *HI = hi + (size_t)q * k;
+// Heap -> ascending rows with the point itself first (slots as knn_kernel).
+__global__ void knn_finalize_kernel(int nq, int k, double *__restrict__ hd, int *__restrict__ hi,
+                                    const int *__restrict__ qlist, int32_t *__restrict__ I,
+                                    double *__restrict__ D) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nq) return;
+  const int q = qlist ? qlist[slot] : slot;
+  double *H = hd + (size_t)slot * k;
+  int *HI = hi + (size_t)slot * k;
   int32_t *Iq = I + (size_t)q * (k + 1);
   double *Dq = D + (size_t)q * (k + 1);
   Iq[0] = q;
@@ -187,6 +196,445 @@ __global__ void knn_finalize_kernel(int n, int k, double *__restrict__ hd, int *
     }
     H[pos] = cd;
     HI[pos] = ci;
+  }
+}
+
+// ---- kNN through an FP32 matrix-core filter --------------------------------
+// The exact distances above cost two FP64 vector operations per (pair,
+// dimension).  Here the squared distances of the mean-centred points, rounded
+// to FP32, come from matrix-core dot products, |x|^2 + |y|^2 - 2 x.y, and each
+// query keeps its k + kFiltSlack nearest candidates under that approximation.
+// A rigorous bound on the approximation error then certifies, per query, that
+// no point outside the candidate list can be among the exact k nearest; the
+// candidates' distances are recomputed with exactly the arithmetic of
+// knn_kernel and the k smallest (distance, index) keys are kept, so a
+// certified row equals knn_kernel's row bit for bit.  Rows that cannot be
+// certified (ties or near-ties wider than the slack) go through knn_kernel.
+constexpr int kFiltQ = 64;        // queries per work-group, 16 per wave
+constexpr int kFiltR = 64;        // references per tile, 4 blocks of 16
+constexpr int kFiltMaxDp = 128;   // padded dimensions (A operands held in registers)
+constexpr int kFiltRS = 80;       // LDS row stride of the transposed tile (16 mod 32 banks)
+constexpr int kFiltSlack = 16;    // candidates kept beyond k
+constexpr int kRerank = 256;      // re-rank threads and sort width (k + slack <= 256)
+
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bool key_lessf(float da, int ia, float db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// A reference tile is kFiltR rows of NST float4.  Element t of the tile is
+// row t % kFiltR, float4 column t / kFiltR: adjacent threads take adjacent
+// rows, so the transposed LDS stores are conflict-free.  All of a thread's
+// loads are issued before any is used.
+template <int T, int NST>
+__device__ __forceinline__ void tile_load(const float *__restrict__ Xc, const float *__restrict__ n32, int n, int r0,
+                                          float4 (&buf)[kFiltR * NST / T], float &rnv) {
+  const float4 *X4 = reinterpret_cast<const float4 *>(Xc);
+#pragma unroll
+  for (int u = 0; u < kFiltR * NST / T; ++u) {
+    const int t = threadIdx.x + u * T;
+    const int j = t % kFiltR, c4 = t / kFiltR;
+    buf[u] = r0 + j < n ? X4[(size_t)(r0 + j) * NST + c4] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  rnv = (threadIdx.x < kFiltR && r0 + (int)threadIdx.x < n) ? n32[r0 + threadIdx.x] : 0.0f;
+}
+
+// Stores a loaded tile transposed: rt[c * kFiltRS + j] = Xc[r0 + j][c].
+template <int T, int NST>
+__device__ __forceinline__ void tile_store(float *rt, float *rn, const float4 (&buf)[kFiltR * NST / T], float rnv) {
+#pragma unroll
+  for (int u = 0; u < kFiltR * NST / T; ++u) {
+    const int t = threadIdx.x + u * T;
+    const int j = t % kFiltR, c4 = t / kFiltR;
+    float *col = rt + 4 * c4 * kFiltRS + j;
+    col[0] = buf[u].x;
+    col[kFiltRS] = buf[u].y;
+    col[2 * kFiltRS] = buf[u].z;
+    col[3 * kFiltRS] = buf[u].w;
+  }
+  if (threadIdx.x < kFiltR) rn[threadIdx.x] = rnv;
+}
+
+// The dot products of a wave's 16 queries with the tile's 64 references:
+// acc[t][v] = x(query 16 w + 4 (lane / 16) + v) . x(reference 16 t + lane % 16).
+template <int NST>
+__device__ __forceinline__ void tile_dots(const float (&areg)[NST], const float *rt, int lane, fv4 (&acc)[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = fv4{0.0f, 0.0f, 0.0f, 0.0f};
+  // B operand of step s, block t: row (dimension) 4 s + lane / 16, column
+  // (reference) 16 t + lane % 16
+  const float *rrow = rt + (lane >> 4) * kFiltRS + (lane & 15);
+#pragma unroll
+  for (int st = 0; st < NST; ++st)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[st], rrow[4 * st * kFiltRS + 16 * t], acc[t], 0, 0, 0);
+}
+
+// Column means (one work-group of 256 threads per dimension).
+__global__ __launch_bounds__(256) void knn_mean_kernel(const double *__restrict__ X, int n, int dim,
+                                                       double *__restrict__ mu) {
+  __shared__ double part[4];
+  const int l = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += X[(size_t)i * dim + l];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) mu[l] = (part[0] + part[1] + part[2] + part[3]) / (double)n;
+}
+
+// Centred points rounded to FP32 and zero-padded to dp dimensions, their
+// squared norms in FP32 (filter) and norms in FP64 (error bound); the
+// largest norm goes to *rmax (non-negative doubles order as their bits).
+__global__ void knn_center_kernel(const double *__restrict__ X, int n, int dim, int dp,
+                                  const double *__restrict__ mu, float *__restrict__ Xc,
+                                  float *__restrict__ n32, double *__restrict__ nrm,
+                                  unsigned long long *__restrict__ rmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ss = 0.0;
+  for (int l = 0; l < dp; ++l) {
+    const float v = l < dim ? (float)(X[(size_t)i * dim + l] - mu[l]) : 0.0f;
+    Xc[(size_t)i * dp + l] = v;
+    ss += (double)v * (double)v;
+  }
+  n32[i] = (float)ss;
+  const double r = sqrt(ss);
+  nrm[i] = r;
+  atomicMax(rmax, (unsigned long long)__double_as_longlong(r));
+}
+
+// Candidate lists: a max-heap of kp (approximate distance, index) keys per
+// query in global memory (lists longer than the LDS version holds).  Wave w
+// owns queries q0 + 16w .. q0 + 16w + 15.  The points have 4 NST dimensions.
+template <int NST>
+__global__ __launch_bounds__(256) void knn_filter_kernel(const float *__restrict__ Xc,
+                                                         const float *__restrict__ n32, int n, int kp,
+                                                         float *__restrict__ hd, int *__restrict__ hi) {
+  __shared__ float rt[4 * NST * kFiltRS];
+  __shared__ float rn[kFiltR];
+  __shared__ float qd[kFiltQ][kFiltR];
+  __shared__ int qi[kFiltQ][kFiltR];
+  __shared__ int qcnt[kFiltQ];
+  __shared__ float thr_d[kFiltQ];
+  __shared__ int thr_i[kFiltQ];
+  __shared__ float qn[kFiltQ];
+
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int q0 = blockIdx.x * kFiltQ;
+  // A operand of step s: row (query) lane % 16, column (dimension) 4 s + lane / 16
+  float areg[NST];
+  {
+    const int qa = q0 + 16 * w + (lane & 15);
+#pragma unroll
+    for (int st = 0; st < NST; ++st) areg[st] = qa < n ? Xc[(size_t)qa * 4 * NST + 4 * st + (lane >> 4)] : 0.0f;
+  }
+  if (tid < kFiltQ) {
+    qcnt[tid] = 0;
+    thr_d[tid] = INFINITY;
+    thr_i[tid] = INT_MAX;
+    const int q = q0 + tid;
+    qn[tid] = q < n ? n32[q] : 0.0f;
+    if (q < n)
+      for (int e = 0; e < kp; ++e) {
+        hd[(size_t)q * kp + e] = INFINITY;
+        hi[(size_t)q * kp + e] = INT_MAX;
+      }
+  }
+  float4 buf[kFiltR * NST / 256];
+  float rnv;
+  tile_load<256, NST>(Xc, n32, n, 0, buf, rnv);
+  for (int r0 = 0; r0 < n; r0 += kFiltR) {
+    __syncthreads();  // the previous tile's LDS is free
+    tile_store<256, NST>(rt, rn, buf, rnv);
+    __syncthreads();
+    // the next tile's loads fly while this one is multiplied and merged
+    if (r0 + kFiltR < n) tile_load<256, NST>(Xc, n32, n, r0 + kFiltR, buf, rnv);
+    fv4 acc[4];
+    tile_dots<NST>(areg, rt, lane, acc);
+    // output v of block t: query 16 w + 4 (lane / 16) + v, reference 16 t + lane % 16
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int jl = 16 * t + (lane & 15), r = r0 + jl;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ql = 16 * w + 4 * (lane >> 4) + v, q = q0 + ql;
+        const float dd = (qn[ql] + rn[jl]) - 2.0f * acc[t][v];
+        if (q < n && r < n && r != q && key_lessf(dd, r, thr_d[ql], thr_i[ql])) {
+          const int pos = atomicAdd(&qcnt[ql], 1);
+          qd[ql][pos] = dd;
+          qi[ql][pos] = r;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < kFiltQ) {
+      const int q = q0 + tid;
+      const int cnt = qcnt[tid];
+      if (q < n && cnt > 0) {
+        float *H = hd + (size_t)q * kp;
+        int *HI = hi + (size_t)q * kp;
+        float rd = H[0];
+        int ri = HI[0];
+        for (int c = 0; c < cnt; ++c) {
+          const float cd = qd[tid][c];
+          const int ci = qi[tid][c];
+          if (!key_lessf(cd, ci, rd, ri)) continue;
+          int pos = 0;
+          for (;;) {
+            const int l = 2 * pos + 1, rr = l + 1;
+            int m = -1;
+            float md = cd;
+            int mi = ci;
+            if (l < kp && key_lessf(md, mi, H[l], HI[l])) { m = l; md = H[l]; mi = HI[l]; }
+            if (rr < kp && key_lessf(md, mi, H[rr], HI[rr])) { m = rr; md = H[rr]; mi = HI[rr]; }
+            if (m < 0) break;
+            H[pos] = md;
+            HI[pos] = mi;
+            pos = m;
+          }
+          H[pos] = cd;
+          HI[pos] = ci;
+          rd = H[0];
+          ri = HI[0];
+        }
+        thr_d[tid] = rd;
+        thr_i[tid] = ri;
+      }
+      qcnt[tid] = 0;
+    }
+  }
+}
+
+// The same filter with the candidate heaps in LDS (lists of up to kF2MaxKp):
+// 32 queries per work-group (two waves).  A tile's approximate distances go
+// to LDS; each query's candidates of the tile form a 64-bit mask built from
+// wave ballots (16 bits per 16-lane group), and the merge walks the mask in
+// reference order and sifts into the LDS heap.
+constexpr int kF2Q = 32;
+constexpr int kF2MaxKp = 128;
+
+template <int NST>
+__global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__restrict__ Xc,
+                                                             const float *__restrict__ n32, int n, int kp,
+                                                             float *__restrict__ hd, int *__restrict__ hi) {
+  extern __shared__ float dyn[];  // heap keys [kF2Q][kp] | heap indices [kF2Q][kp]
+  float *hk = dyn;
+  int *hx = reinterpret_cast<int *>(hk + (size_t)kF2Q * kp);
+  __shared__ float rt[4 * NST * kFiltRS];
+  __shared__ float dist[kF2Q][kFiltR];
+  __shared__ unsigned long long mask[kF2Q];
+  __shared__ float rn[kFiltR];
+  __shared__ float qn[kF2Q];
+
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4;
+  const int q0 = blockIdx.x * kF2Q;
+  float areg[NST];
+  {
+    const int qa = q0 + 16 * w + (lane & 15);
+#pragma unroll
+    for (int st = 0; st < NST; ++st) areg[st] = qa < n ? Xc[(size_t)qa * 4 * NST + 4 * st + (lane >> 4)] : 0.0f;
+  }
+  for (int t = tid; t < kF2Q * kp; t += 128) {
+    hk[t] = INFINITY;
+    hx[t] = INT_MAX;
+  }
+  if (tid < kF2Q) qn[tid] = q0 + tid < n ? n32[q0 + tid] : 0.0f;
+  float4 buf[kFiltR * NST / 128];
+  float rnv;
+  tile_load<128, NST>(Xc, n32, n, 0, buf, rnv);
+  for (int r0 = 0; r0 < n; r0 += kFiltR) {
+    __syncthreads();  // the previous tile's merge is done
+    tile_store<128, NST>(rt, rn, buf, rnv);
+    __syncthreads();
+    if (r0 + kFiltR < n) tile_load<128, NST>(Xc, n32, n, r0 + kFiltR, buf, rnv);
+    fv4 acc[4];
+    tile_dots<NST>(areg, rt, lane, acc);
+    // this lane's four query rows 16 w + 4 g + v: norms and current worst keys
+    float qv[4], thd[4];
+    int thi[4];
+    bool qok[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ql = 16 * w + 4 * g + v;
+      qv[v] = qn[ql];
+      thd[v] = hk[ql * kp];
+      thi[v] = hx[ql * kp];
+      qok[v] = q0 + ql < n;
+    }
+    unsigned long long mq[4] = {0ull, 0ull, 0ull, 0ull};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int jl = 16 * t + (lane & 15), r = r0 + jl;
+      const float rv = rn[jl];
+      const bool rok = r < n;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ql = 16 * w + 4 * g + v;
+        const float dd = (qv[v] + rv) - 2.0f * acc[t][v];
+        dist[ql][jl] = dd;
+        const bool cand = qok[v] && rok && r != q0 + ql && key_lessf(dd, r, thd[v], thi[v]);
+        const unsigned long long b = __ballot(cand);
+        mq[v] |= ((b >> (16 * g)) & 0xFFFFull) << (16 * t);
+      }
+    }
+    if ((lane & 15) == 0)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) mask[16 * w + 4 * g + v] = mq[v];
+    __syncthreads();
+    if (tid < kF2Q) {
+      float *H = hk + tid * kp;
+      int *HI = hx + tid * kp;
+      unsigned long long m = mask[tid];
+      while (m) {
+        const int j = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const float cd = dist[tid][j];
+        const int ci = r0 + j;
+        if (!key_lessf(cd, ci, H[0], HI[0])) continue;
+        int pos = 0;
+        for (;;) {
+          const int l = 2 * pos + 1, rr = l + 1;
+          int mm = -1;
+          float md = cd;
+          int mi = ci;
+          if (l < kp && key_lessf(md, mi, H[l], HI[l])) { mm = l; md = H[l]; mi = HI[l]; }
+          if (rr < kp && key_lessf(md, mi, H[rr], HI[rr])) { mm = rr; md = H[rr]; mi = HI[rr]; }
+          if (mm < 0) break;
+          H[pos] = md;
+          HI[pos] = mi;
+          pos = mm;
+        }
+        H[pos] = cd;
+        HI[pos] = ci;
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < kF2Q * kp; t += 128) {
+    const int ql = t / kp, q = q0 + ql;
+    if (q < n) {
+      hd[(size_t)q * kp + (t - ql * kp)] = hk[t];
+      hi[(size_t)q * kp + (t - ql * kp)] = hx[t];
+    }
+  }
+}
+
+// Step counts (dimensions / 4, rounded up) the filters are built for.
+template <int NST>
+int launch_filter(hipStream_t s, const float *Xc, const float *n32, int n, int kp, float *hd, int *hi) {
+  if (kp <= kF2MaxKp) {
+    const size_t lds = (sizeof(float) + sizeof(int)) * (size_t)kF2Q * kp;
+    knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(Xc, n32, n, kp, hd, hi);
+  } else {
+    knn_filter_kernel<NST><<<div_up(n, kFiltQ), 256, 0, s>>>(Xc, n32, n, kp, hd, hi);
+  }
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int filter_steps(int dim) {
+  const int nst = (dim + 3) / 4;
+  return nst <= 4 ? 4 : nst <= 8 ? 8 : nst <= 12 ? 12 : nst <= 16 ? 16 : nst <= 24 ? 24 : 32;
+}
+
+int launch_filter_for(hipStream_t s, int nst, const float *Xc, const float *n32, int n, int kp, float *hd, int *hi) {
+  switch (nst) {
+    case 4: return launch_filter<4>(s, Xc, n32, n, kp, hd, hi);
+    case 8: return launch_filter<8>(s, Xc, n32, n, kp, hd, hi);
+    case 12: return launch_filter<12>(s, Xc, n32, n, kp, hd, hi);
+    case 16: return launch_filter<16>(s, Xc, n32, n, kp, hd, hi);
+    case 24: return launch_filter<24>(s, Xc, n32, n, kp, hd, hi);
+    default: return launch_filter<32>(s, Xc, n32, n, kp, hd, hi);
+  }
+}
+
+// Ascending bitonic sort of kRerank (key, index) pairs in LDS by the key, then
+// the index.  Every thread of the work-group calls it.
+template <class K>
+__device__ void bitonic_sort(K *key, int *id) {
+  const int t = threadIdx.x;
+  for (int size = 2; size <= kRerank; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      const int p = t ^ stride;
+      if (p > t) {
+        const bool up = (t & size) == 0;
+        const K a = key[t], b = key[p];
+        const int ia = id[t], ib = id[p];
+        const bool a_after_b = b < a || (b == a && ib < ia);
+        if (a_after_b == up) {
+          key[t] = b;
+          key[p] = a;
+          id[t] = ib;
+          id[p] = ia;
+        }
+      }
+    }
+  __syncthreads();
+}
+
+// One work-group per query: certify the candidate list, recompute the
+// candidates' distances as knn_kernel does (diff = x_q - x_c in FP64,
+// acc = fma(diff, diff, acc) over the dimensions in order) and write the row.
+// cE, ce: the filter's error in squared distance is at most cE (|x_q| + R)^2
+// and the FP32 rounding of the centred points moves a distance by at most
+// ce (|x_q| + R), with R the largest centred norm.
+__global__ __launch_bounds__(kRerank) void knn_rerank_kernel(
+    const double *__restrict__ X, int n, int dim, int k, int kp, const float *__restrict__ hd,
+    const int *__restrict__ hi, const double *__restrict__ nrm, const unsigned long long *__restrict__ rmax,
+    double cE, double ce, int32_t *__restrict__ I, double *__restrict__ D, int *__restrict__ fb_list,
+    int *__restrict__ fb_cnt) {
+  __shared__ float sk[kRerank];
+  __shared__ double sd[kRerank];
+  __shared__ int si[kRerank];
+  __shared__ int ok;
+  const int q = blockIdx.x, t = threadIdx.x;
+  sk[t] = t < kp ? hd[(size_t)q * kp + t] : INFINITY;
+  si[t] = t < kp ? hi[(size_t)q * kp + t] : INT_MAX;
+  bitonic_sort(sk, si);
+  if (t == 0) {
+    bool good = kp >= n - 1;  // every other point is a candidate
+    if (!good) {
+      const double R = __longlong_as_double((long long)*rmax);
+      const double sn = nrm[q] + R;
+      const double E = cE * sn * sn, e = ce * sn;
+      const double lo = sqrt(fmax((double)sk[kp - 1] - E, 0.0)) - e;  // any non-candidate is at least this far
+      const double hi_k = sqrt(fmax((double)sk[k - 1] + E, 0.0)) + e;  // the k nearest candidates are within this
+      good = lo > hi_k * (1.0 + 1e-12) + 1e-300;
+    }
+    ok = good;
+  }
+  const int c = t < kp ? si[t] : INT_MAX;
+  double acc = INFINITY;
+  // a list left short (NaN distances never enter it) is not certified
+  const bool invalid = t < kp && (c < 0 || c >= n);
+  if (t < kp && c >= 0 && c < n) {
+    acc = 0.0;
+    const double *xq = X + (size_t)q * dim;
+    const double *xc = X + (size_t)c * dim;
+    for (int l = 0; l < dim; ++l) {
+      const double df = xq[l] - xc[l];
+      acc = fma(df, df, acc);
+    }
+  }
+  __syncthreads();  // thread 0 has written ok
+  if (invalid) ok = 0;
+  sd[t] = acc;
+  si[t] = c;
+  bitonic_sort(sd, si);
+  int32_t *Iq = I + (size_t)q * (k + 1);
+  double *Dq = D + (size_t)q * (k + 1);
+  if (t == 0) {
+    Iq[0] = q;
+    Dq[0] = 0.0;
+    if (!ok) fb_list[atomicAdd(fb_cnt, 1)] = q;
+  }
+  if (t < k) {
+    Iq[1 + t] = si[t];
+    Dq[1 + t] = sd[t];
   }
 }
 
@@ -419,20 +867,85 @@ __global__ void to_double_kernel(const float *__restrict__ in, double *__restric
 
 }  // namespace
 
-int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D) {
-  SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "kNN needs n > k >= 1 (n=%d k=%d)", n, k);
-  SGT_CHECK(k <= kKnnR * 64, SGTSNE_ERR_ARG, "k too large");
+namespace {
+
+// Exact kNN of the query slots [0, nq) (points qlist[slot], or all points).
+int knn_exact_rows(hipStream_t s, const double *X, int n, int dim, int k, const int *qlist, int nq, int32_t *I,
+                   double *D) {
   double *hd = nullptr;
   int *hi = nullptr;
-  SGT_HIP(hipMallocAsync((void **)&hd, sizeof(double) * (size_t)n * k, s));
-  SGT_HIP(hipMallocAsync((void **)&hi, sizeof(int) * (size_t)n * k, s));
-  knn_kernel<<<div_up(n, kKnnQ), kKnnThreads, 0, s>>>(X, n, dim, k, hd, hi);
+  SGT_HIP(hipMallocAsync((void **)&hd, sizeof(double) * (size_t)nq * k, s));
+  SGT_HIP(hipMallocAsync((void **)&hi, sizeof(int) * (size_t)nq * k, s));
+  knn_kernel<<<div_up(nq, kKnnQ), kKnnThreads, 0, s>>>(X, n, dim, k, hd, hi, qlist, nq);
   SGT_LAUNCH_CHECK();
-  knn_finalize_kernel<<<div_up(n, 256), 256, 0, s>>>(n, k, hd, hi, I, D);
+  knn_finalize_kernel<<<div_up(nq, 256), 256, 0, s>>>(nq, k, hd, hi, qlist, I, D);
   SGT_LAUNCH_CHECK();
   SGT_HIP(hipFreeAsync(hd, s));
   SGT_HIP(hipFreeAsync(hi, s));
   return SGTSNE_OK;
+}
+
+}  // namespace
+
+int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
+               int *fallback_rows) {
+  SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "kNN needs n > k >= 1 (n=%d k=%d)", n, k);
+  SGT_CHECK(k <= kKnnR * 64, SGTSNE_ERR_ARG, "k too large");
+  if (fallback_rows) *fallback_rows = -1;
+  const int nst = filter_steps(dim), dp = 4 * nst;
+  const bool filter_ok = k + kFiltSlack <= kRerank && dim <= kFiltMaxDp && n >= kFiltQ;
+  if (method == SGTSNE_KNN_EXACT_FP64 || (method == SGTSNE_KNN_AUTO && !filter_ok))
+    return knn_exact_rows(s, X, n, dim, k, nullptr, n, I, D);
+  SGT_CHECK(filter_ok, SGTSNE_ERR_ARG, "the filtered kNN needs k <= %d, dim <= %d and n >= %d",
+            kRerank - kFiltSlack, kFiltMaxDp, kFiltQ);
+  const int kp = std::min(k + kFiltSlack, n - 1);
+  double *mu = nullptr, *nrm = nullptr;
+  float *Xc = nullptr, *n32 = nullptr, *hd = nullptr;
+  int *hi = nullptr, *fb = nullptr;
+  unsigned long long *rmax = nullptr;
+  SGT_HIP(hipMallocAsync((void **)&mu, sizeof(double) * dim, s));
+  SGT_HIP(hipMallocAsync((void **)&nrm, sizeof(double) * (size_t)n, s));
+  SGT_HIP(hipMallocAsync((void **)&Xc, sizeof(float) * (size_t)n * dp, s));
+  SGT_HIP(hipMallocAsync((void **)&n32, sizeof(float) * (size_t)n, s));
+  SGT_HIP(hipMallocAsync((void **)&hd, sizeof(float) * (size_t)n * kp, s));
+  SGT_HIP(hipMallocAsync((void **)&hi, sizeof(int) * (size_t)n * kp, s));
+  SGT_HIP(hipMallocAsync((void **)&fb, sizeof(int) * ((size_t)n + 1), s));  // [0]: count, then rows
+  SGT_HIP(hipMallocAsync((void **)&rmax, sizeof(unsigned long long), s));
+  SGT_HIP(hipMemsetAsync(rmax, 0, sizeof(unsigned long long), s));
+  SGT_HIP(hipMemsetAsync(fb, 0, sizeof(int), s));
+  knn_mean_kernel<<<dim, 256, 0, s>>>(X, n, dim, mu);
+  SGT_LAUNCH_CHECK();
+  knn_center_kernel<<<div_up(n, 256), 256, 0, s>>>(X, n, dim, dp, mu, Xc, n32, nrm, rmax);
+  SGT_LAUNCH_CHECK();
+  SGT_TRY(launch_filter_for(s, nst, Xc, n32, n, kp, hd, hi));
+  // error constants (unit roundoff u = 2^-24): the squared distance from the
+  // filter is within (2 dp + 3) u (|x| + |y|)^2 of that of the FP32 points
+  // under round-to-nearest; allowing twice that covers directed rounding in
+  // the accumulation, and 1e-30 covers underflow.  Rounding the centred points
+  // to FP32 moves a distance by at most 1.01 u (|x| + |y|).
+  const double u = std::ldexp(1.0, -24);
+  const double cE = 2.0 * (2.0 * dp + 8.0) * u, ce = 1.05 * u;
+  knn_rerank_kernel<<<n, kRerank, 0, s>>>(X, n, dim, k, kp, hd, hi, nrm, rmax, cE, ce, I, D, fb + 1, fb);
+  SGT_LAUNCH_CHECK();
+  int nfb = 0;
+  SGT_HIP(hipMemcpyAsync(&nfb, fb, sizeof(int), hipMemcpyDeviceToHost, s));
+  SGT_HIP(hipStreamSynchronize(s));
+  int st = SGTSNE_OK;
+  if (nfb > 0) st = knn_exact_rows(s, X, n, dim, k, fb + 1, nfb, I, D);
+  if (fallback_rows) *fallback_rows = nfb;
+  (void)hipFreeAsync(mu, s);
+  (void)hipFreeAsync(nrm, s);
+  (void)hipFreeAsync(Xc, s);
+  (void)hipFreeAsync(n32, s);
+  (void)hipFreeAsync(hd, s);
+  (void)hipFreeAsync(hi, s);
+  (void)hipFreeAsync(fb, s);
+  (void)hipFreeAsync(rmax, s);
+  return st;
+}
+
+int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D) {
+  return launch_knn(s, X, n, dim, k, I, D, SGTSNE_KNN_AUTO, nullptr);
 }
 
 int launch_perplexity(hipStream_t s, const double *D, int n, int k, double u, float *val,

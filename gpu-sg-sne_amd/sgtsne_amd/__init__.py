@@ -30,9 +30,12 @@ ZETA_ALGEBRAIC = 1
 ERR_FULL = -8
 JOB_RUNNING = 0
 JOB_DONE = 1
+KNN_AUTO = 0
+KNN_EXACT_FP64 = 1
+KNN_FILTERED = 2
 
 EXPORTED = [
-    "sgtsne_params_default", "sgtsne_last_error", "sgtsne_version", "sgtsne_knn_exact",
+    "sgtsne_params_default", "sgtsne_last_error", "sgtsne_version", "sgtsne_knn_exact", "sgtsne_knn",
     "sgtsne_perplexity_equalize", "sgtsne_grid_conv", "sgtsne_run", "sgtsne_ctx_create", "sgtsne_ctx_upload_P",
     "sgtsne_ctx_get_P", "sgtsne_ctx_set_Y", "sgtsne_ctx_get_Y", "sgtsne_ctx_step",
     "sgtsne_ctx_terms", "sgtsne_ctx_kl", "sgtsne_ctx_repulsive_exact", "sgtsne_ctx_timing",
@@ -91,6 +94,7 @@ def lib():
         "sgtsne_last_error": (C.c_char_p, []),
         "sgtsne_version": (C.c_char_p, []),
         "sgtsne_knn_exact": (C.c_int, [f32p, C.c_int, C.c_int, C.c_int, i32p, f64p]),
+        "sgtsne_knn": (C.c_int, [f32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p, f64p, C.POINTER(C.c_int)]),
         "sgtsne_perplexity_equalize": (C.c_int, [i32p, f64p, C.c_int, C.c_int, C.c_double, i32p, i32p, f32p]),
         "sgtsne_grid_conv": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, f32p, f32p]),
         "sgtsne_run": (C.c_int, [C.c_int, C.c_int64, i32p, i32p, f32p, P, vp, f32p, vp]),
@@ -187,6 +191,18 @@ def knn_exact(X, k):
     D = np.empty((n, k + 1), np.float64)
     _check(lib().sgtsne_knn_exact(X, n, dim, int(k), I, D))
     return I, D
+
+
+def knn(X, k, method=KNN_AUTO):
+    """knn_exact with a chosen method (KNN_AUTO, KNN_EXACT_FP64, KNN_FILTERED);
+    returns (I, D, rows the filtered method recomputed exactly, or -1)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, dim = X.shape
+    I = np.empty((n, k + 1), np.int32)
+    D = np.empty((n, k + 1), np.float64)
+    fb = C.c_int(-1)
+    _check(lib().sgtsne_knn(X, n, dim, int(k), int(method), I, D, C.byref(fb)))
+    return I, D, fb.value
 
 
 def grid_conv(V, h, zdirect=False):

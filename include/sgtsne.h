@@ -113,6 +113,23 @@ const char *sgtsne_version(void);
  * FLANN search the reference calls from CU/testing_modules/demo_perplexity.cu:131. */
 int sgtsne_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D);
 
+/* kNN methods.  Every method returns the rows of sgtsne_knn_exact bit for bit.
+ * EXACT_FP64: FP64 distances for every pair.  FILTERED: FP32 matrix-core
+ * distances of the mean-centred points select k + 16 candidates per point; a
+ * rigorous bound on their error certifies that the exact k nearest are among
+ * them, and the candidates' FP64 distances are recomputed exactly as
+ * EXACT_FP64 does; rows that cannot be certified (ties or near-ties wider than
+ * the 16 spare candidates) are recomputed with EXACT_FP64.  FILTERED needs
+ * k <= 240, dim <= 128 and n >= 64; AUTO takes it when it applies. */
+#define SGTSNE_KNN_AUTO 0
+#define SGTSNE_KNN_EXACT_FP64 1
+#define SGTSNE_KNN_FILTERED 2
+
+/* sgtsne_knn_exact with a chosen method; *fallback_rows (may be NULL)
+ * receives the rows FILTERED recomputed exactly (-1 when it did not run).
+ * No counterpart in the reference; sgtsne_knn_exact uses AUTO. */
+int sgtsne_knn(const float *X, int n, int dim, int k, int method, int32_t *I, double *D, int *fallback_rows);
+
 /* Perplexity equalisation on the GPU (FP64 bisection, |H - log u| < 1e-5,
  * <= 200 steps, self column dropped, rows normalised).  I, D as above with
  * k = nn.  Output CSR: rowptr [n+1] (= k*i), col [n*k], val [n*k] float.

@@ -54,6 +54,47 @@ def test_knn_ties(sg, O):
     assert np.array_equal(I, Io) and np.array_equal(D, Do)
 
 
+@pytest.mark.parametrize("method", ["EXACT_FP64", "FILTERED"])
+@pytest.mark.parametrize("n,dim,k", [(1000, 50, 90), (777, 7, 30), (130, 3, 113), (5000, 50, 90), (3000, 128, 45)])
+def test_knn_methods_bitexact(sg, O, method, n, dim, k):
+    # both methods give the oracle's rows bit for bit; (130, 3, 113): every
+    # other point is a candidate, so the filter's certificate holds trivially
+    X = _mixture(n, dim, 5, n + dim)
+    I, D, fb = sg.knn(X, k, getattr(sg, "KNN_" + method))
+    Io, Do = O.knn_exact(X, k)
+    assert np.array_equal(I, Io) and np.array_equal(D, Do)
+    assert (fb == -1) == (method == "EXACT_FP64")
+
+
+def test_knn_filtered_falls_back_on_wide_ties(sg, O):
+    # integer lattice: around an interior point the 33rd to 56th neighbours
+    # all lie at squared distance 5, a tie wider than the 16 spare candidates,
+    # so those rows cannot be certified and are recomputed exactly; every row
+    # still equals the oracle's
+    g = np.arange(8, dtype=np.float32)
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    I, D, fb = sg.knn(X, 33, sg.KNN_FILTERED)
+    Io, Do = O.knn_exact(X, 33)
+    assert fb > 0
+    assert np.array_equal(I, Io) and np.array_equal(D, Do)
+
+
+def test_knn_filtered_offset_data(sg, O):
+    # a large common offset is removed by the centring before FP32 rounding
+    X = _mixture(2000, 20, 4, 9) + np.float32(3.0e3)
+    I, D, fb = sg.knn(X, 40, sg.KNN_FILTERED)
+    Io, Do = O.knn_exact(X, 40)
+    assert np.array_equal(I, Io) and np.array_equal(D, Do)
+
+
+def test_knn_methods_agree_large(sg):
+    X = _mixture(20000, 50, 12, 3)
+    Ie, De, _ = sg.knn(X, 90, sg.KNN_EXACT_FP64)
+    If, Df, fb = sg.knn(X, 90, sg.KNN_FILTERED)
+    assert np.array_equal(Ie, If) and np.array_equal(De, Df)
+    assert 0 <= fb < 200
+
+
 # ---------------------------------------------------------------- P construction
 def test_perplexity_golden(sg, mnist):
     f, g = mnist
