@@ -262,8 +262,10 @@ def main_leg(sg, args):
     prof = os.path.join(ROOT, "profiles", "attractive_spmv_powerlaw1.3M_d2.json")
     if os.path.exists(prof) and args.workload == "powerlaw40_50d" and args.d == 2 and n == 1_300_000:
         pj = json.load(open(prof))
-        if pj["kernel"].replace(" ", "") == roofline["kernel"].replace(" ", ""):
-            roofline["traffic"] = pj["traffic_bytes_per_launch"]
+        kern = {k.replace(" ", ""): v for k, v in pj["kernels"].items()}
+        hit = kern.get(roofline["kernel"].replace(" ", ""))
+        if hit is not None:
+            roofline["traffic"] = hit["traffic_bytes_per_launch"]
             roofline["traffic_source"] = "profiles/attractive_spmv_powerlaw1.3M_d2.json (rocprofv3 --pmc)"
 
     cpu = None
@@ -340,7 +342,7 @@ def main():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--d", type=int, default=2)
     ap.add_argument("--format", default="CSR_SEG")
-    ap.add_argument("--reorder", type=int, default=2, help="device point order: 0 given, 1 BFS of P, 2 BFS + Morton of Y")
+    ap.add_argument("--reorder", type=int, default=2, help="device point order: 0 given, 1 reverse Cuthill-McKee of P, 2 that plus Morton re-orders of Y kept when faster")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-iters", type=int, default=30)

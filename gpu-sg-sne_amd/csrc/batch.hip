@@ -92,7 +92,7 @@ struct sgtsne_batch {
   hipGraphExec_t exec = nullptr;
   bool graph_valid = false;
   bool use_graph = true;
-  int64_t graph_replays = 0;
+  int64_t graph_replays = 0, graph_updates = 0, graph_instantiations = 0;
 };
 
 namespace {
@@ -122,7 +122,12 @@ int auto_ng_cap(int d, int n) {
   return n <= 20000 ? 64 : 128;
 }
 
-void invalidate_graph(sgtsne_batch *b) {
+// The captured iteration no longer matches the arena: the next step
+// re-captures it and updates the executable graph in place.
+void invalidate_graph(sgtsne_batch *b) { b->graph_valid = false; }
+
+// Destroys the executable graph (no launch of it may be pending).
+void release_graph(sgtsne_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -196,20 +201,41 @@ int enqueue_iteration(sgtsne_batch *b) {
   return SGTSNE_OK;
 }
 
+// Captures one iteration.  When an executable graph exists, it is updated
+// from the new capture (same kernels, new arguments: much cheaper than a new
+// instantiation); only a refused update re-instantiates.
 int capture(sgtsne_batch *b) {
-  invalidate_graph(b);
+  b->graph_valid = false;
   if (!b->use_graph || b->nitems == 0) return SGTSNE_OK;
   SGT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   const int st = enqueue_iteration(b);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
-  SGT_TRY(st);
+  if (st != SGTSNE_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
   if (e != hipSuccess || g == nullptr) {
     set_error("graph capture failed: %s", hipGetErrorString(e));
     return SGTSNE_ERR_HIP;
   }
+  if (b->exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate(b->exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->graph = g;
+      b->graph_updates++;
+      b->graph_valid = true;
+      return SGTSNE_OK;
+    }
+    (void)hipGetLastError();
+    SGT_HIP(hipStreamSynchronize(b->stream));  // launches of the old graph may be pending
+    release_graph(b);
+  }
   b->graph = g;
   SGT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_instantiations++;
   b->graph_valid = true;
   return SGTSNE_OK;
 }
@@ -523,7 +549,7 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
 
 void destroy_all(sgtsne_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  invalidate_graph(b);
+  release_graph(b);
   // per-job grids come from the stream-ordered pool: release them the same way
   for (auto &h : b->hj)
     if (h.gmem) {
@@ -617,6 +643,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
       st = SGTSNE_ERR_HIP;
     }
   };
+  retain_pool_memory();
   chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   chk(hipStreamCreateWithFlags(&b->prep, hipStreamNonBlocking), "stream");
   chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
@@ -872,7 +899,7 @@ int sgtsne_batch_set_graph(sgtsne_batch *b, int on) {
   SGT_TRY(check_batch(b));
   SGT_HIP(hipStreamSynchronize(b->stream));
   b->use_graph = on != 0;
-  invalidate_graph(b);
+  release_graph(b);
   return SGTSNE_OK;
 }
 

@@ -917,6 +917,7 @@ int sgtsne_ctx_create(sgtsne_ctx **out, const sgtsne_params *params) {
     sgtsne_ctx_destroy(c);
     return code;
   };
+  retain_pool_memory();
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return fail(SGTSNE_ERR_HIP);

@@ -102,6 +102,7 @@ int init_twiddles(hipStream_t s, float2 *tw);
 int fft_length_for(int ng);
 
 // ---- P construction (kernels_prep.hip) ----
+void retain_pool_memory();
 int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D);
 int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
                int *fallback_rows);
@@ -126,6 +127,7 @@ int spatial_order(hipStream_t s, int n, int d, int ds, const float *Y, const dou
 int launch_gather_points(hipStream_t s, int n, int ds, const int32_t *order, const float *in, float *out);
 int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
                        const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
-                       int32_t *col2, float *val2);
+                       int32_t *col2, float *val2,
+                       bool sort_rows = true);
 
 }  // namespace sgt

@@ -968,6 +968,17 @@ int launch_rowids(hipStream_t s, int n, const int32_t *rowptr, int32_t *row) {
   return SGTSNE_OK;
 }
 
+// Keeps memory freed to the device's stream-ordered pool cached in the pool:
+// with the default release threshold (0) every synchronisation returns it to
+// the driver, and the next hipMallocAsync blocks the host to map it again.
+void retain_pool_memory() {
+  int dev = 0;
+  hipMemPool_t pool = nullptr;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+  uint64_t keep = UINT64_MAX;
+  (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+}
+
 int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n) {
   size_t tb = 0;
   SGT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, s));
@@ -1046,13 +1057,20 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, c
 
 int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
                        const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
-                       int32_t *col2, float *val2) {
+                       int32_t *col2, float *val2, bool sort_rows) {
   int32_t *len = nullptr;
   SGT_HIP(hipMallocAsync((void **)&len, sizeof(int32_t) * ((size_t)n + 1), s));
   perm_len_kernel<<<div_up(n, 256), 256, 0, s>>>(n, perm, rp, len);
   SGT_LAUNCH_CHECK();
   SGT_HIP(hipMemsetAsync(len + n, 0, sizeof(int32_t), s));
   SGT_TRY(device_exclusive_scan(s, len, rp2, n + 1));
+  if (!sort_rows) {
+    // rows keep the permuted order of their columns (no host round trip)
+    perm_fill_kernel<<<div_up((int64_t)n * 64, 256), 256, 0, s>>>(n, perm, iperm, rp, col, val, rp2, col2, val2);
+    SGT_LAUNCH_CHECK();
+    SGT_HIP(hipFreeAsync(len, s));
+    return SGTSNE_OK;
+  }
   int32_t *ctmp = nullptr;
   float *vtmp = nullptr;
   SGT_HIP(hipMallocAsync((void **)&ctmp, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1), s));
