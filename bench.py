@@ -184,6 +184,30 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True):
     return out
 
 
+def knn_leg(sg, n=200_000, k=90):
+    """Front-end kNN of one 50-D Gaussian-mixture job: the filtered method
+    against the exact FP64 one (rows must be identical).  Times include the
+    host-side conversion and copies of the C-ABI call."""
+    from sgtsne_amd import synth
+
+    X, _ = synth.gaussian_job(n, seed=5)
+    sg.knn(X[:2000], k, sg.KNN_FILTERED)  # module load, pools
+    res = {}
+    rows = {}
+    for name in ("FILTERED", "EXACT_FP64"):
+        t0 = time.perf_counter()
+        I, D, fb = sg.knn(X, k, getattr(sg, "KNN_" + name))
+        res[name.lower() + "_s"] = round(time.perf_counter() - t0, 4)
+        rows[name] = (I, D)
+        if name == "FILTERED":
+            res["fallback_rows"] = fb
+    res["identical"] = bool(np.array_equal(rows["FILTERED"][0], rows["EXACT_FP64"][0])
+                            and np.array_equal(rows["FILTERED"][1], rows["EXACT_FP64"][1]))
+    res.update({"n": n, "dim": X.shape[1], "k": k})
+    log(f"[bench] knn: {res}")
+    return res
+
+
 def main_leg(sg, args):
     """N=1.3M (by default) single-job ms/iter with roofline, CPU baseline and KL check."""
     k, u = 90, 30.0
@@ -352,7 +376,8 @@ def main():
     ap.add_argument("--kl-check", action="store_true",
                     help="run the FP64 oracle for all iterations from the same P_sym and start; compare final KL")
     ap.add_argument("--legs", default="main,batched,served",
-                    help="comma list of main (N=1.3M ms/iter), batched (256 jobs), served (request stream)")
+                    help="comma list of main (N=1.3M ms/iter), batched (256 jobs), served (request stream), "
+                         "knn (front-end kNN methods at N=200k; not in the default list)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-sample", type=int, default=16)
@@ -387,6 +412,7 @@ def main():
     g = not args.no_graph
     batched = batched_leg(sg, args.batch_jobs, args.batch_sample, graph=g) if "batched" in legs else None
     served = served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g) if "served" in legs else None
+    knn = knn_leg(sg) if "knn" in legs else None
 
     value = m.get("value")
     out = {
@@ -413,6 +439,7 @@ def main():
         "kl_check": m.get("kl_check"),
         "batched": batched,
         "served": served,
+        "knn": knn,
     }
     print(json.dumps(out), flush=True)
 
