@@ -128,28 +128,51 @@ __global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n,
   const int node = lane % NODES, slot = lane / NODES;
   const int i0 = blockIdx.x * chunk;
   const int i1 = min(n, i0 + chunk);
-  for (int base = i0 + wave * PPW * U; base < i1; base += nwaves * PPW * U) {
+  // each point slot of each wave spreads a contiguous run of points, and a
+  // lane adds its successive points in registers while they hit the same grid
+  // node: in the locality order consecutive points mostly share a cell, so
+  // most of the LDS atomics of the plain per-point form become register adds
+  const int per_wave = (chunk + nwaves - 1) / nwaves;
+  const int per_slot = (per_wave + PPW - 1) / PPW;
+  const int w0 = min(i1, i0 + wave * per_wave + slot * per_slot);
+  const int w1 = min(min(i1, i0 + (wave + 1) * per_wave), w0 + per_slot);
+  float *cp = sgrid + (slot % copies) * MG;  // the point slots of one instruction use different copies
+  int cur = -1;
+  float acc[M];
+#pragma unroll
+  for (int a = 0; a < M; ++a) acc[a] = 0.0f;
+  for (int base = w0; base < w1; base += U) {
     float y[U][D];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = base + u * PPW + slot;
-      if (i < i1) load_pt<D>(Y, i, y[u]);
+      const int i = base + u;
+      if (i < w1) load_pt<D>(Y, i, y[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int i = base + u * PPW + slot;
-      if (i < i1) {
+      const int i = base + u;
+      if (i < w1) {
         int idx;
         float ww;
         node_of_lane<D>(g, y[u], node, idx, ww);
-        // the point slots of one instruction (and consecutive instructions of
-        // one slot) write different copies: no same-address LDS atomics
-        float *cp = sgrid + ((slot + u) % copies) * MG;
-        atomicAdd(&cp[idx], ww);
+        if (idx != cur) {
+          if (cur >= 0) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) atomicAdd(&cp[(a + 1) * G + idx], (y[u][a] - g.center[a]) * ww);
+            for (int a = 0; a < M; ++a) atomicAdd(&cp[a * G + cur], acc[a]);
+          }
+          cur = idx;
+#pragma unroll
+          for (int a = 0; a < M; ++a) acc[a] = 0.0f;
+        }
+        acc[0] += ww;
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a + 1] += (y[u][a] - g.center[a]) * ww;
       }
     }
+  }
+  if (cur >= 0) {
+#pragma unroll
+    for (int a = 0; a < M; ++a) atomicAdd(&cp[a * G + cur], acc[a]);
   }
   __syncthreads();
   // sum of the copies -> this block's slice of the partial buffer (coalesced)
