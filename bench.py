@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=40.0)
-    ap.add_argument("--serve-tick", type=int, default=25)
+    ap.add_argument("--serve-tick", type=int, default=10)
     ap.add_argument("--no-graph", action="store_true",
                     help="batched and served engines launch each kernel instead of replaying a graph")
     args = ap.parse_args()

@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <random>
 #include <vector>
 
@@ -46,7 +47,21 @@ struct HostJob {
   size_t gbytes = 0;
   void *gmem = nullptr;  // vfix | phi | W | Kt1 | Kt2 | Kh
   int reorders = 0;
+  // recorded on the iteration stream after the step that ran the job's last
+  // iteration: its Y and record can then be read without waiting for later
+  // iterations (unless a compaction moved the job since)
+  hipEvent_t done_ev = nullptr;
+  bool done_marked = false, moved_after_done = false;
 };
+
+// A span of the pinned staging ring and the event after the copies reading it.
+struct StageSpan {
+  size_t off, len;
+  hipEvent_t ev;  // nullptr until the copies reading the span are queued
+};
+
+constexpr size_t kStageBytes = 16u << 20;  // pinned staging for host -> device updates
+constexpr int kMarks = 16;                 // tick marks kept
 
 }  // namespace
 
@@ -55,7 +70,17 @@ struct sgtsne_batch {
   int d = 2, ds = 2;
   hipStream_t stream = nullptr;  // iterations and every arena write
   hipStream_t prep = nullptr;    // front end of new jobs (kNN, perplexity, P + P^T)
+  hipStream_t copy = nullptr;    // reads of finished jobs
   hipEvent_t ev_prep = nullptr;
+  // host -> device updates go through a pinned ring, so queuing them never
+  // waits for the iterations queued ahead of them
+  char *stage = nullptr;
+  size_t stage_head = 0;
+  std::deque<StageSpan> stage_live;
+  std::vector<hipEvent_t> stage_free;
+  // marks recorded on the iteration stream (pacing of queued iterations)
+  hipEvent_t marks[kMarks] = {};
+  int64_t nmarks = 0;
   int max_jobs = 0;
   int64_t max_points = 0, max_nnz = 0;
   size_t grid_budget = 0, grid_used = 0;
@@ -126,6 +151,63 @@ int auto_ng_cap(int d, int n) {
 // re-captures it and updates the executable graph in place.
 void invalidate_graph(sgtsne_batch *b) { b->graph_valid = false; }
 
+// Reserves `bytes` of the staging ring, first waiting for the copies that
+// still read an overlapping span; nullptr when the request exceeds the ring.
+void *stage_alloc(sgtsne_batch *b, size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes > kStageBytes || !b->stage) return nullptr;
+  size_t off = b->stage_head;
+  if (off + bytes > kStageBytes) off = 0;
+  for (auto it = b->stage_live.begin(); it != b->stage_live.end();) {
+    const bool overlap = it->off < off + bytes && off < it->off + it->len;
+    const bool done = it->ev && hipEventQuery(it->ev) == hipSuccess;
+    if (overlap || done) {
+      if (overlap && !done && it->ev) (void)hipEventSynchronize(it->ev);
+      if (it->ev) b->stage_free.push_back(it->ev);
+      it = b->stage_live.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  b->stage_live.push_back(StageSpan{off, bytes, nullptr});
+  b->stage_head = off + bytes;
+  return b->stage + off;
+}
+
+// Records one event after the queued copies that read the spans reserved
+// since the last commit.
+int stage_commit(sgtsne_batch *b) {
+  hipEvent_t ev = nullptr;
+  bool any = false;
+  for (auto &sp : b->stage_live) any = any || sp.ev == nullptr;
+  if (!any) return SGTSNE_OK;
+  if (!b->stage_free.empty()) {
+    ev = b->stage_free.back();
+    b->stage_free.pop_back();
+  } else {
+    SGT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  SGT_HIP(hipEventRecord(ev, b->stream));
+  for (auto &sp : b->stage_live)
+    if (sp.ev == nullptr) sp.ev = ev;
+  return SGTSNE_OK;
+}
+
+// Queues a host -> device copy on the iteration stream; `src` may be reused
+// as soon as this returns.
+int upload(sgtsne_batch *b, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return SGTSNE_OK;
+  void *p = stage_alloc(b, bytes);
+  if (!p) {  // larger than the ring: copy and wait
+    SGT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream));
+    SGT_HIP(hipStreamSynchronize(b->stream));
+    return SGTSNE_OK;
+  }
+  std::memcpy(p, src, bytes);
+  SGT_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, b->stream));
+  return SGTSNE_OK;
+}
+
 // Destroys the executable graph (no launch of it may be pending).
 void release_graph(sgtsne_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -137,10 +219,7 @@ void release_graph(sgtsne_batch *b) {
 
 // Writes the static part of slot j's record (everything before the dynamic
 // fields) so that a running job's iteration state is left alone.
-int write_static(sgtsne_batch *b, int j) {
-  SGT_HIP(hipMemcpyAsync(b->jobs_d + j, &b->jobs_h[j], offsetof(BatchJob, iter), hipMemcpyHostToDevice, b->stream));
-  return SGTSNE_OK;
-}
+int write_static(sgtsne_batch *b, int j) { return upload(b, b->jobs_d + j, &b->jobs_h[j], offsetof(BatchJob, iter)); }
 
 int rebuild_items(sgtsne_batch *b) {
   std::vector<BatchItem> items;
@@ -156,12 +235,11 @@ int rebuild_items(sgtsne_batch *b) {
   }
   SGT_CHECK((int)items.size() <= b->items_cap, SGTSNE_ERR_ARG, "work items exceed capacity");
   b->nitems = (int)items.size();
-  // the H2D copies read pageable host memory: wait for them before it goes away
-  if (!items.empty())
-    SGT_HIP(hipMemcpyAsync(b->items_d, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
-                           b->stream));
+  // stream-ordered after the iterations already queued, which still read the
+  // old table; staged, so nothing here waits for them
+  SGT_TRY(upload(b, b->items_d, items.data(), items.size() * sizeof(BatchItem)));
   for (int j : b->arena) SGT_TRY(write_static(b, j));
-  SGT_HIP(hipStreamSynchronize(b->stream));
+  SGT_TRY(stage_commit(b));
   invalidate_graph(b);
   return SGTSNE_OK;
 }
@@ -254,7 +332,12 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
   for (int j : b->arena) {
     HostJob &h = b->hj[j];
     const int64_t end = (int64_t)h.max_iter + 1;
+    const int64_t t0 = h.t;
     h.t = std::min<int64_t>(h.t + count, end);
+    if (t0 < end && h.t >= end && h.done_ev) {
+      SGT_HIP(hipEventRecord(h.done_ev, b->stream));
+      h.done_marked = true;
+    }
   }
   return SGTSNE_OK;
 }
@@ -327,6 +410,7 @@ int compact(sgtsne_batch *b) {
   for (int j : b->arena) {
     HostJob &h = b->hj[j];
     if (h.off != cur_p || h.nnz_off != cur_z) {
+      if (h.done_marked) h.moved_after_done = true;
       const size_t pb = (size_t)h.n * b->ds * sizeof(float);
       float *arr[3] = {b->Y, b->uY, b->gains};
       for (float *a : arr) {
@@ -386,7 +470,9 @@ int reorder_job(sgtsne_batch *b, int j) {
   SGT_HIP(hipMallocAsync((void **)&val2, sizeof(float) * std::max<int64_t>(nnz, 1), s));
   SGT_TRY(launch_offset_copy_i32(s, b->rowptr + h.off, rpl, n + 1, (int32_t)(-h.nnz_off)));
   SGT_TRY(launch_offset_copy_i32(s, b->col + h.nnz_off, coll, nnz, -h.off));
-  SGT_TRY(launch_permute_csr(s, n, nnz, order, inv, rpl, coll, b->val + h.nnz_off, rp2, col2, val2));
+  // rows re-sorted on the device alone: a host round trip here would wait
+  // for every iteration queued ahead of the re-order
+  SGT_TRY(launch_permute_csr(s, n, nnz, order, inv, rpl, coll, b->val + h.nnz_off, rp2, col2, val2, true, true));
   SGT_TRY(launch_offset_copy_i32(s, rp2, b->rowptr + h.off, n + 1, (int32_t)h.nnz_off));
   SGT_TRY(launch_offset_copy_i32(s, col2, b->col + h.nnz_off, nnz, h.off));
   if (nnz > 0)
@@ -510,8 +596,7 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   BatchJob &J = b->jobs_h[slot];
   J = BatchJob{};
   for (int a = 0; a < 4; ++a) J.mean0[a] = a < d ? (float)(mean[a] / n) : 0.0f;
-  SGT_HIP(hipMemcpyAsync(b->Y + (size_t)h.off * ds, yh.data(), yh.size() * sizeof(float), hipMemcpyHostToDevice,
-                         b->stream));
+  SGT_TRY(upload(b, b->Y + (size_t)h.off * ds, yh.data(), yh.size() * sizeof(float)));
   SGT_TRY(launch_fill(b->stream, b->uY + (size_t)h.off * ds, (int64_t)n * ds, 0.0f));
   SGT_TRY(launch_fill(b->stream, b->gains + (size_t)h.off * ds, (int64_t)n * ds, 1.0f));
   SGT_TRY(launch_iota(b->stream, n, b->perm + h.off));
@@ -534,7 +619,8 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   J.Kh = reinterpret_cast<float2 *>(gm + vb + pb + wb + 2 * kb);
   J.iter = 0;
   J.status = BJ_INIT;
-  SGT_HIP(hipMemcpyAsync(b->jobs_d + slot, &J, sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
+  SGT_TRY(upload(b, b->jobs_d + slot, &J, sizeof(BatchJob)));
+  if (!h.done_ev) SGT_HIP(hipEventCreateWithFlags(&h.done_ev, hipEventDisableTiming));
 
   h.live = true;
   b->used_points += n;
@@ -542,7 +628,7 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   b->grid_used += gbytes;
   // rowptr[used_points] is this job's end: the next job starts there
   b->arena.push_back(slot);
-  SGT_TRY(rebuild_items(b));  // synchronises the stream (yh, J are host locals)
+  SGT_TRY(rebuild_items(b));  // commits the staged uploads above too
   if (job_out) *job_out = slot;
   return SGTSNE_OK;
 }
@@ -575,8 +661,30 @@ void destroy_all(sgtsne_batch *b) {
   bfree(b->prefix_d);
   bfree(b->tw);
   bfree(b->scal_d);
+  if (b->prep) (void)hipStreamSynchronize(b->prep);
+  if (b->copy) (void)hipStreamSynchronize(b->copy);
+  for (auto &h : b->hj)
+    if (h.done_ev) {
+      (void)hipEventDestroy(h.done_ev);
+      h.done_ev = nullptr;
+    }
+  for (auto &sp : b->stage_live)
+    if (sp.ev) b->stage_free.push_back(sp.ev);
+  b->stage_live.clear();
+  std::sort(b->stage_free.begin(), b->stage_free.end());  // one event may guard several spans
+  b->stage_free.erase(std::unique(b->stage_free.begin(), b->stage_free.end()), b->stage_free.end());
+  for (hipEvent_t e : b->stage_free) (void)hipEventDestroy(e);
+  b->stage_free.clear();
+  if (b->stage) (void)hipHostFree(b->stage);
+  b->stage = nullptr;
+  for (hipEvent_t &e : b->marks)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
   if (b->ev_prep) (void)hipEventDestroy(b->ev_prep);
   if (b->prep) (void)hipStreamDestroy(b->prep);
+  if (b->copy) (void)hipStreamDestroy(b->copy);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -586,10 +694,14 @@ int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int lay
   const int n = h.n, d = b->d, ds = b->ds;
   std::vector<float> v((size_t)n * ds);
   std::vector<int32_t> pm(n);
-  SGT_HIP(hipMemcpyAsync(v.data(), src + (size_t)h.off * ds, v.size() * sizeof(float), hipMemcpyDeviceToHost,
-                         b->stream));
-  SGT_HIP(hipMemcpyAsync(pm.data(), b->perm + h.off, sizeof(int32_t) * n, hipMemcpyDeviceToHost, b->stream));
-  SGT_HIP(hipStreamSynchronize(b->stream));
+  // a finished job's rows no longer change: read them right after its last
+  // iteration instead of after everything queued since
+  const bool fast = src == b->Y && h.done_marked && !h.moved_after_done;
+  const hipStream_t cs = fast ? b->copy : b->stream;
+  if (fast) SGT_HIP(hipStreamWaitEvent(b->copy, h.done_ev, 0));
+  SGT_HIP(hipMemcpyAsync(v.data(), src + (size_t)h.off * ds, v.size() * sizeof(float), hipMemcpyDeviceToHost, cs));
+  SGT_HIP(hipMemcpyAsync(pm.data(), b->perm + h.off, sizeof(int32_t) * n, hipMemcpyDeviceToHost, cs));
+  SGT_HIP(hipStreamSynchronize(cs));
   for (int r = 0; r < n; ++r)
     SGT_CHECK(pm[r] >= 0 && pm[r] < n, SGTSNE_ERR_STATE, "job %d: point order corrupt at row %d", job, r);
   for (int r = 0; r < n; ++r) {
@@ -646,7 +758,10 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   retain_pool_memory();
   chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   chk(hipStreamCreateWithFlags(&b->prep, hipStreamNonBlocking), "stream");
+  chk(hipStreamCreateWithFlags(&b->copy, hipStreamNonBlocking), "stream");
   chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
+  for (hipEvent_t &e : b->marks) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+  chk(hipHostMalloc((void **)&b->stage, kStageBytes, hipHostMallocDefault), "pinned staging");
   const size_t P = (size_t)max_points;
   if (!st) st = balloc((void **)&b->rowptr, sizeof(int32_t) * (P + 1));
   if (!st) st = balloc((void **)&b->col, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1));
@@ -705,9 +820,11 @@ int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr,
   const hipStream_t s = b->prep;
   int32_t *rp = nullptr, *cc = nullptr;
   float *vv = nullptr;
-  SGT_TRY(balloc((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
-  SGT_TRY(balloc((void **)&cc, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
-  SGT_TRY(balloc((void **)&vv, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)));
+  // temporaries come from the stream-ordered pool: no release here waits for
+  // the iterations queued on the other stream
+  SGT_HIP(hipMallocAsync((void **)&rp, sizeof(int32_t) * ((size_t)n + 1), s));
+  SGT_HIP(hipMallocAsync((void **)&cc, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1), s));
+  SGT_HIP(hipMallocAsync((void **)&vv, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1), s));
   int st = SGTSNE_OK;
   if (hipMemcpyAsync(rp, rowptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
       (nnz > 0 && (hipMemcpyAsync(cc, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -720,18 +837,27 @@ int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr,
   int64_t m = nnz;
   if (!st && symmetrize) {
     if (b->p.lambda != 1.0) st = launch_lambda_rescale(s, n, rp, vv, b->p.lambda, b->p.drop_leaf);
-    if (!st) st = gpu_symmetrize(s, n, nnz, rp, cc, vv, &srp, &scc, &svv, &m);
+    if (!st) st = gpu_symmetrize(s, n, nnz, rp, cc, vv, &srp, &scc, &svv, &m, true);
   }
-  if (!st && hipStreamSynchronize(s) != hipSuccess) {
+  if (!st && hipStreamSynchronize(s) != hipSuccess) {  // the caller's P is read, and m known
     set_error("preparation of P failed");
     st = SGTSNE_ERR_HIP;
   }
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
-  (void)hipStreamSynchronize(b->stream);
-  if (srp != rp) { bfree(srp); bfree(scc); bfree(svv); }
-  bfree(rp);
-  bfree(cc);
-  bfree(vv);
+  // the admitted P is read by copies queued on the iteration stream
+  const hipStream_t fs = st ? s : b->stream;
+  if (srp != rp) {
+    (void)hipFreeAsync(srp, fs);
+    (void)hipFreeAsync(scc, fs);
+    (void)hipFreeAsync(svv, fs);
+    (void)hipFreeAsync(rp, s);
+    (void)hipFreeAsync(cc, s);
+    (void)hipFreeAsync(vv, s);
+  } else {
+    (void)hipFreeAsync(rp, fs);
+    (void)hipFreeAsync(cc, fs);
+    (void)hipFreeAsync(vv, fs);
+  }
   return st;
 }
 
@@ -747,14 +873,16 @@ int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int
   int32_t *Id = nullptr, *rp = nullptr, *cc = nullptr, *srp = nullptr, *scc = nullptr;
   float *vv = nullptr, *svv = nullptr;
   int64_t m = 0;
-  int st = balloc((void **)&Xf, nx * sizeof(float));
-  if (!st) st = balloc((void **)&Xd, nx * sizeof(double));
-  if (!st) st = balloc((void **)&Dd, (size_t)n * (k + 1) * sizeof(double));
-  if (!st) st = balloc((void **)&Id, (size_t)n * (k + 1) * sizeof(int32_t));
-  if (!st) st = balloc((void **)&rp, ((size_t)n + 1) * sizeof(int32_t));
-  if (!st) st = balloc((void **)&cc, (size_t)n * k * sizeof(int32_t));
-  if (!st) st = balloc((void **)&vv, (size_t)n * k * sizeof(float));
-  if (!st && hipMemcpyAsync(Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
+  // temporaries come from the stream-ordered pool (see sgtsne_batch_add)
+  SGT_HIP(hipMallocAsync((void **)&Xf, nx * sizeof(float), s));
+  SGT_HIP(hipMallocAsync((void **)&Xd, nx * sizeof(double), s));
+  SGT_HIP(hipMallocAsync((void **)&Dd, (size_t)n * (k + 1) * sizeof(double), s));
+  SGT_HIP(hipMallocAsync((void **)&Id, (size_t)n * (k + 1) * sizeof(int32_t), s));
+  SGT_HIP(hipMallocAsync((void **)&rp, ((size_t)n + 1) * sizeof(int32_t), s));
+  SGT_HIP(hipMallocAsync((void **)&cc, (size_t)n * k * sizeof(int32_t), s));
+  SGT_HIP(hipMallocAsync((void **)&vv, (size_t)n * k * sizeof(float), s));
+  int st = SGTSNE_OK;
+  if (hipMemcpyAsync(Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
     set_error("copy of X failed");
     st = SGTSNE_ERR_HIP;
   }
@@ -763,23 +891,19 @@ int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int
   if (!st) st = launch_perplexity(s, Dd, n, k, u, vv, nullptr);
   if (!st) st = launch_knn_csr(s, n, k, Id, rp, cc);
   if (!st && b->p.lambda != 1.0) st = launch_lambda_rescale(s, n, rp, vv, b->p.lambda, b->p.drop_leaf);
-  if (!st) st = gpu_symmetrize(s, n, (int64_t)n * k, rp, cc, vv, &srp, &scc, &svv, &m);
+  if (!st) st = gpu_symmetrize(s, n, (int64_t)n * k, rp, cc, vv, &srp, &scc, &svv, &m, true);
   if (!st && hipStreamSynchronize(s) != hipSuccess) {
     set_error("front end failed");
     st = SGTSNE_ERR_HIP;
   }
-  bfree(Xf);
-  bfree(Xd);
-  bfree(Dd);
-  bfree(Id);
-  bfree(rp);
-  bfree(cc);
-  bfree(vv);
+  void *tmp[] = {Xf, Xd, Dd, Id, rp, cc, vv};
+  for (void *q : tmp) (void)hipFreeAsync(q, s);
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
-  (void)hipStreamSynchronize(b->stream);
-  bfree(srp);
-  bfree(scc);
-  bfree(svv);
+  // the admitted P is read by copies queued on the iteration stream
+  const hipStream_t fs = st ? s : b->stream;
+  if (srp) (void)hipFreeAsync(srp, fs);
+  if (scc) (void)hipFreeAsync(scc, fs);
+  if (svv) (void)hipFreeAsync(svv, fs);
   return st;
 }
 
@@ -805,11 +929,31 @@ int sgtsne_batch_sync(sgtsne_batch *b) {
   return SGTSNE_OK;
 }
 
+int sgtsne_batch_mark(sgtsne_batch *b, int64_t *mark) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(mark != nullptr, SGTSNE_ERR_ARG, "mark is NULL");
+  SGT_HIP(hipEventRecord(b->marks[b->nmarks % kMarks], b->stream));
+  *mark = b->nmarks++;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_wait(sgtsne_batch *b, int64_t mark) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(mark >= 0 && mark < b->nmarks, SGTSNE_ERR_ARG, "no mark %lld", (long long)mark);
+  // the slot's event may have been recorded again for a later mark: waiting
+  // for that later point covers this one too
+  SGT_HIP(hipEventSynchronize(b->marks[mark % kMarks]));
+  return SGTSNE_OK;
+}
+
 int sgtsne_batch_job_status(sgtsne_batch *b, int job, int *status, int *iter) {
   SGT_TRY(check_job(b, job));
   int v[2] = {0, 0};
-  SGT_HIP(hipMemcpyAsync(v, &b->jobs_d[job].iter, 2 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  SGT_HIP(hipStreamSynchronize(b->stream));
+  const HostJob &h = b->hj[job];
+  const hipStream_t cs = h.done_marked ? b->copy : b->stream;  // a finished job's record no longer changes
+  if (h.done_marked) SGT_HIP(hipStreamWaitEvent(b->copy, h.done_ev, 0));
+  SGT_HIP(hipMemcpyAsync(v, &b->jobs_d[job].iter, 2 * sizeof(int), hipMemcpyDeviceToHost, cs));
+  SGT_HIP(hipStreamSynchronize(cs));
   if (iter) *iter = v[0];
   if (status) {
     const int s = v[1];
@@ -851,12 +995,13 @@ int sgtsne_batch_terms(sgtsne_batch *b, int job, float *Fattr, float *Frep, doub
 
 int sgtsne_batch_remove(sgtsne_batch *b, int job) {
   SGT_TRY(check_job(b, job));
-  SGT_HIP(hipStreamSynchronize(b->stream));
+  // everything below is stream-ordered after the queued iterations
   HostJob &h = b->hj[job];
   BatchJob empty{};
   empty.status = BJ_EMPTY;
-  SGT_HIP(hipMemcpyAsync(b->jobs_d + job, &empty, sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
+  SGT_TRY(upload(b, b->jobs_d + job, &empty, sizeof(BatchJob)));
   SGT_HIP(hipFreeAsync(h.gmem, b->stream));
+  if (h.done_ev) (void)hipEventDestroy(h.done_ev);
   b->grid_used -= h.gbytes;
   b->arena.erase(std::find(b->arena.begin(), b->arena.end(), job));
   const bool tail = h.off + h.n == b->used_points;

@@ -113,7 +113,8 @@ int launch_lambda_rescale(hipStream_t s, int n, const int32_t *rowptr, float *va
 int launch_rowids(hipStream_t s, int n, const int32_t *rowptr, int32_t *row);
 int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d,
                    const int32_t *col_d, const float *val_d, int32_t **rowptr_out,
-                   int32_t **col_out, float **val_out, int64_t *nnz_out);
+                   int32_t **col_out, float **val_out, int64_t *nnz_out,
+                   bool stream_ordered = false);
 int launch_build_ell(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
                      const float *val, int32_t *ecol, float *eval, int32_t *tail_cnt);
 int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
@@ -128,6 +129,6 @@ int launch_gather_points(hipStream_t s, int n, int ds, const int32_t *order, con
 int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
                        const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
                        int32_t *col2, float *val2,
-                       bool sort_rows = true);
+                       bool sort_rows = true, bool device_sort = false);
 
 }  // namespace sgt

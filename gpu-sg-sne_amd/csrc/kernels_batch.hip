@@ -105,35 +105,54 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
     }
     const bool in_lds = (int64_t)wcells * M <= (int64_t)lds_slots;
     __syncthreads();  // sbox is reset by the next item
+    // each point slot of each wave takes a contiguous run of the work unit's
+    // points; a lane adds its points' integer contributions in registers while
+    // they land on the same node (consecutive points mostly share a cell) and
+    // issues one atomic per run.  Integer sums are exact, so the grid is the
+    // same bits as with one atomic per point.
+    const int len = itm.end - itm.begin;
+    const int per_wave = (len + nwaves - 1) / nwaves, per_slot = (per_wave + PPW - 1) / PPW;
+    const int r0 = min(itm.end, itm.begin + wave * per_wave + slot * per_slot);
+    const int r1 = min(min(itm.end, itm.begin + (wave + 1) * per_wave), r0 + per_slot);
+    unsigned long long acc[M];
+    int cur = -1;
     if (in_lds) {
       for (int e = threadIdx.x; e < wcells * M; e += blockDim.x) win[e] = 0ull;
       __syncthreads();
-      for (int base = itm.begin + wave * PPW; base < itm.end; base += nwaves * PPW) {
-        const int i = base + slot;
-        if (i < itm.end) {
-          float y[D];
-          load_pt<D>(Y, i, y);
-          int f[D];
-          float w[D][4];
-          stencil<D>(g, y, f, w);
-          int loc;
-          float ww;
-          if constexpr (D == 1) {
-            loc = f[0] + a0 - wlo[0];
-            ww = w[0][a0];
-          } else if constexpr (D == 2) {
-            loc = (f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
-            ww = w[1][a1] * w[0][a0];
-          } else {
-            loc = ((f[2] + a2 - wlo[2]) * wd[1] + f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
-            ww = w[2][a2] * w[1][a1] * w[0][a0];
-          }
-          atomicAdd(&win[loc], (unsigned long long)llrint((double)ww * fs));
-#pragma unroll
-          for (int a = 0; a < D; ++a)
-            atomicAdd(&win[(a + 1) * wcells + loc],
-                      (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs));
+      for (int i = r0; i < r1; ++i) {
+        float y[D];
+        load_pt<D>(Y, i, y);
+        int f[D];
+        float w[D][4];
+        stencil<D>(g, y, f, w);
+        int loc;
+        float ww;
+        if constexpr (D == 1) {
+          loc = f[0] + a0 - wlo[0];
+          ww = w[0][a0];
+        } else if constexpr (D == 2) {
+          loc = (f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
+          ww = w[1][a1] * w[0][a0];
+        } else {
+          loc = ((f[2] + a2 - wlo[2]) * wd[1] + f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
+          ww = w[2][a2] * w[1][a1] * w[0][a0];
         }
+        if (loc != cur) {
+          if (cur >= 0) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) atomicAdd(&win[m * wcells + cur], acc[m]);
+          }
+          cur = loc;
+#pragma unroll
+          for (int m = 0; m < M; ++m) acc[m] = 0ull;
+        }
+        acc[0] += (unsigned long long)llrint((double)ww * fs);
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a + 1] += (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs);
+      }
+      if (cur >= 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) atomicAdd(&win[m * wcells + cur], acc[m]);
       }
       __syncthreads();
       for (int e = threadIdx.x; e < wcells; e += blockDim.x) {
@@ -157,20 +176,28 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
       }
       __syncthreads();  // the window is cleared by the next item
     } else {
-      for (int base = itm.begin + wave * PPW; base < itm.end; base += nwaves * PPW) {
-        const int i = base + slot;
-        if (i < itm.end) {
-          float y[D];
-          load_pt<D>(Y, i, y);
-          int gi;
-          float ww;
-          node_of_lane<D>(g, y, node, gi, ww);
-          atomicAdd(&V[gi], (unsigned long long)llrint((double)ww * fs));
+      for (int i = r0; i < r1; ++i) {
+        float y[D];
+        load_pt<D>(Y, i, y);
+        int gi;
+        float ww;
+        node_of_lane<D>(g, y, node, gi, ww);
+        if (gi != cur) {
+          if (cur >= 0) {
 #pragma unroll
-          for (int a = 0; a < D; ++a)
-            atomicAdd(&V[(int64_t)(a + 1) * G + gi],
-                      (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs));
+            for (int m = 0; m < M; ++m) atomicAdd(&V[(int64_t)m * G + cur], acc[m]);
+          }
+          cur = gi;
+#pragma unroll
+          for (int m = 0; m < M; ++m) acc[m] = 0ull;
         }
+        acc[0] += (unsigned long long)llrint((double)ww * fs);
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a + 1] += (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs);
+      }
+      if (cur >= 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) atomicAdd(&V[(int64_t)m * G + cur], acc[m]);
       }
     }
   }

@@ -807,6 +807,50 @@ __global__ void perm_fill_kernel(int n, const int32_t *__restrict__ perm, const 
   }
 }
 
+// Columns ascending within each row, in place, on the device alone: one
+// work-group per row sorts its (column, value) pairs with a bitonic sort in
+// LDS.  Rows longer than kRowSortMax keep their order (the order of a row
+// only changes which lanes gather neighbouring points, never the result's
+// definition), so nothing here needs the host.
+constexpr int kRowSortMax = 4096;
+
+__global__ __launch_bounds__(256) void row_sort_kernel(int n, const int32_t *__restrict__ rp, int32_t *__restrict__ col,
+                                                       float *__restrict__ val) {
+  __shared__ int32_t kc[kRowSortMax];
+  __shared__ float kv[kRowSortMax];
+  const int r = blockIdx.x;
+  if (r >= n) return;
+  const int b = rp[r], len = rp[r + 1] - b;
+  if (len < 2 || len > kRowSortMax) return;  // the whole work-group leaves together
+  int P = 2;
+  while (P < len) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += blockDim.x) {
+    kc[t] = t < len ? col[b + t] : INT_MAX;
+    kv[t] = t < len ? val[b + t] : 0.0f;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
+        const int i = 2 * stride * (t / stride) + (t % stride), j = i + stride;
+        const bool up = (i & size) == 0;
+        const int ci = kc[i], cj = kc[j];
+        if ((ci > cj) == up) {
+          kc[i] = cj;
+          kc[j] = ci;
+          const float v = kv[i];
+          kv[i] = kv[j];
+          kv[j] = v;
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = threadIdx.x; t < len; t += blockDim.x) {
+    col[b + t] = kc[t];
+    val[b + t] = kv[t];
+  }
+}
+
 // ---- spatial order of the embedding -------------------------------------------
 // Morton key of each point's position on a 2^B-per-axis lattice over the
 // bounding box (B = 16 / 10 / 32 bits per axis for d = 2 / 3 / 1).
@@ -991,7 +1035,7 @@ int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n)
 
 int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
                    const float *val_d, int32_t **rowptr_out, int32_t **col_out, float **val_out,
-                   int64_t *nnz_out) {
+                   int64_t *nnz_out, bool stream_ordered) {
   const int64_t m2 = 2 * nnz;
   SGT_CHECK(m2 < (int64_t)INT_MAX, SGTSNE_ERR_ARG, "2*nnz exceeds the int32 range");
   unsigned long long *k0 = nullptr, *k1 = nullptr, *uk = nullptr;
@@ -1032,9 +1076,17 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, c
   SGT_HIP(hipFreeAsync(tmp, s));
   int32_t *rp = nullptr, *cc = nullptr;
   float *vv = nullptr;
-  SGT_HIP(hipMalloc((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
-  SGT_HIP(hipMalloc((void **)&cc, sizeof(int32_t) * (size_t)(runs > 0 ? runs : 1)));
-  SGT_HIP(hipMalloc((void **)&vv, sizeof(float) * (size_t)(runs > 0 ? runs : 1)));
+  // the outputs come from the stream-ordered pool when the caller releases
+  // them with hipFreeAsync, else from hipMalloc (released with hipFree)
+  if (stream_ordered) {
+    SGT_HIP(hipMallocAsync((void **)&rp, sizeof(int32_t) * ((size_t)n + 1), s));
+    SGT_HIP(hipMallocAsync((void **)&cc, sizeof(int32_t) * (size_t)(runs > 0 ? runs : 1), s));
+    SGT_HIP(hipMallocAsync((void **)&vv, sizeof(float) * (size_t)(runs > 0 ? runs : 1), s));
+  } else {
+    SGT_HIP(hipMalloc((void **)&rp, sizeof(int32_t) * ((size_t)n + 1)));
+    SGT_HIP(hipMalloc((void **)&cc, sizeof(int32_t) * (size_t)(runs > 0 ? runs : 1)));
+    SGT_HIP(hipMalloc((void **)&vv, sizeof(float) * (size_t)(runs > 0 ? runs : 1)));
+  }
   if (runs > 0) {
     sym_finish_kernel<<<div_up(runs, 256), 256, 0, s>>>(runs, uk, uv, total, cc, vv);
     SGT_LAUNCH_CHECK();
@@ -1057,17 +1109,20 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, c
 
 int launch_permute_csr(hipStream_t s, int n, int64_t nnz, const int32_t *perm, const int32_t *iperm,
                        const int32_t *rp, const int32_t *col, const float *val, int32_t *rp2,
-                       int32_t *col2, float *val2, bool sort_rows) {
+                       int32_t *col2, float *val2, bool sort_rows, bool device_sort) {
   int32_t *len = nullptr;
   SGT_HIP(hipMallocAsync((void **)&len, sizeof(int32_t) * ((size_t)n + 1), s));
   perm_len_kernel<<<div_up(n, 256), 256, 0, s>>>(n, perm, rp, len);
   SGT_LAUNCH_CHECK();
   SGT_HIP(hipMemsetAsync(len + n, 0, sizeof(int32_t), s));
   SGT_TRY(device_exclusive_scan(s, len, rp2, n + 1));
-  if (!sort_rows) {
-    // rows keep the permuted order of their columns (no host round trip)
+  if (!sort_rows || device_sort) {
     perm_fill_kernel<<<div_up((int64_t)n * 64, 256), 256, 0, s>>>(n, perm, iperm, rp, col, val, rp2, col2, val2);
     SGT_LAUNCH_CHECK();
+    if (sort_rows) {  // in place, without the host (rows above kRowSortMax keep their order)
+      row_sort_kernel<<<n, 256, 0, s>>>(n, rp2, col2, val2);
+      SGT_LAUNCH_CHECK();
+    }
     SGT_HIP(hipFreeAsync(len, s));
     return SGTSNE_OK;
   }

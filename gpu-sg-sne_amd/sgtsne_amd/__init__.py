@@ -42,7 +42,8 @@ EXPORTED = [
     "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
     "sgtsne_ctx_tuning", "sgtsne_ctx_set_attractive", "sgtsne_ctx_reorders",
     "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
-    "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
+    "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_mark", "sgtsne_batch_wait",
+    "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
     "sgtsne_batch_set_graph", "sgtsne_copy_probe", "sgtsne_locality_order",
 ]
@@ -125,6 +126,8 @@ def lib():
                                               C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_batch_step": (C.c_int, [vp, C.c_int]),
         "sgtsne_batch_sync": (C.c_int, [vp]),
+        "sgtsne_batch_mark": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "sgtsne_batch_wait": (C.c_int, [vp, C.c_int64]),
         "sgtsne_batch_job_status": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_batch_get_Y": (C.c_int, [vp, C.c_int, f32p, C.c_int]),
         "sgtsne_batch_terms": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
@@ -431,6 +434,16 @@ class Batch:
 
     def sync(self):
         _check(lib().sgtsne_batch_sync(self._h))
+
+    def mark(self):
+        """A point on the iteration stream after everything queued so far."""
+        m = C.c_int64()
+        _check(lib().sgtsne_batch_mark(self._h, C.byref(m)))
+        return m.value
+
+    def wait(self, mark):
+        """Returns once the device has passed `mark`."""
+        _check(lib().sgtsne_batch_wait(self._h, int(mark)))
 
     def job_status(self, job):
         st, it = C.c_int(), C.c_int()

@@ -4,11 +4,17 @@ A request is a point set X [n][dim] and an embedding dimension d (2 or 3).
 The server keeps one batched engine per d (sgtsne_batch_*), so P, Y and the
 grids of every admitted job stay resident on the device.  Its loop:
 
-  1. queue the next `tick` iterations of every engine (asynchronous);
-  2. while the GPU runs them, admit queued requests whose HBM footprint fits:
-     the front end (exact kNN, perplexity, P + P^T) runs on the engine's
-     second stream, overlapping the queued iterations;
-  3. retire the jobs that reached max_iter (copy Y out, free their slots).
+  1. queue the next `tick` iterations of every engine (asynchronous), keeping
+     at most `depth` ticks in flight per engine: the loop waits for an
+     engine's oldest queued tick, never for its whole queue, so the device
+     always has work while the host admits and retires;
+  2. while the GPU runs them, admit queued requests whose HBM footprint fits,
+     3-D requests first (their iterations are the longest, so starting them
+     early shortens the makespan), then in arrival order: the front end (kNN,
+     perplexity, P + P^T) runs on the engine's second stream, overlapping the
+     queued iterations;
+  3. retire the jobs that reached max_iter: a finished job's Y is read as soon
+     as its own last iteration is done, then its slot is freed.
 
 A job's result does not depend on what else shares its engine (the engine
 guarantees bit-identical trajectories alone and in any mix), so serving
@@ -56,9 +62,9 @@ class Server:
     """hbm_bytes is split between the d=2 and d=3 engines in proportion to
     `share`; each engine admits jobs while their summed footprint fits."""
 
-    def __init__(self, hbm_bytes=48 << 30, max_jobs=256, tick=25, k=90, u=30.0, max_iter=1000,
-                 share=(0.5, 0.5), seed=0):
-        self.k, self.u, self.max_iter, self.tick = k, u, max_iter, tick
+    def __init__(self, hbm_bytes=48 << 30, max_jobs=256, tick=10, k=90, u=30.0, max_iter=1000,
+                 share=(0.5, 0.5), seed=0, depth=3):
+        self.k, self.u, self.max_iter, self.tick, self.depth = k, u, max_iter, tick, depth
         self.engines = {}
         self.budget = {}
         self.used = {}
@@ -73,6 +79,7 @@ class Server:
             self.used[d] = 0
         self.queue = collections.deque()
         self.active = {2: {}, 3: {}}
+        self.inflight = {2: collections.deque(), 3: collections.deque()}  # marks of queued ticks
         self.finished = []
 
     def close(self):
@@ -89,11 +96,12 @@ class Server:
         self.queue.append(req)
 
     def _admit(self, now_fn):
-        # first fit in arrival order: a large request at the head does not
-        # block smaller ones behind it
+        # first fit, 3-D requests first, then in arrival order: a large request
+        # at the head does not block smaller ones behind it
         keep = collections.deque()
-        while self.queue:
-            r = self.queue.popleft()
+        order = sorted(self.queue, key=lambda q: (-q.d, q.arrival, q.rid))
+        self.queue = collections.deque()
+        for r in order:
             fp = footprint_bytes(r.X.shape[0], r.d, self.k)
             if fp > self.budget[r.d]:
                 r.done, r.Y = now_fn(), None  # can never fit: rejected
@@ -118,12 +126,16 @@ class Server:
             r.X = None
             self.used[r.d] += fp
             self.active[r.d][job] = (r, fp)
-        self.queue = keep
+        self.queue = collections.deque(sorted(keep, key=lambda q: (q.arrival, q.rid)))
 
     def _advance(self):
         for d, act in self.active.items():
             if act:
-                self.engines[d].step(self.tick)
+                eng, marks = self.engines[d], self.inflight[d]
+                while len(marks) >= self.depth:
+                    eng.wait(marks.popleft())
+                eng.step(self.tick)
+                marks.append(eng.mark())
                 for r, _ in act.values():
                     r.t_left -= self.tick
 

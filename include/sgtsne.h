@@ -294,6 +294,14 @@ int sgtsne_batch_add_points(sgtsne_batch *batch, const float *X, int n, int dim,
 int sgtsne_batch_step(sgtsne_batch *batch, int iters);
 int sgtsne_batch_sync(sgtsne_batch *batch);
 
+/* Pacing of queued iterations: sgtsne_batch_mark records a point on the
+ * batch's iteration stream (after everything queued so far) and returns its
+ * number; sgtsne_batch_wait(batch, mark) returns once the device has passed
+ * it (or a later mark).  A server keeps a few steps in flight with these
+ * instead of draining the stream.  No counterpart in the reference. */
+int sgtsne_batch_mark(sgtsne_batch *batch, int64_t *mark);
+int sgtsne_batch_wait(sgtsne_batch *batch, int64_t mark);
+
 /* Waits for queued work, then reports SGTSNE_JOB_RUNNING / SGTSNE_JOB_DONE
  * or a negative error of the job and its iteration counter. */
 int sgtsne_batch_job_status(sgtsne_batch *batch, int job, int *status, int *iter);
