@@ -81,6 +81,12 @@ struct sgtsne_batch {
   // marks recorded on the iteration stream (pacing of queued iterations)
   hipEvent_t marks[kMarks] = {};
   int64_t nmarks = 0;
+  int64_t passed_upto = -1;  // the latest mark known to be passed
+  // scratch of the re-orders and compactions, reused by each: they all run
+  // on the iteration stream, in order, so one buffer serves every call and
+  // no allocation or release sits between queued iterations
+  char *ws = nullptr;
+  size_t ws_bytes = 0;
   int max_jobs = 0;
   int64_t max_points = 0, max_nnz = 0;
   size_t grid_budget = 0, grid_used = 0;
@@ -136,6 +142,32 @@ void bfree(T *&p) {
   if (p) (void)hipFree((void *)p);
   p = nullptr;
 }
+
+// Grows the iteration-stream scratch to at least `bytes` (rare: only a job
+// larger than any before it); queued work may still read the old buffer.
+int ws_reserve(sgtsne_batch *b, size_t bytes) {
+  if (bytes <= b->ws_bytes) return SGTSNE_OK;
+  SGT_HIP(hipStreamSynchronize(b->stream));
+  bfree(b->ws);
+  b->ws_bytes = 0;
+  const size_t cap = std::max(bytes + bytes / 2, (size_t)1 << 20);
+  SGT_TRY(balloc((void **)&b->ws, cap));
+  b->ws_bytes = cap;
+  return SGTSNE_OK;
+}
+
+// Consecutive 256-byte-aligned pieces of one buffer; with a null base it
+// only adds up the size the same pieces need.
+struct Carve {
+  char *base;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t count) {
+    T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+    off += (sizeof(T) * std::max<size_t>(count, 1) + 255) & ~(size_t)255;
+    return p;
+  }
+};
 
 double bdefault_h(int d) { return d == 1 ? 0.5 : d == 2 ? 0.7 : 1.2; }
 
@@ -405,8 +437,8 @@ int compact(sgtsne_batch *b) {
     const HostJob &h = b->hj[j];
     need = std::max(need, std::max((size_t)h.n * b->ds * sizeof(float), (size_t)(h.nnz + h.n + 1) * sizeof(int32_t)));
   }
-  void *tmp = nullptr;
-  SGT_HIP(hipMallocAsync(&tmp, std::max<size_t>(need, 16), b->stream));
+  SGT_TRY(ws_reserve(b, need));
+  void *tmp = b->ws;
   for (int j : b->arena) {
     HostJob &h = b->hj[j];
     if (h.off != cur_p || h.nnz_off != cur_z) {
@@ -432,7 +464,6 @@ int compact(sgtsne_batch *b) {
     cur_p += h.n;
     cur_z += h.nnz;
   }
-  SGT_HIP(hipFreeAsync(tmp, b->stream));
   b->used_points = cur_p;
   b->used_nnz = cur_z;
   return rebuild_items(b);
@@ -445,29 +476,34 @@ int reorder_job(sgtsne_batch *b, int j) {
   const int n = h.n;
   const int64_t nnz = h.nnz;
   const hipStream_t s = b->stream;
-  uint32_t *k0 = nullptr, *k1 = nullptr;
-  int32_t *i0 = nullptr, *order = nullptr, *inv = nullptr, *rpl = nullptr, *coll = nullptr, *rp2 = nullptr,
-          *col2 = nullptr, *itmp = nullptr;
-  float *val2 = nullptr, *ftmp = nullptr;
-  SGT_HIP(hipMallocAsync((void **)&k0, sizeof(uint32_t) * n, s));
-  SGT_HIP(hipMallocAsync((void **)&k1, sizeof(uint32_t) * n, s));
-  SGT_HIP(hipMallocAsync((void **)&i0, sizeof(int32_t) * n, s));
-  SGT_HIP(hipMallocAsync((void **)&order, sizeof(int32_t) * n, s));
-  SGT_HIP(hipMallocAsync((void **)&inv, sizeof(int32_t) * n, s));
-  SGT_TRY(launch_bmorton(s, b->d, b->jobs_d, j, h.off, n, b->Y, k0, i0));
   size_t tb = 0;
-  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, order, n, 0, 32, s));
-  void *tmp = nullptr;
-  SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (int32_t *)nullptr, (int32_t *)nullptr, n, 0, 32, s));
+  uint32_t *k0, *k1;
+  int32_t *i0, *order, *inv, *rpl, *coll, *rp2, *col2, *itmp;
+  float *val2, *ftmp;
+  void *tmp;
+  for (int pass = 0; pass < 2; ++pass) {  // sizes, then the pieces of the scratch
+    Carve c{pass ? b->ws : nullptr};
+    k0 = c.take<uint32_t>(n);
+    k1 = c.take<uint32_t>(n);
+    i0 = c.take<int32_t>(n);
+    order = c.take<int32_t>(n);
+    inv = c.take<int32_t>(n);
+    tmp = c.take<char>(tb);
+    rpl = c.take<int32_t>((size_t)n + 1);
+    rp2 = c.take<int32_t>((size_t)n + 1);
+    coll = c.take<int32_t>(nnz);
+    col2 = c.take<int32_t>(nnz);
+    val2 = c.take<float>(nnz);
+    ftmp = c.take<float>((size_t)n * b->ds);
+    itmp = c.take<int32_t>(n);
+    if (pass == 0) SGT_TRY(ws_reserve(b, c.off));
+  }
+  SGT_TRY(launch_bmorton(s, b->d, b->jobs_d, j, h.off, n, b->Y, k0, i0));
   SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, order, n, 0, 32, s));
-  SGT_HIP(hipFreeAsync(tmp, s));
   SGT_TRY(launch_invert(s, n, order, inv));
   // the job's rows as a local CSR, permuted, written back with offsets
-  SGT_HIP(hipMallocAsync((void **)&rpl, sizeof(int32_t) * (n + 1), s));
-  SGT_HIP(hipMallocAsync((void **)&coll, sizeof(int32_t) * std::max<int64_t>(nnz, 1), s));
-  SGT_HIP(hipMallocAsync((void **)&rp2, sizeof(int32_t) * (n + 1), s));
-  SGT_HIP(hipMallocAsync((void **)&col2, sizeof(int32_t) * std::max<int64_t>(nnz, 1), s));
-  SGT_HIP(hipMallocAsync((void **)&val2, sizeof(float) * std::max<int64_t>(nnz, 1), s));
   SGT_TRY(launch_offset_copy_i32(s, b->rowptr + h.off, rpl, n + 1, (int32_t)(-h.nnz_off)));
   SGT_TRY(launch_offset_copy_i32(s, b->col + h.nnz_off, coll, nnz, -h.off));
   // rows re-sorted on the device alone: a host round trip here would wait
@@ -478,18 +514,14 @@ int reorder_job(sgtsne_batch *b, int j) {
   if (nnz > 0)
     SGT_HIP(hipMemcpyAsync(b->val + h.nnz_off, val2, sizeof(float) * nnz, hipMemcpyDeviceToDevice, s));
   // point state
-  SGT_HIP(hipMallocAsync((void **)&ftmp, sizeof(float) * (size_t)n * b->ds, s));
   float *arr[3] = {b->Y, b->uY, b->gains};
   for (float *a : arr) {
     SGT_TRY(launch_gather_points(s, n, b->ds, order, a + (size_t)h.off * b->ds, ftmp));
     SGT_HIP(hipMemcpyAsync(a + (size_t)h.off * b->ds, ftmp, sizeof(float) * (size_t)n * b->ds,
                            hipMemcpyDeviceToDevice, s));
   }
-  SGT_HIP(hipMallocAsync((void **)&itmp, sizeof(int32_t) * n, s));
   SGT_TRY(launch_gather_i32(s, n, order, b->perm + h.off, itmp));
   SGT_HIP(hipMemcpyAsync(b->perm + h.off, itmp, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
-  void *fr[] = {k0, k1, i0, order, inv, rpl, coll, rp2, col2, val2, ftmp, itmp};
-  for (void *q : fr) SGT_HIP(hipFreeAsync(q, s));
   h.reorders++;
   return SGTSNE_OK;
 }
@@ -661,6 +693,8 @@ void destroy_all(sgtsne_batch *b) {
   bfree(b->prefix_d);
   bfree(b->tw);
   bfree(b->scal_d);
+  bfree(b->ws);
+  b->ws_bytes = 0;
   if (b->prep) (void)hipStreamSynchronize(b->prep);
   if (b->copy) (void)hipStreamSynchronize(b->copy);
   for (auto &h : b->hj)
@@ -940,9 +974,32 @@ int sgtsne_batch_mark(sgtsne_batch *b, int64_t *mark) {
 int sgtsne_batch_wait(sgtsne_batch *b, int64_t mark) {
   SGT_TRY(check_batch(b));
   SGT_CHECK(mark >= 0 && mark < b->nmarks, SGTSNE_ERR_ARG, "no mark %lld", (long long)mark);
+  if (mark <= b->passed_upto) return SGTSNE_OK;
   // the slot's event may have been recorded again for a later mark: waiting
   // for that later point covers this one too
+  const int64_t slot_mark = b->nmarks - 1 - ((b->nmarks - 1 - mark) % kMarks);
   SGT_HIP(hipEventSynchronize(b->marks[mark % kMarks]));
+  b->passed_upto = std::max(b->passed_upto, slot_mark);
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_passed(sgtsne_batch *b, int64_t mark, int *passed) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(mark >= 0 && mark < b->nmarks && passed, SGTSNE_ERR_ARG, "no mark %lld", (long long)mark);
+  if (mark <= b->passed_upto) {
+    *passed = 1;
+    return SGTSNE_OK;
+  }
+  // the slot holds this mark or the latest later one recorded into it
+  const int64_t slot_mark = b->nmarks - 1 - ((b->nmarks - 1 - mark) % kMarks);
+  const hipError_t e = hipEventQuery(b->marks[mark % kMarks]);
+  if (e == hipErrorNotReady) {
+    *passed = 0;
+    return SGTSNE_OK;
+  }
+  SGT_HIP(e);
+  b->passed_upto = std::max(b->passed_upto, slot_mark);
+  *passed = 1;
   return SGTSNE_OK;
 }
 

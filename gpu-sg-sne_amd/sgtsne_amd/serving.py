@@ -4,17 +4,20 @@ A request is a point set X [n][dim] and an embedding dimension d (2 or 3).
 The server keeps one batched engine per d (sgtsne_batch_*), so P, Y and the
 grids of every admitted job stay resident on the device.  Its loop:
 
-  1. queue the next `tick` iterations of every engine (asynchronous), keeping
-     at most `depth` ticks in flight per engine: the loop waits for an
-     engine's oldest queued tick, never for its whole queue, so the device
-     always has work while the host admits and retires;
-  2. while the GPU runs them, admit queued requests whose HBM footprint fits,
-     3-D requests first (their iterations are the longest, so starting them
-     early shortens the makespan), then in arrival order: the front end (kNN,
-     perplexity, P + P^T) runs on the engine's second stream, overlapping the
-     queued iterations;
-  3. retire the jobs that reached max_iter: a finished job's Y is read as soon
-     as its own last iteration is done, then its slot is freed.
+  1. queue the next `tick` iterations of every engine (asynchronous) that has
+     fewer than `depth` ticks in flight (checked without blocking), so the
+     device always has work while the host admits and retires;
+  2. while the GPU runs them, admit one queued request whose HBM footprint
+     fits, 3-D requests first (their iterations are the longest, so starting
+     them early shortens the makespan), then in arrival order: the front end
+     (kNN, perplexity, P + P^T) runs on the engine's second stream,
+     overlapping the queued iterations, and the loop returns to step 1
+     before the next admission;
+  3. retire the jobs whose last iteration the device has passed (checked
+     without blocking, so a job still queued holds up neither engine): its Y
+     is read, then its slot is freed.  With nothing to do (every engine at
+     `depth` queued ticks, nothing admissible, nothing finished) the host
+     sleeps `poll_s` before looking again.
 
 A job's result does not depend on what else shares its engine (the engine
 guarantees bit-identical trajectories alone and in any mix), so serving
@@ -48,7 +51,8 @@ def footprint_bytes(n, d, k=90):
 
 
 class Request:
-    __slots__ = ("rid", "X", "d", "arrival", "seed", "admitted", "done", "Y", "engine", "job", "t_left")
+    __slots__ = ("rid", "X", "d", "arrival", "seed", "admitted", "done", "Y", "engine", "job", "t_left",
+                 "end_mark")
 
     def __init__(self, rid, X, d, arrival, seed=0):
         self.rid, self.X, self.d, self.arrival, self.seed = rid, X, d, arrival, seed
@@ -56,6 +60,7 @@ class Request:
         self.Y = None
         self.engine = self.job = None
         self.t_left = 0
+        self.end_mark = None
 
 
 class Server:
@@ -63,8 +68,9 @@ class Server:
     `share`; each engine admits jobs while their summed footprint fits."""
 
     def __init__(self, hbm_bytes=48 << 30, max_jobs=256, tick=10, k=90, u=30.0, max_iter=1000,
-                 share=(0.5, 0.5), seed=0, depth=3):
+                 share=(0.5, 0.5), seed=0, depth=3, poll_s=2e-4, engine_factory=None):
         self.k, self.u, self.max_iter, self.tick, self.depth = k, u, max_iter, tick, depth
+        self.poll_s = poll_s
         self.engines = {}
         self.budget = {}
         self.used = {}
@@ -73,8 +79,11 @@ class Server:
             nnz = min(int(0.6 * b) // 8, 2_000_000_000)   # P_sym: 8 B per nonzero
             pts = max(1, nnz // k)                         # >= the points of any mix (>= k nonzeros per row)
             grid_budget = b  # the footprint check below is the binding limit
-            self.engines[d] = Batch(tsneparams(d=d, maxIter=max_iter, seed=seed), max_jobs, pts, nnz,
-                                    grid_budget)
+            params = tsneparams(d=d, maxIter=max_iter, seed=seed)
+            if engine_factory is None:
+                self.engines[d] = Batch(params, max_jobs, pts, nnz, grid_budget)
+            else:  # any object with Batch's add_points/step/mark/passed/job_status/get_Y/remove
+                self.engines[d] = engine_factory(params, max_jobs, pts, nnz, grid_budget)
             self.budget[d] = b
             self.used[d] = 0
         self.queue = collections.deque()
@@ -95,13 +104,19 @@ class Server:
     def submit(self, req):
         self.queue.append(req)
 
-    def _admit(self, now_fn):
+    def _admit(self, now_fn, limit=None):
         # first fit, 3-D requests first, then in arrival order: a large request
-        # at the head does not block smaller ones behind it
+        # at the head does not block smaller ones behind it.  At most `limit`
+        # admissions per call: each runs a front end on the host's time, and
+        # the loop tops the engines up between them.
+        admitted = 0
         keep = collections.deque()
         order = sorted(self.queue, key=lambda q: (-q.d, q.arrival, q.rid))
         self.queue = collections.deque()
-        for r in order:
+        for i, r in enumerate(order):
+            if limit is not None and admitted >= limit:
+                keep.extend(order[i:])
+                break
             fp = footprint_bytes(r.X.shape[0], r.d, self.k)
             if fp > self.budget[r.d]:
                 r.done, r.Y = now_fn(), None  # can never fit: rejected
@@ -126,30 +141,49 @@ class Server:
             r.X = None
             self.used[r.d] += fp
             self.active[r.d][job] = (r, fp)
+            admitted += 1
         self.queue = collections.deque(sorted(keep, key=lambda q: (q.arrival, q.rid)))
+        return admitted
 
     def _advance(self):
+        """Queues a tick on every engine whose jobs have iterations left;
+        returns whether any tick was queued."""
+        stepped = False
         for d, act in self.active.items():
-            if act:
-                eng, marks = self.engines[d], self.inflight[d]
-                while len(marks) >= self.depth:
-                    eng.wait(marks.popleft())
-                eng.step(self.tick)
-                marks.append(eng.mark())
-                for r, _ in act.values():
-                    r.t_left -= self.tick
+            live = [r for r, _ in act.values() if r.t_left > 0]
+            if not live:
+                continue
+            eng, marks = self.engines[d], self.inflight[d]
+            while marks and eng.passed(marks[0]):
+                marks.popleft()
+            if len(marks) >= self.depth:
+                continue  # enough queued: this engine is topped up on a later pass
+            eng.step(self.tick)
+            m = eng.mark()
+            marks.append(m)
+            for r in live:
+                r.t_left -= self.tick
+                if r.t_left <= 0:
+                    r.end_mark = m  # its last iteration is queued before this mark
+            stepped = True
+        return stepped
 
     def _retire(self, now_fn):
-        for d, act in self.active.items():
-            for job in [j for j, (r, _) in act.items() if r.t_left <= 0]:
-                r, fp = act.pop(job)
-                b = self.engines[d]
-                st, it = b.job_status(job)
-                r.Y = b.get_Y(job) if st >= 0 else None
-                r.done = now_fn()
-                b.remove(job)
-                self.used[d] -= fp
-                self.finished.append(r)
+        """Retires the jobs whose last iteration the device has passed (a
+        non-blocking check, so a job still queued holds up neither engine);
+        returns how many."""
+        due = [(d, job) for d, act in self.active.items() for job, (r, _) in act.items() if r.t_left <= 0]
+        ready = [(d, job) for d, job in due if self.engines[d].passed(self.active[d][job][0].end_mark)]
+        for d, job in ready:
+            r, fp = self.active[d].pop(job)
+            b = self.engines[d]
+            st, it = b.job_status(job)
+            r.Y = b.get_Y(job) if st >= 0 else None
+            r.done = now_fn()
+            b.remove(job)
+            self.used[d] -= fp
+            self.finished.append(r)
+        return len(ready)
 
     def busy(self):
         return bool(self.queue) or any(self.active[d] for d in self.active)
@@ -170,9 +204,11 @@ class Server:
             if not self.busy():
                 time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
                 continue
-            self._advance()       # queued on the GPU; returns at once
-            self._admit(now)      # front ends overlap the queued iterations
-            self._retire(now)
+            stepped = self._advance()            # queued on the GPU; returns at once
+            admitted = self._admit(now, limit=1)  # a front end overlaps the queued iterations
+            retired = self._retire(now)
+            if not (stepped or admitted or retired):
+                time.sleep(self.poll_s)  # every engine has `depth` ticks queued
         return sorted(self.finished, key=lambda r: r.rid)
 
 

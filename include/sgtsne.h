@@ -301,6 +301,9 @@ int sgtsne_batch_sync(sgtsne_batch *batch);
  * instead of draining the stream.  No counterpart in the reference. */
 int sgtsne_batch_mark(sgtsne_batch *batch, int64_t *mark);
 int sgtsne_batch_wait(sgtsne_batch *batch, int64_t mark);
+/* Non-blocking form of sgtsne_batch_wait: *passed = 1 once the device has
+ * passed the mark, else 0.  No counterpart in the reference. */
+int sgtsne_batch_passed(sgtsne_batch *batch, int64_t mark, int *passed);
 
 /* Waits for queued work, then reports SGTSNE_JOB_RUNNING / SGTSNE_JOB_DONE
  * or a negative error of the job and its iteration counter. */

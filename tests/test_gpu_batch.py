@@ -205,3 +205,25 @@ def test_batch_numeric_error_is_reported(sg, O):
         st, it = B.job_status(good)
         assert st == sg.JOB_RUNNING and it == 4
         assert np.all(np.isfinite(B.get_Y(good)))
+
+
+def test_batch_marks_wait_and_passed(sg, O):
+    # marks are points on the iteration stream: once the latest one is waited
+    # for, every earlier one (including those whose event slot was reused by
+    # a later mark) reads as passed; an unknown mark is an argument error
+    S, _ = _job(sg, O, 1500, 51)
+    with _batch(sg, max_jobs=2) as B:
+        B.add(S, symmetrize=False)
+        marks = []
+        for _ in range(40):
+            B.step(2)
+            marks.append(B.mark())
+        assert marks == list(range(marks[0], marks[0] + 40))
+        B.wait(marks[-1])
+        assert all(B.passed(m) for m in marks)
+        B.step(5)
+        m = B.mark()
+        B.sync()
+        assert B.passed(m)
+        with pytest.raises(sg.SgtsneError):
+            B.passed(m + 1)

@@ -1,0 +1,128 @@
+"""The serving loop's host logic (sgtsne_amd.serving.Server) on the CPU,
+against a stand-in engine that models a device queue: queued batch
+iterations run a few at a time, each time the host looks at the device.
+
+Checks: every request finishes; a job is retired only after the device has
+run all its iterations (one admission iteration + max_iter); no engine ever
+has more than `depth` ticks queued; 3-D requests are admitted ahead of 2-D
+ones waiting with them; a request whose footprint can never fit is rejected
+at once; at most one admission per loop pass."""
+import numpy as np
+import pytest
+
+from sgtsne_amd import serving
+
+
+class FakeEngine:
+    """Batch's serving surface over a modelled queue of batch iterations."""
+
+    def __init__(self, d, log, speed):
+        self.d, self.log, self.speed = d, log, speed
+        self.host_iter = 0     # batch iterations queued
+        self.dev_iter = 0      # batch iterations the device has run
+        self.mark_at = []      # host_iter at each mark
+        self.jobs = {}
+        self.next_job = 0
+        self.max_queued = 0
+
+    def _run(self):
+        self.dev_iter = min(self.host_iter, self.dev_iter + self.speed)
+
+    def add_points(self, X, k, u, y_in, max_iter):
+        self._run()
+        job = self.next_job
+        self.next_job += 1
+        self.jobs[job] = (self.host_iter, max_iter, X.shape[0])
+        self.log.append(("admit", self.d, X.shape[0]))
+        return job
+
+    def step(self, iters):
+        self.host_iter += iters
+        self.max_queued = max(self.max_queued, self.host_iter - self.dev_iter)
+
+    def mark(self):
+        self.mark_at.append(self.host_iter)
+        return len(self.mark_at) - 1
+
+    def passed(self, m):
+        self._run()
+        return self.dev_iter >= self.mark_at[m]
+
+    def wait(self, m):
+        self.dev_iter = max(self.dev_iter, self.mark_at[m])
+
+    def job_status(self, job):
+        start, max_iter, _ = self.jobs[job]
+        done = self.dev_iter - start
+        assert done >= max_iter + 1, "retired before its last iteration ran"
+        return 1, max_iter
+
+    def get_Y(self, job):
+        return np.zeros((self.jobs[job][2], self.d), np.float32)
+
+    def remove(self, job):
+        del self.jobs[job]
+        self.log.append(("remove", self.d))
+
+    def close(self):
+        pass
+
+
+def _server(log, speed=7, **kw):
+    engines = {}
+
+    def factory(params, max_jobs, pts, nnz, grid_budget):
+        e = FakeEngine(params.d, log, speed)
+        engines[params.d] = e
+        return e
+
+    args = dict(hbm_bytes=8 << 30, tick=10, k=30, max_iter=60, depth=3, poll_s=0.0, engine_factory=factory)
+    args.update(kw)
+    return serving.Server(**args), engines
+
+
+def test_server_finishes_every_request_after_its_iterations():
+    log = []
+    srv, eng = _server(log)
+    arrivals = [(0.0005 * i, 500 + 37 * i, 2 + (i % 2), i) for i in range(12)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 12 and all(r.Y is not None for r in got)
+    assert [r.rid for r in got] == list(range(12))
+    for e in eng.values():
+        assert e.max_queued <= 3 * 10
+        assert not e.jobs
+    assert all(r.admitted >= r.arrival and r.done >= r.admitted for r in got)
+
+
+def test_server_admits_3d_first_one_per_pass_and_rejects_oversize():
+    log = []
+    srv, eng = _server(log, hbm_bytes=1 << 30)
+    too_big = 2_000_000  # its footprint exceeds the engine's whole share
+    assert serving.footprint_bytes(too_big, 2, k=30) > srv.budget[2]
+    for rid, (n, d) in enumerate([(400, 2), (300, 2), (too_big, 2), (500, 3), (600, 3)]):
+        srv.submit(serving.Request(rid, np.zeros((n, 4), np.float32), d, 0.0, rid))
+    assert srv._admit(lambda: 0.0, limit=1) == 1
+    assert log == [("admit", 3, 500)]
+    assert srv._admit(lambda: 0.0, limit=1) == 1
+    assert log[-1] == ("admit", 3, 600)
+    srv._admit(lambda: 0.0)  # no limit: the rest that fit, in arrival order
+    assert [x for x in log if x[0] == "admit"][2:] == [("admit", 2, 400), ("admit", 2, 300)]
+    rejected = [r for r in srv.finished if r.Y is None]
+    assert len(rejected) == 1 and rejected[0].rid == 2
+    assert not srv.queue
+    srv.close()
+
+
+@pytest.mark.parametrize("speed", [1, 50])
+def test_server_depth_bounds_queued_ticks(speed):
+    # a slow device (1 iteration per look) and a fast one: never more than
+    # `depth` ticks queued, and the fast one never leaves a finished job waiting
+    log = []
+    srv, eng = _server(log, speed=speed, depth=2, max_iter=40)
+    arrivals = [(0.0, 800, 2, 0), (0.0, 900, 3, 1), (0.0002, 700, 2, 2)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 3
+    for e in eng.values():
+        assert e.max_queued <= 2 * 10
