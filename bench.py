@@ -184,6 +184,32 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True):
     return out
 
 
+def engine_leg(sg, d, k=90, u=30.0, graph=True, warm=801, timed=190):
+    """One batched engine holding every d-dimensional job of the served
+    request stream at once: front ends, `warm` batch iterations, then `timed`
+    ones; ms per batch iteration and the jobs' grid sides at the end."""
+    from sgtsne_amd import synth
+
+    arr = [a for a in synth.serving_requests(32, 40.0) if a[2] == d]
+    with sg.Batch(sg.tsneparams(d=d), 32, 2_000_000, 300_000_000, 24 << 30) as B:
+        B.set_graph(graph)
+        ids = []
+        for _, n, _, seed in arr:
+            X, _ = synth.gaussian_job(n, seed=seed)
+            ids.append(B.add_points(X, k=k, u=u, y_in=synth.random_init(n, d, seed=seed), max_iter=1000))
+        B.step(warm)
+        B.sync()
+        t0 = time.perf_counter()
+        B.step(timed)
+        B.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / timed
+        ngs = sorted(B.terms(j)[3] for j in ids)
+    res = {"d": d, "jobs": len(ids), "points": sum(a[1] for a in arr), "ms_per_batch_iteration": round(ms, 4),
+           "iterations": [warm, timed], "ng": ngs, "graph": graph}
+    log(f"[bench] engine: {res}")
+    return res
+
+
 def knn_leg(sg, n=200_000, k=90):
     """Front-end kNN of one 50-D Gaussian-mixture job: the filtered method
     against the exact FP64 one (rows must be identical).  Times include the
@@ -377,7 +403,8 @@ def main():
                     help="run the FP64 oracle for all iterations from the same P_sym and start; compare final KL")
     ap.add_argument("--legs", default="main,batched,served",
                     help="comma list of main (N=1.3M ms/iter), batched (256 jobs), served (request stream), "
-                         "knn (front-end kNN methods at N=200k; not in the default list)")
+                         "knn (front-end kNN methods at N=200k), engine (one batched engine holding the "
+                         "served stream's jobs of --engine-d); knn and engine are not in the default list")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-sample", type=int, default=16)
@@ -385,6 +412,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-tick", type=int, default=10)
+    ap.add_argument("--engine-d", type=int, default=3,
+                    help="engine leg: the served stream's jobs of this d in one batch")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched and served engines launch each kernel instead of replaying a graph")
     args = ap.parse_args()
@@ -413,6 +442,7 @@ def main():
     batched = batched_leg(sg, args.batch_jobs, args.batch_sample, graph=g) if "batched" in legs else None
     served = served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g) if "served" in legs else None
     knn = knn_leg(sg) if "knn" in legs else None
+    engine = engine_leg(sg, args.engine_d, graph=g) if "engine" in legs else None
 
     value = m.get("value")
     out = {
@@ -440,6 +470,7 @@ def main():
         "batched": batched,
         "served": served,
         "knn": knn,
+        "engine": engine,
     }
     print(json.dumps(out), flush=True)
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -82,6 +83,9 @@ struct sgtsne_batch {
   hipEvent_t marks[kMarks] = {};
   int64_t nmarks = 0;
   int64_t passed_upto = -1;  // the latest mark known to be passed
+  // guards the batch's host state: every entry point holds it (the front end
+  // of sgtsne_batch_add / _add_points only around the admission itself)
+  std::mutex mu;
   // scratch of the re-orders and compactions, reused by each: they all run
   // on the iteration stream, in order, so one buffer serves every call and
   // no allocation or release sits between queued iterations
@@ -877,6 +881,7 @@ int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr,
     set_error("preparation of P failed");
     st = SGTSNE_ERR_HIP;
   }
+  std::lock_guard<std::mutex> lock(b->mu);
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
   // the admitted P is read by copies queued on the iteration stream
   const hipStream_t fs = st ? s : b->stream;
@@ -932,6 +937,7 @@ int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int
   }
   void *tmp[] = {Xf, Xd, Dd, Id, rp, cc, vv};
   for (void *q : tmp) (void)hipFreeAsync(q, s);
+  std::lock_guard<std::mutex> lock(b->mu);
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
   // the admitted P is read by copies queued on the iteration stream
   const hipStream_t fs = st ? s : b->stream;
@@ -943,6 +949,7 @@ int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int
 
 int sgtsne_batch_step(sgtsne_batch *b, int iters) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_CHECK(iters >= 0, SGTSNE_ERR_ARG, "iters < 0");
   if (b->arena.empty()) return SGTSNE_OK;
   if (b->tuned_points == 0 || b->used_points > 2 * b->tuned_points || 2 * b->used_points < b->tuned_points)
@@ -959,12 +966,14 @@ int sgtsne_batch_step(sgtsne_batch *b, int iters) {
 
 int sgtsne_batch_sync(sgtsne_batch *b) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_HIP(hipStreamSynchronize(b->stream));
   return SGTSNE_OK;
 }
 
 int sgtsne_batch_mark(sgtsne_batch *b, int64_t *mark) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_CHECK(mark != nullptr, SGTSNE_ERR_ARG, "mark is NULL");
   SGT_HIP(hipEventRecord(b->marks[b->nmarks % kMarks], b->stream));
   *mark = b->nmarks++;
@@ -973,6 +982,7 @@ int sgtsne_batch_mark(sgtsne_batch *b, int64_t *mark) {
 
 int sgtsne_batch_wait(sgtsne_batch *b, int64_t mark) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_CHECK(mark >= 0 && mark < b->nmarks, SGTSNE_ERR_ARG, "no mark %lld", (long long)mark);
   if (mark <= b->passed_upto) return SGTSNE_OK;
   // the slot's event may have been recorded again for a later mark: waiting
@@ -985,6 +995,7 @@ int sgtsne_batch_wait(sgtsne_batch *b, int64_t mark) {
 
 int sgtsne_batch_passed(sgtsne_batch *b, int64_t mark, int *passed) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_CHECK(mark >= 0 && mark < b->nmarks && passed, SGTSNE_ERR_ARG, "no mark %lld", (long long)mark);
   if (mark <= b->passed_upto) {
     *passed = 1;
@@ -1004,6 +1015,8 @@ int sgtsne_batch_passed(sgtsne_batch *b, int64_t mark, int *passed) {
 }
 
 int sgtsne_batch_job_status(sgtsne_batch *b, int job, int *status, int *iter) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_TRY(check_job(b, job));
   int v[2] = {0, 0};
   const HostJob &h = b->hj[job];
@@ -1022,7 +1035,19 @@ int sgtsne_batch_job_status(sgtsne_batch *b, int job, int *status, int *iter) {
   return SGTSNE_OK;
 }
 
+int sgtsne_batch_job_left(sgtsne_batch *b, int job, int64_t *left) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
+  SGT_TRY(check_job(b, job));
+  SGT_CHECK(left != nullptr, SGTSNE_ERR_ARG, "left is NULL");
+  const HostJob &h = b->hj[job];
+  *left = std::max<int64_t>(0, (int64_t)h.max_iter + 1 - h.t);
+  return SGTSNE_OK;
+}
+
 int sgtsne_batch_get_Y(sgtsne_batch *b, int job, float *y, int layout) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_TRY(check_job(b, job));
   SGT_CHECK(y != nullptr && (layout == SGTSNE_LAYOUT_SOA || layout == SGTSNE_LAYOUT_AOS), SGTSNE_ERR_ARG,
             "bad arguments");
@@ -1030,6 +1055,8 @@ int sgtsne_batch_get_Y(sgtsne_batch *b, int job, float *y, int layout) {
 }
 
 int sgtsne_batch_terms(sgtsne_batch *b, int job, float *Fattr, float *Frep, double *Z, int *ng, int layout) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_TRY(check_job(b, job));
   SGT_CHECK(layout == SGTSNE_LAYOUT_SOA || layout == SGTSNE_LAYOUT_AOS, SGTSNE_ERR_ARG, "bad layout");
   int st_iter[2] = {0, 0};
@@ -1051,6 +1078,8 @@ int sgtsne_batch_terms(sgtsne_batch *b, int job, float *Fattr, float *Frep, doub
 }
 
 int sgtsne_batch_remove(sgtsne_batch *b, int job) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_TRY(check_job(b, job));
   // everything below is stream-ordered after the queued iterations
   HostJob &h = b->hj[job];
@@ -1079,6 +1108,7 @@ int sgtsne_batch_remove(sgtsne_batch *b, int job) {
 int sgtsne_batch_info(sgtsne_batch *b, int *jobs, int64_t *points, int64_t *nnz, int64_t *grid_bytes,
                       int64_t *graph_replays) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   if (jobs) *jobs = (int)b->arena.size();
   if (points) *points = b->used_points;
   if (nnz) *nnz = b->used_nnz;
@@ -1089,6 +1119,7 @@ int sgtsne_batch_info(sgtsne_batch *b, int *jobs, int64_t *points, int64_t *nnz,
 
 int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_block, int *window, int *use_graph) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   if (window) *window = b->spmv_win;
   if (group) *group = b->spmv_group;
   if (unroll) *unroll = b->spmv_unroll;
@@ -1099,6 +1130,7 @@ int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_
 
 int sgtsne_batch_set_graph(sgtsne_batch *b, int on) {
   SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
   SGT_HIP(hipStreamSynchronize(b->stream));
   b->use_graph = on != 0;
   release_graph(b);

@@ -43,7 +43,7 @@ EXPORTED = [
     "sgtsne_ctx_tuning", "sgtsne_ctx_set_attractive", "sgtsne_ctx_reorders",
     "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
     "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_mark", "sgtsne_batch_wait",
-    "sgtsne_batch_passed",
+    "sgtsne_batch_passed", "sgtsne_batch_job_left",
     "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
     "sgtsne_batch_set_graph", "sgtsne_copy_probe", "sgtsne_locality_order",
@@ -130,6 +130,7 @@ def lib():
         "sgtsne_batch_mark": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "sgtsne_batch_wait": (C.c_int, [vp, C.c_int64]),
         "sgtsne_batch_passed": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int)]),
+        "sgtsne_batch_job_left": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64)]),
         "sgtsne_batch_job_status": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_batch_get_Y": (C.c_int, [vp, C.c_int, f32p, C.c_int]),
         "sgtsne_batch_terms": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]),
@@ -452,6 +453,13 @@ class Batch:
         p = C.c_int()
         _check(lib().sgtsne_batch_passed(self._h, int(mark), C.byref(p)))
         return bool(p.value)
+
+    def job_left(self, job):
+        """Batch iterations the job needs before its last one is queued (no
+        device access)."""
+        v = C.c_int64()
+        _check(lib().sgtsne_batch_job_left(self._h, int(job), C.byref(v)))
+        return v.value
 
     def job_status(self, job):
         st, it = C.c_int(), C.c_int()

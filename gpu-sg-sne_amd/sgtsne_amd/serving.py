@@ -82,7 +82,7 @@ class Server:
             params = tsneparams(d=d, maxIter=max_iter, seed=seed)
             if engine_factory is None:
                 self.engines[d] = Batch(params, max_jobs, pts, nnz, grid_budget)
-            else:  # any object with Batch's add_points/step/mark/passed/job_status/get_Y/remove
+            else:  # any object with Batch's add_points/step/mark/passed/wait/job_left/job_status/get_Y/remove/close
                 self.engines[d] = engine_factory(params, max_jobs, pts, nnz, grid_budget)
             self.budget[d] = b
             self.used[d] = 0
@@ -103,6 +103,14 @@ class Server:
 
     def submit(self, req):
         self.queue.append(req)
+
+    def _register(self, r, fp, job, now_fn):
+        eng = self.engines[r.d]
+        r.engine, r.job, r.admitted = r.d, job, now_fn()
+        r.t_left = eng.job_left(job)  # one admission iteration plus max_iter
+        r.X = None
+        self.used[r.d] += fp
+        self.active[r.d][job] = (r, fp)
 
     def _admit(self, now_fn, limit=None):
         # first fit, 3-D requests first, then in arrival order: a large request
@@ -136,11 +144,7 @@ class Server:
                 else:
                     keep.append(r)
                 continue
-            r.engine, r.job, r.admitted = r.d, job, now_fn()
-            r.t_left = self.max_iter + 1  # one admission iteration, then max_iter
-            r.X = None
-            self.used[r.d] += fp
-            self.active[r.d][job] = (r, fp)
+            self._register(r, fp, job, now_fn)
             admitted += 1
         self.queue = collections.deque(sorted(keep, key=lambda q: (q.arrival, q.rid)))
         return admitted

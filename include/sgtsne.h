@@ -308,6 +308,12 @@ int sgtsne_batch_passed(sgtsne_batch *batch, int64_t mark, int *passed);
 /* Waits for queued work, then reports SGTSNE_JOB_RUNNING / SGTSNE_JOB_DONE
  * or a negative error of the job and its iteration counter. */
 int sgtsne_batch_job_status(sgtsne_batch *batch, int job, int *status, int *iter);
+/* Batch iterations the job still needs before its last one is queued (host
+ * bookkeeping, no device access): max_iter + 1 right after admission, 0 once
+ * a step has queued its last iteration.  No counterpart in the reference. */
+int sgtsne_batch_job_left(sgtsne_batch *batch, int job, int64_t *left);
+
+/* Threads: call a batch's functions from one host thread at a time. */
 int sgtsne_batch_get_Y(sgtsne_batch *batch, int job, float *y, int layout);
 
 /* The job's attractive term, repulsive term (normalised by Z) and Z at its

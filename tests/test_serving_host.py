@@ -7,6 +7,9 @@ run all its iterations (one admission iteration + max_iter); no engine ever
 has more than `depth` ticks queued; 3-D requests are admitted ahead of 2-D
 ones waiting with them; a request whose footprint can never fit is rejected
 at once; at most one admission per loop pass."""
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -16,8 +19,9 @@ from sgtsne_amd import serving
 class FakeEngine:
     """Batch's serving surface over a modelled queue of batch iterations."""
 
-    def __init__(self, d, log, speed):
-        self.d, self.log, self.speed = d, log, speed
+    def __init__(self, d, log, speed, front_end_s=0.0):
+        self.d, self.log, self.speed, self.front_end_s = d, log, speed, front_end_s
+        self.lock = threading.Lock()  # the library's batch mutex
         self.host_iter = 0     # batch iterations queued
         self.dev_iter = 0      # batch iterations the device has run
         self.mark_at = []      # host_iter at each mark
@@ -29,50 +33,63 @@ class FakeEngine:
         self.dev_iter = min(self.host_iter, self.dev_iter + self.speed)
 
     def add_points(self, X, k, u, y_in, max_iter):
-        self._run()
-        job = self.next_job
-        self.next_job += 1
-        self.jobs[job] = (self.host_iter, max_iter, X.shape[0])
-        self.log.append(("admit", self.d, X.shape[0]))
-        return job
+        time.sleep(self.front_end_s)  # the front end holds no lock
+        with self.lock:
+            self._run()
+            job = self.next_job
+            self.next_job += 1
+            self.jobs[job] = (self.host_iter, max_iter, X.shape[0])
+            self.log.append(("admit", self.d, X.shape[0]))
+            return job
 
     def step(self, iters):
-        self.host_iter += iters
-        self.max_queued = max(self.max_queued, self.host_iter - self.dev_iter)
+        with self.lock:
+            self.host_iter += iters
+            self.max_queued = max(self.max_queued, self.host_iter - self.dev_iter)
 
     def mark(self):
-        self.mark_at.append(self.host_iter)
-        return len(self.mark_at) - 1
+        with self.lock:
+            self.mark_at.append(self.host_iter)
+            return len(self.mark_at) - 1
 
     def passed(self, m):
-        self._run()
-        return self.dev_iter >= self.mark_at[m]
+        with self.lock:
+            self._run()
+            return self.dev_iter >= self.mark_at[m]
 
     def wait(self, m):
-        self.dev_iter = max(self.dev_iter, self.mark_at[m])
+        with self.lock:
+            self.dev_iter = max(self.dev_iter, self.mark_at[m])
+
+    def job_left(self, job):
+        with self.lock:
+            start, max_iter, _ = self.jobs[job]
+            return max(0, max_iter + 1 - (self.host_iter - start))
 
     def job_status(self, job):
-        start, max_iter, _ = self.jobs[job]
-        done = self.dev_iter - start
-        assert done >= max_iter + 1, "retired before its last iteration ran"
-        return 1, max_iter
+        with self.lock:
+            start, max_iter, _ = self.jobs[job]
+            done = self.dev_iter - start
+            assert done >= max_iter + 1, "retired before its last iteration ran"
+            return 1, max_iter
 
     def get_Y(self, job):
         return np.zeros((self.jobs[job][2], self.d), np.float32)
 
     def remove(self, job):
-        del self.jobs[job]
-        self.log.append(("remove", self.d))
+        with self.lock:
+            del self.jobs[job]
+            self.log.append(("remove", self.d))
 
     def close(self):
         pass
 
 
-def _server(log, speed=7, **kw):
+def _server(log, speed=7, front_end_s=0.0, **kw):
     engines = {}
 
     def factory(params, max_jobs, pts, nnz, grid_budget):
-        e = FakeEngine(params.d, log, speed)
+        e = FakeEngine(params.d, log, speed, front_end_s)
         engines[params.d] = e
         return e
 
@@ -83,7 +100,7 @@ def _server(log, speed=7, **kw):
 
 def test_server_finishes_every_request_after_its_iterations():
     log = []
-    srv, eng = _server(log)
+    srv, eng = _server(log, front_end_s=0.001)
     arrivals = [(0.0005 * i, 500 + 37 * i, 2 + (i % 2), i) for i in range(12)]
     with srv:
         got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
