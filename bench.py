@@ -9,6 +9,7 @@ power-law sizes, anisotropic covariance), exact kNN k=90, perplexity 30, d=2,
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -204,8 +205,12 @@ def engine_leg(sg, d, k=90, u=30.0, graph=True, warm=801, timed=190):
         B.sync()
         ms = (time.perf_counter() - t0) * 1e3 / timed
         ngs = sorted(B.terms(j)[3] for j in ids)
+        # fingerprint of every job's Y: equal across builds that compute the same bits
+        h = hashlib.sha1()
+        for j in ids:
+            h.update(B.get_Y(j).tobytes())
     res = {"d": d, "jobs": len(ids), "points": sum(a[1] for a in arr), "ms_per_batch_iteration": round(ms, 4),
-           "iterations": [warm, timed], "ng": ngs, "graph": graph}
+           "iterations": [warm, timed], "ng": ngs, "graph": graph, "y_sha1": h.hexdigest()[:16]}
     log(f"[bench] engine: {res}")
     return res
 

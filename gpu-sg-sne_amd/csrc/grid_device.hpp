@@ -106,23 +106,44 @@ struct NodeLanes {
   static constexpr int PPW = 64 / NODES;  // points per wave per step
 };
 
+// The base cell f of a point's stencil and the weight of its node
+// (a0, a1, a2), evaluating only that node's Lagrange factor along each axis:
+// the same values stencil<D> gives, with half the polynomial work.
+template <int D>
+__device__ __forceinline__ void lane_node(const GridParams &g, const float (&y)[D], int a0, int a1, int a2,
+                                          int (&f)[D], float &weight) {
+  const int an[3] = {a0, a1, a2};
+  float wa[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float t = (y[a] - g.minv[a]) * g.scale;
+    t = fminf(fmaxf(t, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
+    const int fi = (int)t;
+    const float dl = t - (float)fi;
+    f[a] = fi;
+    const int k = an[a];
+    const float x = k == 0 ? 1.0f + dl : k == 1 ? dl : k == 2 ? 1.0f - dl : 2.0f - dl;
+    const float l1 = lg1(x), l2 = lg2(x);
+    wa[a] = (k == 1 || k == 2) ? l1 : l2;
+  }
+  if constexpr (D == 1) weight = wa[0];
+  else if constexpr (D == 2) weight = wa[1] * wa[0];
+  else weight = wa[2] * wa[1] * wa[0];
+}
+
 template <int D>
 __device__ __forceinline__ void node_of_lane(const GridParams &g, const float (&y)[D], int node, int &idx,
                                              float &weight) {
   int f[D];
-  float w[D][4];
-  stencil<D>(g, y, f, w);
   const int a0 = node & 3, a1 = (node >> 2) & 3, a2 = node >> 4;
+  lane_node<D>(g, y, a0, a1, a2, f, weight);
   const int ng = g.ng;
   if constexpr (D == 1) {
     idx = f[0] + a0;
-    weight = w[0][a0];
   } else if constexpr (D == 2) {
     idx = (f[1] + a1) * ng + f[0] + a0;
-    weight = w[1][a1] * w[0][a0];
   } else {
     idx = ((f[2] + a2) * ng + f[1] + a1) * ng + f[0] + a0;
-    weight = w[2][a2] * w[1][a1] * w[0][a0];
   }
 }
 

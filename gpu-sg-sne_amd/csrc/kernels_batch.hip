@@ -27,6 +27,14 @@ namespace {
 using namespace fft;
 
 constexpr int kBT = 256;  // threads of the point kernels
+
+// x * 2^e rounded to the nearest integer (ties to even), for the spread's
+// fixed-point scale 2^e (1 <= e <= 60) and |x * 2^e| < 2^62: the product is
+// exact in FP32, so this is the integer llrint((double)x * 2^e) gives,
+// without FP64 arithmetic.
+__device__ __forceinline__ unsigned long long fix_round(float x, float scale) {
+  return (unsigned long long)llrintf(x * scale);
+}
 // threads of the gather: one per point of a work item, so the 4^d-cell
 // stencil reads of a whole item are in flight at once
 constexpr int kG2sThreads = kBatchChunk;
@@ -69,7 +77,7 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
       if (itm.begin >= itm.end) continue;
     }
     const GridParams g = J->g;
-    const double fs = J->fix_scale;
+    const float fsf = (float)J->fix_scale;  // 2^e, exact in FP32
     unsigned long long *V = reinterpret_cast<unsigned long long *>(J->vfix);
     const int ng = g.ng;
     const int G = grid_cells<D>(ng);
@@ -126,19 +134,15 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
         float y[D];
         load_pt<D>(Y, i, y);
         int f[D];
-        float w[D][4];
-        stencil<D>(g, y, f, w);
-        int loc;
         float ww;
+        lane_node<D>(g, y, a0, a1, a2, f, ww);
+        int loc;
         if constexpr (D == 1) {
           loc = f[0] + a0 - wlo[0];
-          ww = w[0][a0];
         } else if constexpr (D == 2) {
           loc = (f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
-          ww = w[1][a1] * w[0][a0];
         } else {
           loc = ((f[2] + a2 - wlo[2]) * wd[1] + f[1] + a1 - wlo[1]) * wd[0] + f[0] + a0 - wlo[0];
-          ww = w[2][a2] * w[1][a1] * w[0][a0];
         }
         if (loc != cur) {
           if (cur >= 0) {
@@ -149,9 +153,9 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
 #pragma unroll
           for (int m = 0; m < M; ++m) acc[m] = 0ull;
         }
-        acc[0] += (unsigned long long)llrint((double)ww * fs);
+        acc[0] += fix_round(ww, fsf);
 #pragma unroll
-        for (int a = 0; a < D; ++a) acc[a + 1] += (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs);
+        for (int a = 0; a < D; ++a) acc[a + 1] += fix_round((y[a] - g.center[a]) * ww, fsf);
       }
       if (cur >= 0) {
 #pragma unroll
@@ -194,9 +198,9 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
 #pragma unroll
           for (int m = 0; m < M; ++m) acc[m] = 0ull;
         }
-        acc[0] += (unsigned long long)llrint((double)ww * fs);
+        acc[0] += fix_round(ww, fsf);
 #pragma unroll
-        for (int a = 0; a < D; ++a) acc[a + 1] += (unsigned long long)llrint((double)((y[a] - g.center[a]) * ww) * fs);
+        for (int a = 0; a < D; ++a) acc[a + 1] += fix_round((y[a] - g.center[a]) * ww, fsf);
       }
       if (cur >= 0) {
 #pragma unroll
