@@ -209,8 +209,10 @@ def engine_leg(sg, d, k=90, u=30.0, graph=True, warm=801, timed=190):
         h = hashlib.sha1()
         for j in ids:
             h.update(B.get_Y(j).tobytes())
+        tuning = B.tuning()
     res = {"d": d, "jobs": len(ids), "points": sum(a[1] for a in arr), "ms_per_batch_iteration": round(ms, 4),
-           "iterations": [warm, timed], "ng": ngs, "graph": graph, "y_sha1": h.hexdigest()[:16]}
+           "iterations": [warm, timed], "ng": ngs, "graph": graph, "y_sha1": h.hexdigest()[:16],
+           "attractive_tuning": tuning}
     log(f"[bench] engine: {res}")
     return res
 

@@ -88,23 +88,46 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg2_kernel(int n, const int3
     for (int a = 0; a < D; ++a) acc[a] = 0.0f;
     const int e = rowptr[row + 1];
     int t = rowptr[row] + lane;
-    for (; t + (U - 1) * G < e; t += U * G) {
-      int c[U];
-      float v[U];
+    if constexpr (U >= 8) {
+      // masked steps: a lane's next U entries (short rows: all of them) have
+      // their column and value loads, then their gathers, in flight together;
+      // the entries are added in the same order as below
+      for (; t < e; t += U * G) {
+        int c[U];
+        float v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+        for (int u = 0; u < U; ++u) {
+          const bool in = t + u * G < e;
+          c[u] = in ? __builtin_nontemporal_load(col + t + u * G) : -1;
+          v[u] = in ? __builtin_nontemporal_load(val + t + u * G) : 0.0f;
+        }
+        float yj[U][D];
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
-      float yj[U][D];
+        for (int u = 0; u < U; ++u)
+          if (c[u] >= 0) load_pt<D>(Y, c[u], yj[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+        for (int u = 0; u < U; ++u)
+          if (c[u] >= 0) accum_pair<D>(yi, yj[u], v[u], acc);
+      }
+    } else {
+      for (; t + (U - 1) * G < e; t += U * G) {
+        int c[U];
+        float v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
-    }
-    for (; t < e; t += G) {
-      float yj[D];
-      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
-      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+        for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
+        float yj[U][D];
+#pragma unroll
+        for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
+      }
+      for (; t < e; t += G) {
+        float yj[D];
+        load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+        accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+      }
     }
 #pragma unroll
     for (int a = 0; a < D; ++a)
@@ -498,7 +521,7 @@ int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, con
   auto per_g = [&](auto dd, auto uu) {
     go(dd, I4{}, uu); go(dd, I8{}, uu); go(dd, I16{}, uu); go(dd, I32{}, uu); go(dd, I64{}, uu);
   };
-  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); per_g(dd, I4{}); };
+  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); per_g(dd, I4{}); go(dd, I16{}, I8{}); };
   per_d(I1{});
   per_d(I2{});
   per_d(I3{});
