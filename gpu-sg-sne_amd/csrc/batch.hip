@@ -122,9 +122,20 @@ struct sgtsne_batch {
   std::vector<double> tune_ms;
   int s2g_blocks = 1024, s2g_lds = 32768, conv_blocks = 1024, pt_blocks = 2048;
 
-  // captured iteration
+  // captured iteration: two executable graphs used in turn, so that an
+  // update only ever touches the one whose launches have all completed
   hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec = nullptr;  // the one launched now (execs[cur])
+  hipGraphExec_t execs[2] = {nullptr, nullptr};
+  hipEvent_t exec_done[2] = {nullptr, nullptr};  // after each one's last launch
+  int cur = 0;
+  // memory released on the iteration stream, returned to the pool once the
+  // queued work that used it has passed its event
+  struct Deferred {
+    void *p[3];
+    hipEvent_t ev;
+  };
+  std::vector<Deferred> deferred;
   bool graph_valid = false;
   bool use_graph = true;
   int64_t graph_replays = 0, graph_updates = 0, graph_instantiations = 0;
@@ -244,13 +255,41 @@ int upload(sgtsne_batch *b, void *dst, const void *src, size_t bytes) {
   return SGTSNE_OK;
 }
 
-// Destroys the executable graph (no launch of it may be pending).
+// Destroys the executable graphs (no launch of them may be pending).
 void release_graph(sgtsne_batch *b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (int k = 0; k < 2; ++k) {
+    if (b->execs[k]) (void)hipGraphExecDestroy(b->execs[k]);
+    b->execs[k] = nullptr;
+  }
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   b->graph_valid = false;
+}
+
+// Queues the release of up to three blocks once the iteration stream has
+// passed this point (see release_deferred).
+int defer_free(sgtsne_batch *b, void *p0, void *p1 = nullptr, void *p2 = nullptr) {
+  hipEvent_t ev = nullptr;
+  SGT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  SGT_HIP(hipEventRecord(ev, b->stream));
+  b->deferred.push_back(sgtsne_batch::Deferred{{p0, p1, p2}, ev});
+  return SGTSNE_OK;
+}
+
+// Returns to the pool the deferred blocks whose event has completed (all of
+// them with `all`, after the caller has drained the stream).
+void release_deferred(sgtsne_batch *b, bool all = false) {
+  for (auto it = b->deferred.begin(); it != b->deferred.end();) {
+    if (all || hipEventQuery(it->ev) == hipSuccess) {
+      for (void *q : it->p)
+        if (q) (void)hipFreeAsync(q, b->copy);  // idle stream: the block is free at once
+      (void)hipEventDestroy(it->ev);
+      it = b->deferred.erase(it);
+    } else {
+      ++it;
+    }
+  }
 }
 
 // Writes the static part of slot j's record (everything before the dynamic
@@ -333,22 +372,29 @@ int capture(sgtsne_batch *b) {
     set_error("graph capture failed: %s", hipGetErrorString(e));
     return SGTSNE_ERR_HIP;
   }
-  if (b->exec) {
+  // the other executable graph: its launches all precede the current one's,
+  // so waiting for its last one rarely waits at all
+  const int k = b->exec ? 1 - b->cur : b->cur;
+  if (b->execs[k] && b->exec_done[k]) SGT_HIP(hipEventSynchronize(b->exec_done[k]));
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph = g;
+  if (b->execs[k]) {
     hipGraphNode_t err_node = nullptr;
     hipGraphExecUpdateResult res = hipGraphExecUpdateError;
-    if (hipGraphExecUpdate(b->exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
-      if (b->graph) (void)hipGraphDestroy(b->graph);
-      b->graph = g;
+    if (hipGraphExecUpdate(b->execs[k], g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      b->cur = k;
+      b->exec = b->execs[k];
       b->graph_updates++;
       b->graph_valid = true;
       return SGTSNE_OK;
     }
     (void)hipGetLastError();
-    SGT_HIP(hipStreamSynchronize(b->stream));  // launches of the old graph may be pending
-    release_graph(b);
+    (void)hipGraphExecDestroy(b->execs[k]);  // no launch of it is pending (waited above)
+    b->execs[k] = nullptr;
   }
-  b->graph = g;
-  SGT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  SGT_HIP(hipGraphInstantiate(&b->execs[k], g, nullptr, nullptr, 0));
+  b->cur = k;
+  b->exec = b->execs[k];
   b->graph_instantiations++;
   b->graph_valid = true;
   return SGTSNE_OK;
@@ -364,6 +410,11 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
     } else {
       SGT_TRY(enqueue_iteration(b));
     }
+  }
+  if (b->graph_valid) {
+    hipEvent_t &ev = b->exec_done[b->cur];
+    if (!ev) SGT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SGT_HIP(hipEventRecord(ev, b->stream));
   }
   for (int j : b->arena) {
     HostJob &h = b->hj[j];
@@ -672,6 +723,12 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
 void destroy_all(sgtsne_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   release_graph(b);
+  release_deferred(b, true);
+  for (hipEvent_t &e : b->exec_done)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
   // per-job grids come from the stream-ordered pool: release them the same way
   for (auto &h : b->hj)
     if (h.gmem) {
@@ -882,20 +939,20 @@ int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr,
     st = SGTSNE_ERR_HIP;
   }
   std::lock_guard<std::mutex> lock(b->mu);
+  release_deferred(b);
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
-  // the admitted P is read by copies queued on the iteration stream
-  const hipStream_t fs = st ? s : b->stream;
   if (srp != rp) {
-    (void)hipFreeAsync(srp, fs);
-    (void)hipFreeAsync(scc, fs);
-    (void)hipFreeAsync(svv, fs);
     (void)hipFreeAsync(rp, s);
     (void)hipFreeAsync(cc, s);
     (void)hipFreeAsync(vv, s);
+  }
+  // the admitted P is read by copies queued on the iteration stream
+  if (st) {
+    (void)hipFreeAsync(srp, s);
+    (void)hipFreeAsync(scc, s);
+    (void)hipFreeAsync(svv, s);
   } else {
-    (void)hipFreeAsync(rp, fs);
-    (void)hipFreeAsync(cc, fs);
-    (void)hipFreeAsync(vv, fs);
+    SGT_TRY(defer_free(b, srp, scc, svv));
   }
   return st;
 }
@@ -938,12 +995,16 @@ int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int
   void *tmp[] = {Xf, Xd, Dd, Id, rp, cc, vv};
   for (void *q : tmp) (void)hipFreeAsync(q, s);
   std::lock_guard<std::mutex> lock(b->mu);
+  release_deferred(b);
   if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
   // the admitted P is read by copies queued on the iteration stream
-  const hipStream_t fs = st ? s : b->stream;
-  if (srp) (void)hipFreeAsync(srp, fs);
-  if (scc) (void)hipFreeAsync(scc, fs);
-  if (svv) (void)hipFreeAsync(svv, fs);
+  if (st) {
+    if (srp) (void)hipFreeAsync(srp, s);
+    if (scc) (void)hipFreeAsync(scc, s);
+    if (svv) (void)hipFreeAsync(svv, s);
+  } else {
+    SGT_TRY(defer_free(b, srp, scc, svv));
+  }
   return st;
 }
 
@@ -951,6 +1012,7 @@ int sgtsne_batch_step(sgtsne_batch *b, int iters) {
   SGT_TRY(check_batch(b));
   std::lock_guard<std::mutex> lock(b->mu);
   SGT_CHECK(iters >= 0, SGTSNE_ERR_ARG, "iters < 0");
+  release_deferred(b);
   if (b->arena.empty()) return SGTSNE_OK;
   if (b->tuned_points == 0 || b->used_points > 2 * b->tuned_points || 2 * b->used_points < b->tuned_points)
     SGT_TRY(tune(b));
@@ -1086,7 +1148,8 @@ int sgtsne_batch_remove(sgtsne_batch *b, int job) {
   BatchJob empty{};
   empty.status = BJ_EMPTY;
   SGT_TRY(upload(b, b->jobs_d + job, &empty, sizeof(BatchJob)));
-  SGT_HIP(hipFreeAsync(h.gmem, b->stream));
+  release_deferred(b);
+  SGT_TRY(defer_free(b, h.gmem));
   if (h.done_ev) (void)hipEventDestroy(h.done_ev);
   b->grid_used -= h.gbytes;
   b->arena.erase(std::find(b->arena.begin(), b->arena.end(), job));
