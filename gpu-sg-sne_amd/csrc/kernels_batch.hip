@@ -35,9 +35,11 @@ constexpr int kBT = 256;  // threads of the point kernels
 __device__ __forceinline__ unsigned long long fix_round(float x, float scale) {
   return (unsigned long long)llrintf(x * scale);
 }
-// threads of the gather: one per point of a work item, so the 4^d-cell
-// stencil reads of a whole item are in flight at once
-constexpr int kG2sThreads = kBatchChunk;
+// threads of the gather: in 3-D one per point of a work item, so the 64-cell
+// stencil reads of a whole item are in flight at once (measured: 181 -> 120
+// us on the served mix); in 1-D and 2-D the 256-thread form is faster
+template <int D>
+constexpr int g2s_threads() { return D == 3 ? kBatchChunk : kBT; }
 
 __constant__ int kGridSizesDev[] = {SGT_GRID_SIZE_LIST};
 constexpr int kGridSizesHost[] = {SGT_GRID_SIZE_LIST};
@@ -234,11 +236,11 @@ __global__ __launch_bounds__(kFftThreads) void bpass_kernel(const PassArgs *__re
 
 // ---- G2S + forces + Z partials -------------------------------------------------
 template <int D, bool ZDIRECT>
-__global__ __launch_bounds__(kG2sThreads) void bg2s_kernel(const BatchJob *__restrict__ jobs,
+__global__ __launch_bounds__(g2s_threads<D>()) void bg2s_kernel(const BatchJob *__restrict__ jobs,
                                                    const BatchItem *__restrict__ items, int nitems,
                                                    const float *__restrict__ Y, float *__restrict__ Frep,
                                                    double *__restrict__ zpart) {
-  __shared__ double lds[2 * (kG2sThreads / 64)];
+  __shared__ double lds[2 * (g2s_threads<D>() / 64)];
   constexpr int C = ZDIRECT ? D + 2 : D + 1;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem itm = items[it];
@@ -724,8 +726,8 @@ int launch_bg2s(hipStream_t s, int d, int zdirect, const BatchJob *jobs, const B
   if (nitems <= 0) return SGTSNE_OK;
 #define BG2S_CASE(DD)                                                                                     \
   case DD:                                                                                                \
-    if (zdirect) bg2s_kernel<DD, true><<<blocks, kG2sThreads, 0, s>>>(jobs, items, nitems, Y, Frep, zpart); \
-    else bg2s_kernel<DD, false><<<blocks, kG2sThreads, 0, s>>>(jobs, items, nitems, Y, Frep, zpart);        \
+    if (zdirect) bg2s_kernel<DD, true><<<blocks, g2s_threads<DD>(), 0, s>>>(jobs, items, nitems, Y, Frep, zpart); \
+    else bg2s_kernel<DD, false><<<blocks, g2s_threads<DD>(), 0, s>>>(jobs, items, nitems, Y, Frep, zpart);        \
     break;
   switch (d) {
     BG2S_CASE(1)
