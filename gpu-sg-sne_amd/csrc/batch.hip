@@ -187,11 +187,13 @@ struct Carve {
 double bdefault_h(int d) { return d == 1 ? 0.5 : d == 2 ? 0.7 : 1.2; }
 
 // Largest grid side a job may reach: the embedding extent grows with N, so
-// small jobs get small grid buffers.
+// small jobs get small grid buffers.  3-D jobs of 10k-20k points reach sides
+// of about 60, so their cap leaves room above that (a job that outgrows its
+// cap stops with SGTSNE_ERR_FULL).
 int auto_ng_cap(int d, int n) {
   if (d == 1) return 512;
   if (d == 2) return n <= 20000 ? 256 : 512;
-  return n <= 20000 ? 64 : 128;
+  return n <= 20000 ? 96 : 128;
 }
 
 // The captured iteration no longer matches the arena: the next step
@@ -430,8 +432,9 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
 }
 
 // Attractive-kernel shape for the current arena.  The lanes per row stay
-// fixed (16): every candidate then sums each row in the same order, so the
-// choice never changes a result; only rows per work-group and unroll vary.
+// fixed (the engine's spmv_group): every candidate then sums each row in the
+// same order, so the choice never changes a result; only rows per
+// work-group and unroll vary.
 int tune(sgtsne_batch *b) {
   if (b->used_points < 4096) return SGTSNE_OK;
   struct Cand { int unroll, rpb, win; };
@@ -835,6 +838,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   b->p = *params;
   b->d = params->d;
   b->ds = point_stride(params->d);
+  b->spmv_group = params->d == 3 ? 32 : 16;  // lanes per row, fixed for the engine's life
   b->max_jobs = max_jobs;
   b->max_points = max_points;
   b->max_nnz = max_nnz;

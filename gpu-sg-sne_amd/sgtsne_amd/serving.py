@@ -40,7 +40,7 @@ _POINT_BYTES = {2: 6 * 2 * 4 + 4, 3: 6 * 4 * 4 + 4}  # Y, uY, gains, Fattr, Frep
 def footprint_bytes(n, d, k=90):
     """Device bytes a job holds while admitted: P_sym (<= 2 n k nonzeros,
     8 B each), the per-point state and the grid buffers of its capacity."""
-    grid = {2: (512 if n > 20000 else 256), 3: (128 if n > 20000 else 64)}[d]
+    grid = {2: (512 if n > 20000 else 256), 3: (128 if n > 20000 else 96)}[d]  # the engine's grid cap
     L = 1
     while L < 2 * grid - 1:
         L <<= 1
