@@ -71,7 +71,12 @@ struct sgtsne_batch {
   int d = 2, ds = 2;
   hipStream_t stream = nullptr;  // iterations and every arena write
   hipStream_t prep = nullptr;    // front end of new jobs (kNN, perplexity, P + P^T)
-  hipStream_t copy = nullptr;    // reads of finished jobs
+  // reads of finished jobs: the front-end stream, idle whenever such a read
+  // runs (batch calls come from one thread and a front end returns after its
+  // work is done).  Two streams per batch, not three: a server's two engines
+  // then need four, one per hardware queue (GPU_MAX_HW_QUEUES = 4), and do not
+  // depend on which queues earlier streams of the process left busiest
+  hipStream_t copy = nullptr;
   hipEvent_t ev_prep = nullptr;
   // host -> device updates go through a pinned ring, so queuing them never
   // waits for the iterations queued ahead of them
@@ -782,7 +787,7 @@ void destroy_all(sgtsne_batch *b) {
     }
   if (b->ev_prep) (void)hipEventDestroy(b->ev_prep);
   if (b->prep) (void)hipStreamDestroy(b->prep);
-  if (b->copy) (void)hipStreamDestroy(b->copy);
+  b->copy = nullptr;  // the front-end stream, destroyed above
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -857,7 +862,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   retain_pool_memory();
   chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   chk(hipStreamCreateWithFlags(&b->prep, hipStreamNonBlocking), "stream");
-  chk(hipStreamCreateWithFlags(&b->copy, hipStreamNonBlocking), "stream");
+  b->copy = b->prep;
   chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
   for (hipEvent_t &e : b->marks) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
   chk(hipHostMalloc((void **)&b->stage, kStageBytes, hipHostMallocDefault), "pinned staging");
