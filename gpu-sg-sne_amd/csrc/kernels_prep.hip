@@ -230,6 +230,7 @@ __device__ __forceinline__ bool key_lessf(float da, int ia, float db, int ib) {
 template <int T, int NST>
 __device__ __forceinline__ void tile_load(const float *__restrict__ Xc, const float *__restrict__ n32, int n, int r0,
                                           float4 (&buf)[kFiltR * NST / T], float &rnv) {
+  static_assert((kFiltR * NST) % T == 0, "a tile must split evenly over the work-group");
   const float4 *X4 = reinterpret_cast<const float4 *>(Xc);
 #pragma unroll
   for (int u = 0; u < kFiltR * NST / T; ++u) {
@@ -529,14 +530,24 @@ int launch_filter(hipStream_t s, const float *Xc, const float *n32, int n, int k
     const size_t lds = (sizeof(float) + sizeof(int)) * (size_t)kF2Q * kp;
     knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(Xc, n32, n, kp, hd, hi);
   } else {
-    knn_filter_kernel<NST><<<div_up(n, kFiltQ), 256, 0, s>>>(Xc, n32, n, kp, hd, hi);
+    if constexpr ((kFiltR * NST) % 256 == 0) {
+      knn_filter_kernel<NST><<<div_up(n, kFiltQ), 256, 0, s>>>(Xc, n32, n, kp, hd, hi);
+    } else {
+      SGT_CHECK(false, SGTSNE_ERR_ARG, "no global-heap filter of %d steps", NST);
+    }
   }
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
 
-int filter_steps(int dim) {
+// float4 steps of the filter's padded dimension.  14 (dims 49-56: the usual
+// 50-D PCA input) only with the LDS-heap kernel, whose 128-thread tile load
+// splits 64 x 14 float4 evenly: its LDS then fits three work-groups per CU,
+// where 16 steps fit two.  The zero padding adds exact zeros, so the FP32
+// distances, the candidates and the rows are the same either way.
+int filter_steps(int dim, int kp) {
   const int nst = (dim + 3) / 4;
+  if (nst > 12 && nst <= 14 && kp <= kF2MaxKp) return 14;
   return nst <= 4 ? 4 : nst <= 8 ? 8 : nst <= 12 ? 12 : nst <= 16 ? 16 : nst <= 24 ? 24 : 32;
 }
 
@@ -545,6 +556,7 @@ int launch_filter_for(hipStream_t s, int nst, const float *Xc, const float *n32,
     case 4: return launch_filter<4>(s, Xc, n32, n, kp, hd, hi);
     case 8: return launch_filter<8>(s, Xc, n32, n, kp, hd, hi);
     case 12: return launch_filter<12>(s, Xc, n32, n, kp, hd, hi);
+    case 14: return launch_filter<14>(s, Xc, n32, n, kp, hd, hi);
     case 16: return launch_filter<16>(s, Xc, n32, n, kp, hd, hi);
     case 24: return launch_filter<24>(s, Xc, n32, n, kp, hd, hi);
     default: return launch_filter<32>(s, Xc, n32, n, kp, hd, hi);
@@ -936,13 +948,13 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "kNN needs n > k >= 1 (n=%d k=%d)", n, k);
   SGT_CHECK(k <= kKnnR * 64, SGTSNE_ERR_ARG, "k too large");
   if (fallback_rows) *fallback_rows = -1;
-  const int nst = filter_steps(dim), dp = 4 * nst;
   const bool filter_ok = k + kFiltSlack <= kRerank && dim <= kFiltMaxDp && n >= kFiltQ;
   if (method == SGTSNE_KNN_EXACT_FP64 || (method == SGTSNE_KNN_AUTO && !filter_ok))
     return knn_exact_rows(s, X, n, dim, k, nullptr, n, I, D);
   SGT_CHECK(filter_ok, SGTSNE_ERR_ARG, "the filtered kNN needs k <= %d, dim <= %d and n >= %d",
             kRerank - kFiltSlack, kFiltMaxDp, kFiltQ);
   const int kp = std::min(k + kFiltSlack, n - 1);
+  const int nst = filter_steps(dim, kp), dp = 4 * nst;
   double *mu = nullptr, *nrm = nullptr;
   float *Xc = nullptr, *n32 = nullptr, *hd = nullptr;
   int *hi = nullptr, *fb = nullptr;
