@@ -69,7 +69,9 @@ class Server:
 
     def __init__(self, hbm_bytes=48 << 30, max_jobs=256, tick=10, k=90, u=30.0, max_iter=1000,
                  share=(0.5, 0.5), seed=0, depth=3, poll_s=2e-4, engine_factory=None):
-        self.k, self.u, self.max_iter, self.tick, self.depth = k, u, max_iter, tick, depth
+        self.k, self.u, self.max_iter, self.tick = k, u, max_iter, tick
+        # ticks in flight per engine: an int for both, or {d: depth}
+        self.depth = depth if isinstance(depth, dict) else {2: depth, 3: depth}
         self.poll_s = poll_s
         self.engines = {}
         self.budget = {}
@@ -160,7 +162,7 @@ class Server:
             eng, marks = self.engines[d], self.inflight[d]
             while marks and eng.passed(marks[0]):
                 marks.popleft()
-            if len(marks) >= self.depth:
+            if len(marks) >= self.depth[d]:
                 continue  # enough queued: this engine is topped up on a later pass
             eng.step(self.tick)
             m = eng.mark()

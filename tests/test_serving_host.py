@@ -143,3 +143,14 @@ def test_server_depth_bounds_queued_ticks(speed):
     assert len(got) == 3
     for e in eng.values():
         assert e.max_queued <= 2 * 10
+
+
+def test_server_depth_per_engine():
+    # depth may differ per engine: {d: ticks in flight}
+    log = []
+    srv, eng = _server(log, speed=1, depth={2: 1, 3: 2}, max_iter=40)
+    arrivals = [(0.0, 800, 2, 0), (0.0, 900, 3, 1)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 2
+    assert eng[2].max_queued <= 1 * 10 and eng[3].max_queued <= 2 * 10
