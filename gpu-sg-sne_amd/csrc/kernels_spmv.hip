@@ -522,7 +522,8 @@ int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, con
     go(dd, I4{}, uu); go(dd, I8{}, uu); go(dd, I16{}, uu); go(dd, I32{}, uu); go(dd, I64{}, uu);
   };
   auto per_d = [&](auto dd) {
-    per_g(dd, I1{}); per_g(dd, I2{}); per_g(dd, I4{}); go(dd, I16{}, I8{}); go(dd, I32{}, I8{});
+    per_g(dd, I1{}); per_g(dd, I2{}); per_g(dd, I4{});
+    go(dd, I16{}, I8{}); go(dd, I32{}, I8{}); go(dd, I64{}, I8{}); go(dd, I16{}, I16{}); go(dd, I32{}, I16{});
   };
   per_d(I1{});
   per_d(I2{});
