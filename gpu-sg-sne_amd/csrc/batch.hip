@@ -443,7 +443,8 @@ int run_iterations(sgtsne_batch *b, int64_t count) {
 int tune(sgtsne_batch *b) {
   if (b->used_points < 4096) return SGTSNE_OK;
   struct Cand { int unroll, rpb, win; };
-  const Cand cands[] = {{1, 0, 0},       {2, 0, 0},       {4, 0, 0},       {8, 0, 0},      {2, 32, 0},     {2, 64, 0},
+  const Cand cands[] = {{1, 0, 0},       {2, 0, 0},       {4, 0, 0},       {8, 0, 0},      {16, 0, 0},     {2, 32, 0},
+                        {2, 64, 0},
                         {1, 64, 0},      {2, 128, 0},     {1, 128, 0},     {2, 256, 0},    {1, 256, 4096},
                         {2, 512, 4096},  {1, 512, 8192},  {2, 1024, 8192}};
   hipEvent_t e0, e1;
