@@ -124,6 +124,7 @@ struct sgtsne_batch {
   // persistent work-group counts)
   int spmv_group = 16, spmv_unroll = 2, spmv_rpb = 0, spmv_win = 0;
   int64_t tuned_points = 0;
+  bool tune_pinned = false;  // shape pinned by sgtsne_batch_set_attractive
   std::vector<double> tune_ms;
   int s2g_blocks = 1024, s2g_lds = 32768, conv_blocks = 1024, pt_blocks = 2048;
 
@@ -1024,7 +1025,8 @@ int sgtsne_batch_step(sgtsne_batch *b, int iters) {
   SGT_CHECK(iters >= 0, SGTSNE_ERR_ARG, "iters < 0");
   release_deferred(b);
   if (b->arena.empty()) return SGTSNE_OK;
-  if (b->tuned_points == 0 || b->used_points > 2 * b->tuned_points || 2 * b->used_points < b->tuned_points)
+  if (!b->tune_pinned &&
+      (b->tuned_points == 0 || b->used_points > 2 * b->tuned_points || 2 * b->used_points < b->tuned_points))
     SGT_TRY(tune(b));
   int64_t left = iters;
   while (left > 0) {
@@ -1198,6 +1200,34 @@ int sgtsne_batch_tuning(sgtsne_batch *b, int *group, int *unroll, int *rows_per_
   if (unroll) *unroll = b->spmv_unroll;
   if (rows_per_block) *rows_per_block = b->spmv_rpb;
   if (use_graph) *use_graph = b->use_graph ? 1 : 0;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_set_attractive(sgtsne_batch *b, int unroll, int rows_per_block, int window) {
+  SGT_TRY(check_batch(b));
+  std::lock_guard<std::mutex> lock(b->mu);
+  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 && window >= 0 && (window == 0 || rows_per_block > 0),
+            SGTSNE_ERR_ARG, "bad attractive kernel shape");
+  SGT_CHECK((size_t)window * b->ds * sizeof(float) <= 65536, SGTSNE_ERR_ARG, "window does not fit LDS");
+  const int keep[3] = {b->spmv_unroll, b->spmv_rpb, b->spmv_win};
+  b->spmv_unroll = unroll;
+  b->spmv_rpb = rows_per_block;
+  b->spmv_win = window;
+  // one launch on the current arena checks that the shape exists (it only
+  // rewrites Fattr, which every iteration recomputes first)
+  int st = attractive(b);
+  if (st == SGTSNE_OK && hipStreamSynchronize(b->stream) != hipSuccess) {
+    set_error("attractive kernel failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (st != SGTSNE_OK) {
+    b->spmv_unroll = keep[0];
+    b->spmv_rpb = keep[1];
+    b->spmv_win = keep[2];
+    return st;
+  }
+  b->tune_pinned = true;
+  invalidate_graph(b);
   return SGTSNE_OK;
 }
 

@@ -46,7 +46,7 @@ EXPORTED = [
     "sgtsne_batch_passed", "sgtsne_batch_job_left",
     "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
-    "sgtsne_batch_set_graph", "sgtsne_copy_probe", "sgtsne_locality_order",
+    "sgtsne_batch_set_graph", "sgtsne_batch_set_attractive", "sgtsne_copy_probe", "sgtsne_locality_order",
 ]
 
 
@@ -140,6 +140,7 @@ def lib():
         "sgtsne_batch_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "sgtsne_batch_set_graph": (C.c_int, [vp, C.c_int]),
+        "sgtsne_batch_set_attractive": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "sgtsne_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
         "sgtsne_locality_order": (C.c_int, [C.c_int, i32p, i32p, i32p]),
     }
@@ -500,3 +501,7 @@ class Batch:
 
     def set_graph(self, on):
         _check(lib().sgtsne_batch_set_graph(self._h, int(bool(on))))
+
+    def set_attractive(self, unroll, rows_per_block=0, window=0):
+        """Pins the attractive-kernel shape (the lanes per row stay the engine's)."""
+        _check(lib().sgtsne_batch_set_attractive(self._h, int(unroll), int(rows_per_block), int(window)))

@@ -334,6 +334,14 @@ int sgtsne_batch_info(sgtsne_batch *batch, int *jobs, int64_t *points, int64_t *
 int sgtsne_batch_tuning(sgtsne_batch *batch, int *group, int *unroll, int *rows_per_block, int *window,
                         int *use_graph);
 
+/* Pins the batch's attractive-kernel shape (unroll, rows per work-group,
+ * points of Y staged in LDS; the same meaning as sgtsne_batch_tuning's
+ * outputs) and stops re-tuning.  The lanes per row stay the engine's, so the
+ * pin never changes a result; it lets a caller run, test or profile one
+ * shape.  An unsupported shape returns SGTSNE_ERR_ARG and keeps the current
+ * one.  No reference counterpart (the reference has no batched form). */
+int sgtsne_batch_set_attractive(sgtsne_batch *batch, int unroll, int rows_per_block, int window);
+
 /* Graph replay on (default) or off (iterations launched one by one). */
 int sgtsne_batch_set_graph(sgtsne_batch *batch, int on);
 

@@ -23,6 +23,8 @@
 
 #include "sgtsne.h"
 
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -55,16 +57,27 @@ typedef float matval;
 typedef int matidx;
 
 // CU/types.hpp:27-47: CSR (row = offsets [n+1], col = indices [nnz]).  The
-// ELL / COO fields of the reference are built on the device here.
+// block, ELL and COO fields keep the reference's names so that callers that
+// set or read them compile unchanged; this engine builds its own formats on
+// the device from the CSR (sgtsneCUDA reads only m, n, nnz, row, col, val),
+// so they stay zero / null unless the caller fills them.
 template <typename dataPoint>
 struct sparse_matrix {
-  int m = 0;
-  int n = 0;
-  int nnz = 0;
-  int format = 0;
+  int m = 0;       // rows
+  int n = 0;       // columns
+  int nnz = 0;     // nonzeros
+  int format = 0;  // 0 CSR, 1 BSR (never implemented by the reference), 2 COO, 3 hybrid ELL + COO
   matidx *row = nullptr;
   matidx *col = nullptr;
   dataPoint *val = nullptr;
+
+  int nnzb = 0;
+  int blockSize = 0;
+  int blockRows = 0;
+
+  unsigned int *ell_cols = nullptr, *coo_col_ids = nullptr, *coo_row_ids = nullptr;
+  dataPoint *ell_data = nullptr, *coo_data = nullptr;
+  int coo_size = 0, elements_in_rows = 0;
 };
 
 // CU/types.hpp:55-72, same fields and defaults.
@@ -170,8 +183,36 @@ typedef struct {
   double *val;
 } sparse_matrix;
 
-// HY/sgtsne.cpp:37-197.  The column-stochastic CSC of P is the row-stochastic
-// CSR of P^T, and P + P^T is the same matrix: it is passed as CSR unchanged.
+// HY/sparsematrix.cpp:27-67 (makeStochastic): every column j whose sum is
+// nonzero (or whose first stored value is nonzero) and differs from 1 by more
+// than 1e-12 is divided by its sum, in FP64, in place.  Returns the number of
+// columns that were already stochastic.
+inline uint32_t makeStochastic(sparse_matrix P) {
+  uint32_t nStoch = 0;
+  for (int j = 0; j < P.n; j++) {
+    const uint32_t t0 = P.col[j], t1 = P.col[j + 1];
+    double sum = 0;
+    for (uint32_t t = t0; t < t1; t++) sum += P.val[t];
+    const bool first_nonzero = t0 < t1 && P.val[t0] != 0.0;
+    if (sum != 0.0 || first_nonzero) {
+      if (std::abs(sum - 1) > 1e-12)
+        for (uint32_t t = t0; t < t1; t++) P.val[t] /= sum;
+      else
+        nStoch++;
+    }
+  }
+  return nStoch;
+}
+
+// HY/sgtsne.cpp:37-197.  P is first made column stochastic in place, as the
+// reference does (HY/sgtsne.cpp:60); the column-stochastic CSC of P is the
+// row-stochastic CSR of P^T, and P + P^T is the same matrix, so it is passed
+// as CSR unchanged.  The engine computes in FP32 (P and Y are rounded once).
+// *timeInfo, when given, receives HYB's layout: 8 doubles per iteration in
+// seconds (0 attractive, 1 spread, 2 grid convolution, 3 gather, 4 forces and
+// Z, 5 centring and bounding box, 6-7 zero: no CPU/GPU overlap to wait for)
+// and the loop time in seconds at 8*maxIter (HY/gradient_descendHYB.cpp:
+// 121-142,210,248).
 inline double *sgtsne(sparse_matrix P, tsneparams params, double *y_in = nullptr, double **timeInfo = nullptr) {
   sgtsne_params c;
   sgtsne_params_default(&c);
@@ -186,6 +227,7 @@ inline double *sgtsne(sparse_matrix P, tsneparams params, double *y_in = nullptr
   c.np = params.np;
   c.layout = SGTSNE_LAYOUT_AOS;
   const int n = P.n;
+  makeStochastic(P);
   std::vector<int32_t> rp((size_t)n + 1), ci((size_t)P.nnz);
   std::vector<float> v((size_t)P.nnz);
   for (int i = 0; i <= n; i++) rp[i] = (int32_t)P.col[i];
@@ -195,10 +237,30 @@ inline double *sgtsne(sparse_matrix P, tsneparams params, double *y_in = nullptr
   }
   std::vector<float> yi, yo((size_t)n * params.d);
   if (y_in) yi.assign(y_in, y_in + (size_t)n * params.d);
-  double *ti = (timeInfo && *timeInfo) ? *timeInfo : nullptr;
+  const int iters = params.maxIter > 0 ? params.maxIter : 0;
+  std::vector<double> t7;
+  if (timeInfo && *timeInfo) t7.assign((size_t)7 * iters + 1, 0.0);
   sgtsne_mi355x::check(sgtsne_run(n, P.nnz, rp.data(), ci.data(), v.data(), &c, y_in ? yi.data() : nullptr,
-                                  yo.data(), ti),
+                                  yo.data(), t7.empty() ? nullptr : t7.data()),
                        "sgtsne");
+  if (!t7.empty()) {
+    // the C-ABI's CUDA layout (ms: 0 attractive, 1 spread, 2 convolution,
+    // 3 gather, 4 forces and Z, 5 repulsion total, 6 preprocessing)
+    double *ti = *timeInfo;
+    for (int it = 0; it < iters; it++) {
+      const double *s7 = &t7[(size_t)7 * it];
+      double *s8 = ti + (size_t)8 * it;
+      s8[0] = s7[0] / 1e3;
+      s8[1] = s7[1] / 1e3;
+      s8[2] = s7[2] / 1e3;
+      s8[3] = s7[3] / 1e3;
+      s8[4] = s7[4] / 1e3;
+      s8[5] = s7[6] / 1e3;
+      s8[6] = 0.0;
+      s8[7] = 0.0;
+    }
+    ti[(size_t)8 * iters] = t7[(size_t)7 * iters] / 1e3;
+  }
   double *y = static_cast<double *>(std::malloc(sizeof(double) * yo.size()));
   if (!y) sgtsne_mi355x::fail(SGTSNE_ERR_NOMEM, "sgtsne");
   for (size_t t = 0; t < yo.size(); t++) y[t] = yo[t];

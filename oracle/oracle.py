@@ -60,6 +60,10 @@ def lib():
         "or_kl": (C.c_double, [C.c_int, C.c_int, _i64p, _i32p, _f64p, _f64p, C.c_double, C.c_double]),
         "or_run": (None, [C.c_int, C.c_int, _i64p, _i32p, _f64p, C.c_double, C.c_int, C.c_double,
                           C.c_int, C.c_int, _f64p, _f64p, _f64p, C.c_void_p, C.c_void_p]),
+        "or_repulsive_interp_g": (C.c_double, [C.c_int, C.c_int, _f64p, C.c_double, C.c_int, C.c_int, _f64p,
+                                               C.POINTER(C.c_int)]),
+        "or_run_g": (None, [C.c_int, C.c_int, _i64p, _i32p, _f64p, C.c_double, C.c_int, C.c_double, C.c_int,
+                            C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, C.c_void_p, C.c_void_p]),
         "or_set_threads": (None, [C.c_int]),
         "or_get_threads": (C.c_int, []),
     }
@@ -235,14 +239,15 @@ ZETA_DIRECT = 1      # oracle zmode numbering: 0 = algebraic (reference), 1 = di
 ZETA_ALGEBRAIC = 0
 
 
-def repulsive_interp(Y, h, zmode=ZETA_ALGEBRAIC):
+def repulsive_interp(Y, h, zmode=ZETA_ALGEBRAIC, fixed_ng=0):
     """Returns (Frep, Z, ng).  Y is shifted to min 0 in place (reference
-    semantics); pass a copy to keep the original."""
+    semantics); pass a copy to keep the original.  fixed_ng > 0: the CUDA
+    path's fixed grid (nGrid = best_grid_size(fixed_ng)) instead of the h rule."""
     assert Y.dtype == np.float64 and Y.flags.c_contiguous
     n, d = Y.shape
     F = np.zeros_like(Y)
     ng = C.c_int()
-    Z = lib().or_repulsive_interp_z(n, d, Y, float(h), int(zmode), F, C.byref(ng))
+    Z = lib().or_repulsive_interp_g(n, d, Y, float(h), int(zmode), int(fixed_ng), F, C.byref(ng))
     return F, Z, ng.value
 
 
@@ -257,7 +262,7 @@ def kl(P, Y, Z, alpha=1.0):
     return lib().or_kl(n, d, P.rowptr, P.col, P.val, Y, float(Z), float(alpha))
 
 
-def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None, zmode=ZETA_ALGEBRAIC):
+def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None, zmode=ZETA_ALGEBRAIC, fixed_ng=0):
     """Runs iterations [it0, it1) in place; returns (Z per iteration, ng per iteration)."""
     n, d = Y.shape
     if h is None or h <= 0:
@@ -265,6 +270,6 @@ def run(P, Y, uY, gains, it0, it1, alpha0=12.0, early_iter=250, h=None, zmode=ZE
     cnt = it1 - it0
     Z = np.empty(max(cnt, 1))
     ng = np.empty(max(cnt, 1), np.int32)
-    lib().or_run_z(n, d, P.rowptr, P.col, P.val, float(alpha0), int(early_iter), float(h), int(zmode),
-                   int(it0), int(it1), Y, uY, gains, Z.ctypes.data, ng.ctypes.data)
+    lib().or_run_g(n, d, P.rowptr, P.col, P.val, float(alpha0), int(early_iter), float(h), int(zmode),
+                   int(fixed_ng), int(it0), int(it1), Y, uY, gains, Z.ctypes.data, ng.ctypes.data)
     return Z[:cnt], ng[:cnt]

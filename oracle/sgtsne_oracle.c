@@ -677,6 +677,11 @@ double or_repulsive_interp(int n, int d, double *y, double h, double *Frep, int 
 }
 
 double or_repulsive_interp_z(int n, int d, double *y, double h, int zmode, double *Frep, int *ng_out) {
+  return or_repulsive_interp_g(n, d, y, h, zmode, 0, Frep, ng_out);
+}
+
+double or_repulsive_interp_g(int n, int d, double *y, double h, int zmode, int fixed_ng, double *Frep,
+                             int *ng_out) {
   const int m = d + 1;
   double miny[3] = {INFINITY, INFINITY, INFINITY};
   for (int i = 0; i < n; i++)
@@ -685,9 +690,15 @@ double or_repulsive_interp_z(int n, int d, double *y, double h, int zmode, doubl
   for (size_t t = 0; t < (size_t)n * d; t++) y[t] -= miny[t % d];
   double maxy = 0.0;
   for (size_t t = 0; t < (size_t)n * d; t++) maxy = maxy < y[t] ? y[t] : maxy;
-  int nGrid = (int)ceil(maxy / h);
-  if (nGrid < 14) nGrid = 14;
-  nGrid = or_best_grid_size(nGrid);
+  int nGrid;
+  if (fixed_ng > 0) {
+    /* fixed grid: n1 = getBestGridSize(params.ng), CU/gradient_descend.cu:139-140 */
+    nGrid = or_best_grid_size(fixed_ng);
+  } else {
+    nGrid = (int)ceil(maxy / h);
+    if (nGrid < 14) nGrid = 14;
+    nGrid = or_best_grid_size(nGrid);
+  }
   const int ng = nGrid + 2;
   if (ng_out) *ng_out = ng;
 
@@ -809,6 +820,16 @@ void or_run(int n, int d, const int64_t *rowptr, const int32_t *col, const doubl
 void or_run_z(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
               double alpha0, int early_iter, double h, int zmode, int it0, int it1,
               double *Y, double *uY, double *gains, double *zeta_out, int *ng_out) {
+  or_run_g(n, d, rowptr, col, val, alpha0, early_iter, h, zmode, 0, it0, it1, Y, uY, gains, zeta_out, ng_out);
+}
+
+void or_run_g(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
+              double alpha0, int early_iter, double h, int zmode, int fixed_ng, int it0, int it1,
+              double *Y, double *uY, double *gains, double *zeta_out, int *ng_out) {
+  /* kl_minimization, HY/gradient_descendHYB.cpp:160-225 / CU/gradient_descend.cu:158-236:
+   * iteration `it` runs with the alpha and momentum in force, and the switch
+   * happens after the update of iteration stop_lying_iter (resp. 250), so
+   * iterations 0..early_iter use alpha0 and 0..250 use momentum 0.5 */
   const int mom_switch_iter = 250;
   const double eta = 200.0;
   double *Fattr = (double *)malloc(sizeof(double) * (size_t)n * d);
@@ -817,7 +838,7 @@ void or_run_z(int n, int d, const int64_t *rowptr, const int32_t *col, const dou
     double alpha = it <= early_iter ? alpha0 : 1.0;
     double momentum = it <= mom_switch_iter ? 0.5 : 0.8;
     int ng = 0;
-    double Z = or_repulsive_interp_z(n, d, Y, h, zmode, Frep, &ng);
+    double Z = or_repulsive_interp_g(n, d, Y, h, zmode, fixed_ng, Frep, &ng);
     or_attractive(n, d, rowptr, col, val, Y, Fattr);
     or_update(n, d, Fattr, Frep, alpha, momentum, eta, Y, uY, gains);
     if (zeta_out) zeta_out[it - it0] = Z;

@@ -104,6 +104,12 @@ double or_repulsive_interp(int n, int d, double *y, double h, double *Frep, int 
  * the same in both modes (only its normalisation differs). */
 double or_repulsive_interp_z(int n, int d, double *y, double h, int zmode, double *Frep, int *ng_out);
 
+/* As or_repulsive_interp_z; fixed_ng > 0 selects the CUDA path's fixed grid
+ * (nGrid = or_best_grid_size(fixed_ng), CU/gradient_descend.cu:139-140 with
+ * the scaling of CU/nuconv.cu:55-75) instead of the h rule. */
+double or_repulsive_interp_g(int n, int d, double *y, double h, int zmode, int fixed_ng, double *Frep,
+                             int *ng_out);
+
 /* Gradient + update with mean-centring: HY/gradient_descendHYB.cpp:29-80. */
 void or_update(int n, int d, const double *Fattr, const double *Frep, double alpha,
                double momentum, double eta, double *Y, double *uY, double *gains);
@@ -123,6 +129,11 @@ void or_run(int n, int d, const int64_t *rowptr, const int32_t *col, const doubl
 
 void or_run_z(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
               double alpha0, int early_iter, double h, int zmode, int it0, int it1,
+              double *Y, double *uY, double *gains, double *zeta_out, int *ng_out);
+
+/* As or_run_z with the grid rule of or_repulsive_interp_g. */
+void or_run_g(int n, int d, const int64_t *rowptr, const int32_t *col, const double *val,
+              double alpha0, int early_iter, double h, int zmode, int fixed_ng, int it0, int it1,
               double *Y, double *uY, double *gains, double *zeta_out, int *ng_out);
 
 /* Threads used by OpenMP regions (0 = leave the runtime default). */

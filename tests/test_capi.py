@@ -134,3 +134,114 @@ def test_locality_order_components_and_isolated(sg):
     col = np.array([1, 0, 4, 3], np.int32)
     perm = sg.locality_order(sg.CSR(6, rp, col, np.ones(4, np.float32)))
     assert np.array_equal(np.sort(perm), np.arange(6))
+
+
+_HYB_PROG = r'''
+#include "sgtsne.hpp"
+#include <cstdio>
+#include <vector>
+// reads n, nnz, CSC offsets (uint32), row ids (uint32), values (double) from
+// argv[1]; mode "stoch": writes makeStochastic's values and count to argv[2];
+// mode "run": writes sgtsne_hyb::sgtsne's Y (d = 2, 60 iterations, fixed
+// start) to argv[2]
+int main(int argc, char **argv) {
+  FILE *f = std::fopen(argv[1], "rb");
+  int n, nnz;
+  if (std::fread(&n, 4, 1, f) != 1 || std::fread(&nnz, 4, 1, f) != 1) return 2;
+  std::vector<uint32_t> col(n + 1), row(nnz);
+  std::vector<double> val(nnz);
+  if (std::fread(col.data(), 4, n + 1, f) != (size_t)n + 1) return 2;
+  if (std::fread(row.data(), 4, nnz, f) != (size_t)nnz) return 2;
+  if (std::fread(val.data(), 8, nnz, f) != (size_t)nnz) return 2;
+  std::fclose(f);
+  sgtsne_hyb::sparse_matrix P{n, n, nnz, row.data(), col.data(), val.data()};
+  FILE *o = std::fopen(argv[2], "wb");
+  if (std::string(argv[3]) == "stoch") {
+    uint32_t k = sgtsne_hyb::makeStochastic(P);
+    std::fwrite(&k, 4, 1, o);
+    std::fwrite(val.data(), 8, nnz, o);
+  } else {
+    sgtsne_hyb::tsneparams p;
+    p.maxIter = 60;
+    p.earlyIter = 20;
+    std::vector<double> y0(2 * (size_t)n);
+    for (int i = 0; i < 2 * n; i++) y0[i] = 1e-3 * ((i * 7919) % 1000 - 500) / 500.0;
+    double *y = sgtsne_hyb::sgtsne(P, p, y0.data());
+    std::fwrite(y, 8, 2 * (size_t)n, o);
+    std::free(y);
+  }
+  std::fclose(o);
+  return 0;
+}
+'''
+
+
+def _hyb_exe(sg, tmp_path):
+    src = tmp_path / "hyb.cpp"
+    src.write_text(_HYB_PROG)
+    exe = tmp_path / "hyb"
+    libdir = os.path.dirname(sg.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir,
+                    "-lsgtsne_hip", "-o", str(exe)], check=True)
+    return exe, dict(os.environ, LD_LIBRARY_PATH=libdir)
+
+
+def _write_csc(path, n, colptr, rows, vals):
+    with open(path, "wb") as f:
+        np.array([n, rows.size], np.int32).tofile(f)
+        colptr.astype(np.uint32).tofile(f)
+        rows.astype(np.uint32).tofile(f)
+        vals.astype(np.float64).tofile(f)
+
+
+def _weighted_graph(n, k, seed):
+    # a general weighted kNN-like CSC (not stochastic): column j holds k
+    # entries of positive weight, plus an all-zero column and a stochastic one
+    rng = np.random.default_rng(seed)
+    colptr = np.arange(0, n * k + 1, k)
+    rows = np.concatenate([np.sort(rng.choice(np.delete(np.arange(n), j), k, replace=False)) for j in range(n)])
+    vals = rng.uniform(0.1, 3.0, n * k)
+    vals[k * 3:k * 4] = 0.0                                        # zero-sum column 3: left alone
+    vals[k * 5:k * 6] = 1.0 / k                                    # column 5 already stochastic
+    return colptr, rows, vals
+
+
+def test_hyb_make_stochastic_matches_oracle(sg, O, tmp_path):
+    # sgtsne_hyb::makeStochastic (HY/sparsematrix.cpp:27-67) against the
+    # oracle's restatement, on the CSC offsets as the oracle's row pointers
+    n, k = 200, 12
+    colptr, rows, vals = _weighted_graph(n, k, 3)
+    _write_csc(tmp_path / "P.bin", n, colptr, rows, vals)
+    exe, env = _hyb_exe(sg, tmp_path)
+    subprocess.run([str(exe), str(tmp_path / "P.bin"), str(tmp_path / "out.bin"), "stoch"], env=env, check=True)
+    out = np.fromfile(tmp_path / "out.bin", np.uint8)
+    cnt = int(out[:4].view(np.uint32)[0])
+    got = out[4:].view(np.float64)
+    want = vals.copy()
+    cnt_o = O.lib().or_make_stochastic(n, colptr.astype(np.int64), want)
+    assert cnt == cnt_o >= 1
+    assert np.array_equal(got, want)
+    sums = np.add.reduceat(got, colptr[:-1])
+    assert np.allclose(np.delete(sums, 3), 1.0, atol=1e-12) and sums[3] == 0.0
+
+
+@pytest.mark.gpu
+def test_hyb_entry_normalises_like_the_reference(sg, tmp_path):
+    # the HYB drop-in on a non-stochastic P gives the same embedding as on the
+    # same P normalised beforehand (HY/sgtsne.cpp:60), bit for bit.  The
+    # column sums are sequential FP64 sums, as in the reference's loop
+    # (np.cumsum adds in order), so the normalised columns sum to 1 within
+    # 1e-12 and the entry leaves them as they are
+    n, k = 1500, 20
+    colptr, rows, vals = _weighted_graph(n, k, 4)
+    vals[k * 3:k * 4] = 0.5  # every column has a positive sum here
+    norm = vals / np.repeat(np.cumsum(vals.reshape(n, k), axis=1)[:, -1], k)
+    exe, env = _hyb_exe(sg, tmp_path)
+    ys = []
+    for tag, v in (("raw", vals), ("norm", norm)):
+        _write_csc(tmp_path / f"{tag}.bin", n, colptr, rows, v)
+        subprocess.run([str(exe), str(tmp_path / f"{tag}.bin"), str(tmp_path / f"{tag}.y"), "run"], env=env,
+                       check=True, timeout=120)
+        ys.append(np.fromfile(tmp_path / f"{tag}.y", np.float64))
+    assert np.all(np.isfinite(ys[0]))
+    assert np.array_equal(ys[0], ys[1])

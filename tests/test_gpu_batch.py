@@ -64,6 +64,48 @@ def test_batch_terms_vs_oracle(sg, O, d):
             assert abs(Z - Zo) / Zo < TOL
 
 
+# every shape the batch autotuner times (unroll, rows per work-group, points
+# of Y in LDS; batch.hip tune()); unroll 8 and 16 are the masked-step forms
+_BATCH_CANDIDATES = [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (16, 0, 0), (2, 32, 0), (2, 64, 0), (1, 64, 0),
+                     (2, 128, 0), (1, 128, 0), (2, 256, 0), (1, 256, 4096), (2, 512, 4096), (1, 512, 8192),
+                     (2, 1024, 8192)]
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_batch_attractive_candidates_vs_oracle(sg, O, d):
+    # each candidate pinned in turn: every job's Fattr against the FP64 oracle,
+    # and the same bits for every candidate (the lanes per row are fixed)
+    sizes = (1301, 4099, 2203)
+    rng = np.random.default_rng(60 + d)
+    with _batch(sg, d=d) as B:
+        jobs = []
+        for t, n in enumerate(sizes):
+            S, So = _job(sg, O, n, 300 * d + t)
+            jobs.append((B.add(S, symmetrize=False, y_in=_clustered(n, d, rng)), So))
+        B.step(1)
+        Yc = {j: B.get_Y(j).astype(np.float64) for j, _ in jobs}
+        ref = {j: O.attractive(So, Yc[j]) for j, So in jobs}
+        first = None
+        for cand in _BATCH_CANDIDATES:
+            u, rpb, win = cand
+            if win * (4 if d == 3 else d) * 4 > 65536:
+                with pytest.raises(sg.SgtsneError):
+                    B.set_attractive(u, rpb, win)
+                continue
+            B.set_attractive(u, rpb, win)
+            t = B.tuning()
+            assert (t["unroll"], t["rows_per_block"], t["window"]) == cand
+            got = {}
+            for j, _ in jobs:
+                Fa, _, _, _ = B.terms(j)
+                assert rel_l2(Fa, ref[j]) < TOL, (cand, j)
+                got[j] = Fa
+            if first is None:
+                first = got
+            for j in got:
+                assert np.array_equal(got[j], first[j]), (cand, j)
+
+
 def test_batch_algebraic_zeta(sg, O):
     S, So = _job(sg, O, 2000, 5)
     Y = _clustered(2000, 2, np.random.default_rng(3))

@@ -205,16 +205,6 @@ def _terms_oracle(O, S, Y, h, zmode=None):
     return Fa, Fr, Z, ng
 
 
-_GOLDEN_UPLOAD_CRASH = (
-    "this case ended the test process twice with a host segfault inside upload_P: the locality "
-    "order read P back with a null-stream copy that did not wait for the upload queued on the "
-    "context's non-blocking stream, so the sweep indexed host arrays with stale device data. "
-    "The read-back is now stream-ordered and validated (an inconsistent P is an error), but that "
-    "fix has not been run on a GPU yet; test_terms_golden_caller_order covers the same golden "
-    "vectors without the locality order")
-
-
-@pytest.mark.skip(reason=_GOLDEN_UPLOAD_CRASH)
 @pytest.mark.parametrize("fmt", ["CSR_SEG", "CSR", "COO", "HYBRID"])
 @pytest.mark.parametrize("zeta", ["DIRECT", "ALGEBRAIC"])
 def test_terms_golden(sg, mnist, fmt, zeta):
@@ -273,6 +263,31 @@ def test_terms_random(sg, O, d, fmt):
             Fao, Fro, Zo, ngo = _terms_oracle(O, So, Yc, h)
             assert ng == ngo
             assert rel_l2(Fa, Fao) < TOL
+            assert rel_l2(Fr, Fro) < TOL
+            assert abs(Z - Zo) / Zo < TOL
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_terms_fixed_grid(sg, O, d):
+    # the CUDA path's default grid rule (CU/gradient_descend.cu:139-140): the
+    # grid side is getBestGridSize(params.ng) + 2 whatever the extent
+    n = 4000
+    X = _mixture(n, 30, 6, 50 + d)
+    I, D = O.knn_exact(X, 60)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 20.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(70 + d)
+    ngf = {1: 50, 2: 50, 3: 30}[d]
+    cases = [(rng.standard_normal((n, d)) * 1e-3).astype(np.float32), _clustered_embedding(n, d, rng)]
+    with sg.Context(sg.tsneparams(d=d, grid_mode=sg.GRID_FIXED, ng=ngf)) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        for Y in cases:
+            ctx.set_Y(Y)
+            Yc = ctx.get_Y()
+            Fa, Fr, Z, ng = ctx.terms()
+            Fro, Zo, ngo = O.repulsive_interp(Yc.astype(np.float64), 0.7, O.ZETA_DIRECT, fixed_ng=ngf)
+            assert ng == ngo == O.best_grid_size(ngf) + 2
+            assert rel_l2(Fa, O.attractive(So, Yc.astype(np.float64))) < TOL
             assert rel_l2(Fr, Fro) < TOL
             assert abs(Z - Zo) / Zo < TOL
 
@@ -455,14 +470,21 @@ def test_reorder_modes_agree(sg, O):
 
 # every shape of the default CSR kernel: same result as the oracle, and the
 # same bits for every shape with the same lanes per row
+# (lanes per row, unroll, rows per work-group, points of Y in LDS); unroll 8
+# and 16 are the masked-step forms, whose last step of a row is partly masked
 _SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (64, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0, 0), (16, 1, 64, 0),
            (16, 2, 128, 0), (16, 1, 256, 4096), (16, 2, 512, 4096), (16, 2, 1024, 8192), (32, 2, 256, 0),
-           (32, 1, 512, 4096), (16, 0, 0, -1), (32, 0, 0, -1), (64, 0, 0, -1)]
+           (32, 1, 512, 4096), (16, 0, 0, -1), (32, 0, 0, -1), (64, 0, 0, -1),
+           (16, 8, 0, 0), (32, 8, 0, 0), (64, 8, 0, 0), (16, 16, 0, 0), (32, 16, 0, 0),
+           (32, 2, 0, 0), (32, 4, 0, 0), (64, 2, 0, 0), (32, 1, 64, 0), (32, 2, 64, 0), (32, 2, 1024, 8192), (64, 1, 256, 0),
+           (16, 1, 512, 8192)]
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_attractive_kernel_shapes(sg, O, d):
-    n = 12000
+    # n is not a multiple of any work-group's rows, and rows have 30-90
+    # entries: short rows end inside a masked step, long ones take several
+    n = 12007
     X = _mixture(n, 20, 8, 90 + d)
     I, D = O.knn_exact(X, 30)
     So = O.symmetrize(O.perplexity_equalize(I, D, 10.0))

@@ -371,3 +371,93 @@ def test_golden_fixture_regression(O):
     Y = g["ys_Y"].astype(np.float64)
     Fr, Z, ng = O.repulsive_interp(Y.copy(), 0.7)
     assert ng == g["ys_ng"] and abs(Z - g["ys_Z"]) <= 1e-9 * Z
+
+
+def _schedule_case(O, n=300, seed=81):
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(0, 6, size=(3, 10))
+    X = (centres[rng.integers(0, 3, n)] + rng.standard_normal((n, 10))).astype(np.float32)
+    I, D = O.knn_exact(X, 20)
+    return O.symmetrize(O.perplexity_equalize(I, D, 7.0)), rng
+
+
+def _hand_steps(O, S, Y, uY, g, it0, it1, alpha0, early, h=0.7, alpha_at=None, mom_at=250):
+    # the reference's control flow (HY/gradient_descendHYB.cpp:203-222,
+    # CU/gradient_descend.cu:215-236): gradient and update with the current
+    # alpha and momentum, THEN "if (iter == stop_lying_iter) alpha = 1" and
+    # "if (iter == mom_switch_iter) momentum = final_momentum"
+    alpha_at = early if alpha_at is None else alpha_at
+    alpha, mom = alpha0, 0.5
+    for it in range(it1):
+        if it >= it0:
+            Fr, _, _ = O.repulsive_interp(Y, h, O.ZETA_ALGEBRAIC)
+            Fa = O.attractive(S, Y)
+            O.update(Fa, Fr, alpha, mom, 200.0, Y, uY, g)
+        if it == alpha_at:
+            alpha = 1.0
+        if it == mom_at:
+            mom = 0.8
+
+
+def test_run_schedule_alpha_switch(O):
+    # or_run's exaggeration switch against the hand-stepped reference loop;
+    # moving the switch by one iteration either way is detected
+    S, rng = _schedule_case(O)
+    y0 = rng.standard_normal((S.n, 2))
+    runs = {}
+    for tag, at in (("ref", 3), ("early", 2), ("late", 4)):
+        Y, uY, g = y0.copy(), np.zeros_like(y0), np.ones_like(y0)
+        _hand_steps(O, S, Y, uY, g, 0, 7, 12.0, 3, alpha_at=at)
+        runs[tag] = Y
+    Y, uY, g = y0.copy(), np.zeros_like(y0), np.ones_like(y0)
+    O.run(S, Y, uY, g, 0, 7, alpha0=12.0, early_iter=3)
+    assert rel_l2(Y, runs["ref"]) < 1e-13
+    assert rel_l2(runs["early"], runs["ref"]) > 1e-6
+    assert rel_l2(runs["late"], runs["ref"]) > 1e-6
+
+
+def test_run_schedule_momentum_switch(O):
+    # the momentum switch at iteration 250, over the window 247..253, from a
+    # state with nonzero velocity; a switch one iteration off is detected
+    S, rng = _schedule_case(O, seed=82)
+    Y = 1e-2 * rng.standard_normal((S.n, 2))
+    uY, g = np.zeros_like(Y), np.ones_like(Y)
+    O.run(S, Y, uY, g, 0, 247, alpha0=4.0, early_iter=20)
+    assert np.abs(uY).max() > 0
+    state = (Y.copy(), uY.copy(), g.copy())
+    runs = {}
+    for tag, at in (("ref", 250), ("early", 249), ("late", 251)):
+        Yh, uh, gh = (a.copy() for a in state)
+        _hand_steps(O, S, Yh, uh, gh, 247, 254, 4.0, 20, mom_at=at)
+        runs[tag] = Yh
+    Yo, uo, go = (a.copy() for a in state)
+    O.run(S, Yo, uo, go, 247, 254, alpha0=4.0, early_iter=20)
+    assert rel_l2(Yo, runs["ref"]) < 1e-13
+    assert rel_l2(runs["early"], runs["ref"]) > 1e-7
+    assert rel_l2(runs["late"], runs["ref"]) > 1e-7
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_fixed_grid_mode(O, d):
+    # CU/gradient_descend.cu:139-140: nGrid = getBestGridSize(params.ng), so
+    # params.ng = 50 gives a 52-node grid whatever the extent; at the same
+    # nGrid the h rule gives the same numbers, and on the reference's
+    # acceptance inputs (Y ~ U[0,1]^d) it meets the bar against the exact sum
+    rng = np.random.default_rng(90 + d)
+    n = 1500
+    Y = rng.uniform(0, 1, size=(n, d))
+    ng_fixed = {1: 50, 2: 50, 3: 20}[d]
+    F, Z, ng = O.repulsive_interp(Y.copy(), 0.7, O.ZETA_ALGEBRAIC, fixed_ng=ng_fixed)
+    want_grid = next(s for s in GRID_LIST if ng_fixed + 2 <= s)
+    assert ng == want_grid
+    maxy = (Y - Y.min(0)).max()
+    h_same = maxy / (want_grid - 2 - 0.5)  # ceil(maxy / h) = nGrid
+    F2, Z2, ng2 = O.repulsive_interp(Y.copy(), h_same, O.ZETA_ALGEBRAIC)
+    assert ng2 == ng and np.array_equal(F, F2) and Z == Z2
+    Fe, Ze = _exact_np(Y)
+    assert np.abs(F - Fe).max() <= 1e-7 and abs(Z - Ze) / Ze <= 1e-4
+    # a wide embedding keeps the fixed grid (the h rule would grow it)
+    Yw = Y * (30.0 if d == 3 else 60.0)
+    _, _, ngw = O.repulsive_interp(Yw.copy(), 0.7, O.ZETA_ALGEBRAIC, fixed_ng=ng_fixed)
+    _, _, ngh = O.repulsive_interp(Yw.copy(), 0.7, O.ZETA_ALGEBRAIC)
+    assert ngw == want_grid and ngh > want_grid
