@@ -1295,11 +1295,16 @@ int sgtsne_run(int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, co
   std::vector<float> y0;
   const float *yi = y_in;
   if (st == SGTSNE_OK && !y_in) {
-    // N(0, 1e-4^2) start (CU/utils.cu:185-192), seeded
-    std::mt19937_64 gen(params->seed);
-    std::normal_distribution<double> nd(0.0, 1.0);
+    // N(0, 1e-4^2) start: seed 0 draws the reference's own sequence
+    // (CU/utils.cu:185-192); another seed draws from a seeded mt19937_64
     y0.resize((size_t)n * params->d);
-    for (auto &v : y0) v = (float)(1e-4 * nd(gen));
+    if (params->seed == 0) {
+      st = sgtsne_random_start(n, params->d, params->layout, y0.data());
+    } else {
+      std::mt19937_64 gen(params->seed);
+      std::normal_distribution<double> nd(0.0, 1.0);
+      for (auto &v : y0) v = (float)(1e-4 * nd(gen));
+    }
     yi = y0.data();
   }
   if (st == SGTSNE_OK) st = sgtsne_ctx_set_Y(c, yi, params->layout, 1);

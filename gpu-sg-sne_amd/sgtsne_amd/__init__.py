@@ -47,6 +47,9 @@ EXPORTED = [
     "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
     "sgtsne_batch_set_graph", "sgtsne_batch_set_attractive", "sgtsne_copy_probe", "sgtsne_locality_order",
+    "sgtsne_free", "sgtsne_read_X_text", "sgtsne_read_P_mtx", "sgtsne_write_embedding_text",
+    "sgtsne_write_embedding_text_f64", "sgtsne_write_embedding_bin", "sgtsne_write_time_info",
+    "sgtsne_simulation_data", "sgtsne_random_start",
 ]
 
 
@@ -143,6 +146,16 @@ def lib():
         "sgtsne_batch_set_attractive": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "sgtsne_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
         "sgtsne_locality_order": (C.c_int, [C.c_int, i32p, i32p, i32p]),
+        "sgtsne_free": (None, [vp]),
+        "sgtsne_read_X_text": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(vp)]),
+        "sgtsne_read_P_mtx": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                        C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+        "sgtsne_write_embedding_text": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int, C.c_int]),
+        "sgtsne_write_embedding_text_f64": (C.c_int, [C.c_char_p, f64p, C.c_int, C.c_int, C.c_int]),
+        "sgtsne_write_embedding_bin": (C.c_int, [C.c_char_p, f64p, C.c_int, C.c_int]),
+        "sgtsne_write_time_info": (C.c_int, [C.c_char_p, f64p, C.c_int, C.c_int, C.c_int]),
+        "sgtsne_simulation_data": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p]),
+        "sgtsne_random_start": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -170,6 +183,74 @@ def copy_probe(nbytes, reps=10):
     ms = C.c_double()
     _check(lib().sgtsne_copy_probe(int(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def _take(ptr, count, ctype, dtype):
+    """Copies a library-allocated array into numpy and frees it."""
+    try:
+        if count == 0:
+            return np.zeros(0, dtype)
+        buf = C.cast(ptr, C.POINTER(ctype * count)).contents
+        return np.frombuffer(buf, dtype=dtype, count=count).copy()
+    finally:
+        lib().sgtsne_free(ptr)
+
+
+def read_X_text(path, n=0):
+    """Points file of the reference's demo (CU/utils.cu:216-256): float32 [n][dim]."""
+    nn, dim, ptr = C.c_int(int(n)), C.c_int(), C.c_void_p()
+    _check(lib().sgtsne_read_X_text(os.fsencode(path), C.byref(nn), C.byref(dim), C.byref(ptr)))
+    return _take(ptr, nn.value * dim.value, C.c_float, np.float32).reshape(nn.value, dim.value)
+
+
+def read_P_mtx(path):
+    """Matrix Market graph (CU/utils.cu:335-409): (m, n, colptr, row, val) in
+    compressed columns, 0-based."""
+    m, n, nnz = C.c_int(), C.c_int(), C.c_int64()
+    cp, ro, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().sgtsne_read_P_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), C.byref(cp),
+                                   C.byref(ro), C.byref(va)))
+    colptr = _take(cp, n.value + 1, C.c_int32, np.int32)
+    row = _take(ro, nnz.value, C.c_int32, np.int32)
+    val = _take(va, nnz.value, C.c_double, np.float64)
+    return m.value, n.value, colptr, row, val
+
+
+def write_embedding_text(path, Y, layout=LAYOUT_AOS):
+    """One line per point, '%.6g ' per coordinate (the reference's writers).
+    Y is [n][d] for LAYOUT_AOS and [d][n] for LAYOUT_SOA."""
+    Y = np.ascontiguousarray(Y)
+    n, d = (Y.shape if layout == LAYOUT_AOS else Y.shape[::-1])
+    if Y.dtype == np.float64:
+        _check(lib().sgtsne_write_embedding_text_f64(os.fsencode(path), Y.reshape(-1), n, d, int(layout)))
+    else:
+        Y = Y.astype(np.float32, copy=False)
+        _check(lib().sgtsne_write_embedding_text(os.fsencode(path), Y.reshape(-1), n, d, int(layout)))
+
+
+def write_embedding_bin(path, Y):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    _check(lib().sgtsne_write_embedding_bin(os.fsencode(path), Y.reshape(-1), Y.shape[0], Y.shape[1]))
+
+
+def write_time_info(path, time_info, n, d, iters):
+    ti = np.ascontiguousarray(time_info, dtype=np.float64)
+    assert ti.size >= 7 * iters + 3
+    _check(lib().sgtsne_write_time_info(os.fsencode(path), ti, int(n), int(d), int(iters)))
+
+
+def simulation_data(n, dim, clusters):
+    """The reference's synthetic points for its -s option (CU/utils.cu:310-332)."""
+    X = np.empty((n, dim), np.float32)
+    _check(lib().sgtsne_simulation_data(int(n), int(dim), int(clusters), X.reshape(-1)))
+    return X
+
+
+def random_start(n, d, layout=LAYOUT_AOS):
+    """The reference's N(0, 1e-4^2) start (CU/utils.cu:185-192)."""
+    y = np.empty((n, d) if layout == LAYOUT_AOS else (d, n), np.float32)
+    _check(lib().sgtsne_random_start(int(n), int(d), int(layout), y.reshape(-1)))
+    return y
 
 
 def version():

@@ -82,7 +82,7 @@ typedef struct sgtsne_params {
   int ng;             /* grid request for SGTSNE_GRID_FIXED             (50)  */
   int grid_mode;      /* SGTSNE_GRID_*                          (H)           */
   int layout;         /* SGTSNE_LAYOUT_* of y_in / y_out        (SOA)         */
-  uint64_t seed;      /* seed of the N(0, 1e-4^2) start when y_in is NULL     */
+  uint64_t seed;      /* start when y_in is NULL: 0 = the reference sequence, else a seeded N(0, 1e-4^2) */
   int ell_width;      /* hybrid ELL width; <= 0 = automatic                   */
   int zeta_mode;      /* SGTSNE_ZETA_*                          (DIRECT)      */
   int reorder;        /* device order of the points (the API always reads and
@@ -251,6 +251,60 @@ int sgtsne_ctx_destroy(sgtsne_ctx *ctx);
  * batch (bit-identical alone and in any mix).  The reference has no batched
  * form: it runs one sgtsneCUDA call (CU/sgtsne.cu:26-128) per embedding. */
 typedef struct sgtsne_batch sgtsne_batch;
+
+/* ---- files and start data of the reference's command-line programs ----
+ * Host only: none of these touches the device. */
+
+/* Frees an array returned by the readers below. */
+void sgtsne_free(void *p);
+
+/* Point file: readXfromMTX (CU/utils.cu:216-256).  Text, a header "n dim"
+ * then n*dim values, point after point, separated by any whitespace.  A
+ * positive *n on entry reads only the first *n points (the reference's -n
+ * option).  *X receives a malloc'ed float [n][dim] (free with sgtsne_free);
+ * *n and *dim receive the sizes read. */
+int sgtsne_read_X_text(const char *path, int *n, int *dim, float **X);
+
+/* Matrix Market graph: buildPFromMTX (CU/utils.cu:335-409, HY/utils.cpp:
+ * 104-170).  Lines starting with '%' are skipped, then "m n nnz" and nnz
+ * 1-based "row col value" triplets.  Returns P in compressed columns, as the
+ * reference does: colptr [n+1], row [nnz] (0-based) and val [nnz], entries of
+ * a column in file order.  Read as compressed rows, the same arrays hold P^T,
+ * which is what the drop-in sgtsne_run takes for a column-stochastic P (the
+ * HYB convention).  Arrays are malloc'ed (free with sgtsne_free). */
+int sgtsne_read_P_mtx(const char *path, int *m, int *n, int64_t *nnz, int32_t **colptr, int32_t **row,
+                      double **val);
+
+/* Embedding text: extractEmbeddingTextT (CU/utils.cu:121-138, y SoA [d][N])
+ * and extractEmbeddingText (HY/utils.cpp:180-196, y AoS [N][d]).  One line
+ * per point, each coordinate written as std::ostream writes it by default
+ * (printf "%.6g") followed by one space. */
+int sgtsne_write_embedding_text(const char *path, const float *y, int n, int d, int layout);
+int sgtsne_write_embedding_text_f64(const char *path, const double *y, int n, int d, int layout);
+
+/* Embedding binary: extractEmbedding (HY/utils.cpp:199-206), n*d doubles
+ * row-major. */
+int sgtsne_write_embedding_bin(const char *path, const double *y, int n, int d);
+
+/* Timing report: ExtractTimeInfo (CU/utils.cu:72-102) for a time_info array
+ * of 7*iters+3 doubles (sgtsne_run fills 7 per iteration and the loop time at
+ * 7*iters; the caller adds the matrix and total times at 7*iters+1, +2).
+ * As in the reference, the report is appended only when the file already
+ * exists; otherwise an empty file is created. */
+int sgtsne_write_time_info(const char *path, const double *time_info, int n, int d, int iters);
+
+/* The reference's synthetic points for its -s option: loadSimulationData
+ * (CU/utils.cu:310-332), n_clusters groups of consecutive points whose
+ * coordinate a is drawn from N(20, 2) if a >= the group index, else from
+ * N(-20, 2), with one default-seeded std::default_random_engine.  X is
+ * float [n][dim]. */
+int sgtsne_simulation_data(int n, int dim, int n_clusters, float *X);
+
+/* The reference's start embedding for y_in == NULL: generateRandomGaussianCoord
+ * (CU/utils.cu:185-192), 1e-4 * N(0, 1) from a default-seeded
+ * std::default_random_engine, drawn in [d][N] order.  sgtsne_run uses it when
+ * params.seed is 0. */
+int sgtsne_random_start(int n, int d, int layout, float *y);
 
 /* The locality order the engine applies at upload (reorder >= 1): reverse
  * Cuthill-McKee of the graph of P (rowptr [n+1], col [nnz], host),

@@ -35,6 +35,7 @@ def lib():
     sig = {
         "or_best_grid_size": (C.c_int, [C.c_int]),
         "or_knn_exact": (None, [_f32p, C.c_int, C.c_int, C.c_int, _i32p, _f64p]),
+        "or_knn_rows": (None, [_f32p, C.c_int, C.c_int, C.c_int, _i32p, C.c_int, _i32p, _f64p]),
         "or_equalize_vertex": (C.c_int, [_f64p, _f64p, C.c_double, C.c_int, C.POINTER(C.c_double)]),
         "or_perplexity_equalize": (None, [_i32p, _f64p, C.c_int, C.c_int, C.c_double, _i64p, _i32p, _f64p]),
         "or_make_stochastic": (C.c_int, [C.c_int, _i64p, _f64p]),
@@ -118,6 +119,17 @@ def knn_exact(X, k):
     I = np.empty((n, k + 1), np.int32)
     D = np.empty((n, k + 1), np.float64)
     lib().or_knn_exact(X, n, dim, k, I, D)
+    return I, D
+
+
+def knn_rows(X, k, rows):
+    """knn_exact's rows for the query points `rows` only."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    n, dim = X.shape
+    I = np.empty((rows.size, k + 1), np.int32)
+    D = np.empty((rows.size, k + 1), np.float64)
+    lib().or_knn_rows(X, n, dim, k, rows, rows.size, I, D)
     return I, D
 
 

@@ -66,67 +66,82 @@ static inline int key_less(double da, int ia, double db, int ib) {
   return da < db || (da == db && ia < ib);
 }
 
+/* One exact row: the k nearest (squared distance, index) keys of point i
+ * among all others, ascending, written after the point itself. */
+static void knn_row(const float *X, int n, int dim, int k, int i, double *hd, int *hi, int32_t *Ii, double *Di) {
+  const float *xi = X + (size_t)i * dim;
+  int cnt = 0; /* max-heap of the k best (worst at root) */
+  for (int j = 0; j < n; j++) {
+    if (j == i) continue;
+    const float *xj = X + (size_t)j * dim;
+    double acc = 0.0;
+    for (int l = 0; l < dim; l++) {
+      double df = (double)xi[l] - (double)xj[l];
+      acc = fma(df, df, acc);
+    }
+    if (cnt < k) {
+      int c = cnt++;
+      hd[c] = acc; hi[c] = j;
+      while (c > 0) {
+        int p = (c - 1) / 2;
+        if (key_less(hd[p], hi[p], hd[c], hi[c])) {
+          double td = hd[p]; hd[p] = hd[c]; hd[c] = td;
+          int ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
+          c = p;
+        } else break;
+      }
+    } else if (k > 0 && key_less(acc, j, hd[0], hi[0])) {
+      hd[0] = acc; hi[0] = j;
+      int c = 0;
+      for (;;) {
+        int l = 2 * c + 1, r = l + 1, m = c;
+        if (l < k && key_less(hd[m], hi[m], hd[l], hi[l])) m = l;
+        if (r < k && key_less(hd[m], hi[m], hd[r], hi[r])) m = r;
+        if (m == c) break;
+        double td = hd[m]; hd[m] = hd[c]; hd[c] = td;
+        int ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
+        c = m;
+      }
+    }
+  }
+  /* heap-sort ascending into the output row (self first) */
+  Ii[0] = i; Di[0] = 0.0;
+  for (int e = cnt - 1; e >= 0; e--) {
+    Ii[1 + e] = hi[0]; Di[1 + e] = hd[0];
+    hd[0] = hd[e]; hi[0] = hi[e];
+    int c = 0, sz = e;
+    for (;;) {
+      int l = 2 * c + 1, r = l + 1, m = c;
+      if (l < sz && key_less(hd[m], hi[m], hd[l], hi[l])) m = l;
+      if (r < sz && key_less(hd[m], hi[m], hd[r], hi[r])) m = r;
+      if (m == c) break;
+      double td = hd[m]; hd[m] = hd[c]; hd[c] = td;
+      int ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
+      c = m;
+    }
+  }
+}
+
 void or_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D) {
 #pragma omp parallel
   {
     double *hd = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1));
     int *hi = (int *)malloc(sizeof(int) * (size_t)(k > 0 ? k : 1));
 #pragma omp for schedule(dynamic, 16)
-    for (int i = 0; i < n; i++) {
-      const float *xi = X + (size_t)i * dim;
-      int cnt = 0; /* max-heap of the k best (worst at root) */
-      for (int j = 0; j < n; j++) {
-        if (j == i) continue;
-        const float *xj = X + (size_t)j * dim;
-        double acc = 0.0;
-        for (int l = 0; l < dim; l++) {
-          double df = (double)xi[l] - (double)xj[l];
-          acc = fma(df, df, acc);
-        }
-        if (cnt < k) {
-          int c = cnt++;
-          hd[c] = acc; hi[c] = j;
-          while (c > 0) {
-            int p = (c - 1) / 2;
-            if (key_less(hd[p], hi[p], hd[c], hi[c])) {
-              double td = hd[p]; hd[p] = hd[c]; hd[c] = td;
-              int ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
-              c = p;
-            } else break;
-          }
-        } else if (k > 0 && key_less(acc, j, hd[0], hi[0])) {
-          hd[0] = acc; hi[0] = j;
-          int c = 0;
-          for (;;) {
-            int l = 2 * c + 1, r = l + 1, m = c;
-            if (l < k && key_less(hd[m], hi[m], hd[l], hi[l])) m = l;
-            if (r < k && key_less(hd[m], hi[m], hd[r], hi[r])) m = r;
-            if (m == c) break;
-            double td = hd[m]; hd[m] = hd[c]; hd[c] = td;
-            int ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
-            c = m;
-          }
-        }
-      }
-      /* heap-sort ascending into the output row (self first) */
-      int32_t *Ii = I + (size_t)i * (k + 1);
-      double *Di = D + (size_t)i * (k + 1);
-      Ii[0] = i; Di[0] = 0.0;
-      for (int e = cnt - 1; e >= 0; e--) {
-        Ii[1 + e] = hi[0]; Di[1 + e] = hd[0];
-        hd[0] = hd[e]; hi[0] = hi[e];
-        int c = 0, sz = e;
-        for (;;) {
-          int l = 2 * c + 1, r = l + 1, m = c;
-          if (l < sz && key_less(hd[m], hi[m], hd[l], hi[l])) m = l;
-          if (r < sz && key_less(hd[m], hi[m], hd[r], hi[r])) m = r;
-          if (m == c) break;
-          double td = hd[m]; hd[m] = hd[c]; hd[c] = td;
-          int ti = hi[m]; hi[m] = hi[c]; hi[c] = ti;
-          c = m;
-        }
-      }
-    }
+    for (int i = 0; i < n; i++)
+      knn_row(X, n, dim, k, i, hd, hi, I + (size_t)i * (k + 1), D + (size_t)i * (k + 1));
+    free(hd); free(hi);
+  }
+}
+
+void or_knn_rows(const float *X, int n, int dim, int k, const int32_t *rows, int nrows, int32_t *I, double *D) {
+#pragma omp parallel
+  {
+    double *hd = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1));
+    int *hi = (int *)malloc(sizeof(int) * (size_t)(k > 0 ? k : 1));
+#pragma omp for schedule(dynamic, 1)
+    for (int r = 0; r < nrows; r++)
+      knn_row(X, n, dim, k, rows[r], hd, hi, I + (size_t)r * (k + 1), D + (size_t)r * (k + 1));
     free(hd); free(hi);
   }
 }

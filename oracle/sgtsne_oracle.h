@@ -32,6 +32,10 @@ int or_best_grid_size(int nGrid);
  * CU/allknn.cuh:75-92 / HY/demo_perplexity_equalization.cpp:24-45. */
 void or_knn_exact(const float *X, int n, int dim, int k, int32_t *I, double *D);
 
+/* The same rows for the query points rows[0..nrows) only (against all n
+ * points); row r of I and D belongs to point rows[r]. */
+void or_knn_rows(const float *X, int n, int dim, int k, const int32_t *rows, int nrows, int32_t *I, double *D);
+
 /* Perplexity bisection of one row: HY/sgtsne.cpp:201-254 (equalizeVertex).
  * dist has k+1 entries (self first, skipped).  Writes k row-normalised
  * values; returns the number of bisection iterations, beta in *beta. */

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "sgtsne.h")
 
 def declared_symbols():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sgtsne_\w+)\(", src, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*(sgtsne_\w+)\(", src, re.M)))
 
 
 def test_header_declares_binding_list(sg):

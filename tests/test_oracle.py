@@ -206,6 +206,15 @@ def test_knn_brute_force(O):
     assert np.allclose(D[:, 1:], np.take_along_axis(Dn, order, 1), rtol=1e-13)
 
 
+def test_knn_rows_equal_full_rows(O):
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((700, 12)).astype(np.float32)
+    I, D = O.knn_exact(X, 25)
+    rows = rng.choice(700, 40, replace=False)
+    Ir, Dr = O.knn_rows(X, 25, rows)
+    assert np.array_equal(Ir, I[rows]) and np.array_equal(Dr, D[rows])
+
+
 def test_knn_ties_lower_index_first(O):
     # duplicated points: equal distances are ordered by index
     X = np.zeros((6, 3), np.float32)
