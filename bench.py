@@ -100,6 +100,30 @@ def job_kl(sg, P, Y, d):
         return ctx.kl(1.0)
 
 
+def oracle_kl_ratios(P, y0, d, Ys, iters=1000):
+    """Final KL of each embedding in Ys over the FP64 oracle loop's final KL,
+    run from the same P (row-stochastic kNN CSR) and start y0."""
+    from oracle import oracle as O
+
+    S = O.symmetrize(O.CSR(P.n, P.rowptr.astype(np.int64), P.col, P.val.astype(np.float64)))
+    h = H_DEFAULT[d]
+
+    def kl_of(Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        _, Z, _ = O.repulsive_interp(Y.copy(), h, O.ZETA_DIRECT)
+        return O.kl(S, Y, Z)
+
+    Yo = np.ascontiguousarray(y0, dtype=np.float64).copy()
+    O.run(S, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, iters, h=h, zmode=O.ZETA_DIRECT)
+    klo = kl_of(Yo)
+    return [kl_of(Y) / klo for Y in Ys]
+
+
+def _ratio_stats(v):
+    return {"min": round(float(np.min(v)), 5), "mean": round(float(np.mean(v)), 5), "max": round(float(np.max(v)), 5),
+            "within_1pct": int(np.sum(np.abs(np.asarray(v) - 1.0) < 0.01)), "count": len(v)}
+
+
 def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
     """256 jobs (N log-uniform in [500, 50,000], 50-D 5..20-cluster mixtures)
     in one batched engine against the plain method one job at a time."""
@@ -129,14 +153,18 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
     order = np.argsort(sizes)
     pick = order[np.linspace(0, jobs - 1, sample).round().astype(int)]
     t_plain = 0.0
-    ratios = []
+    ratios, vs_oracle_b, vs_oracle_p = [], [], []
     for i in pick:
         X, _ = synth.gaussian_job(int(sizes[i]), seed=seeds[i])
         y0 = synth.random_init(int(sizes[i]), 2, seed=seeds[i])
         t0 = time.perf_counter()
         P, Yp = plain_job(sg, X, 2, k, u, y0)
         t_plain += time.perf_counter() - t0
-        ratios.append(job_kl(sg, P, B.get_Y(int(ids[i])), 2) / job_kl(sg, P, Yp, 2))
+        Yb = B.get_Y(int(ids[i]))
+        ratios.append(job_kl(sg, P, Yb, 2) / job_kl(sg, P, Yp, 2))
+        rb, rp = oracle_kl_ratios(P, y0, 2, [Yb, Yp])
+        vs_oracle_b.append(rb)
+        vs_oracle_p.append(rp)
     B.close()
     t_plain_all = t_plain * jobs / sample
     out = {"jobs": jobs, "points": total, "nnz_sym": info["nnz"], "jobs_done": done,
@@ -148,12 +176,14 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
            "plain_sample": f"{sample} jobs stratified by N, extrapolated to {jobs}",
            "kl_batched_over_plain": {"min": round(float(np.min(ratios)), 5), "mean": round(float(np.mean(ratios)), 5),
                                      "max": round(float(np.max(ratios)), 5)},
+           "kl_over_fp64_oracle": {"batched": _ratio_stats(vs_oracle_b), "plain": _ratio_stats(vs_oracle_p),
+                                   "sample": "the same sampled jobs, FP64 oracle loop from the same P and start"},
            "graph_replays": info["graph_replays"], "attractive_tuning": tune}
     log(f"[bench] batched: {out}")
     return out
 
 
-def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True):
+def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True, oracle_checks=2):
     """Poisson arrivals (N log-uniform in [1,000, 200,000], d in {2, 3}) through
     the continuous-batching server, then one at a time through the plain method."""
     from sgtsne_amd import serving, synth
@@ -177,9 +207,26 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True):
     base = serving.one_at_a_time(arrivals, make_X, k=k, u=u)
     t_base = time.perf_counter() - t0
     b_stats = serving.latency_stats(base)
+    # quality: the smallest requests of each d against the FP64 oracle loop
+    # from the same P and start (served and one-at-a-time results alike)
+    ordered = sorted(arrivals)
+    check = []
+    for dd in (2, 3):
+        cand = sorted((a[1], idx) for idx, a in enumerate(ordered) if a[2] == dd)[:oracle_checks]
+        check += [idx for _, idx in cand]
+    vs_s, vs_b = [], []
+    for idx in check:
+        _, n, dd, seed = ordered[idx]
+        I, D = sg.knn_exact(data[seed], k)
+        P = sg.perplexityEqualization(I, D, n, k, u)
+        rs, rb = oracle_kl_ratios(P, synth.random_init(n, dd, seed=seed), dd, [got[idx].Y, base[idx].Y])
+        vs_s.append(rs)
+        vs_b.append(rb)
     out = {"requests": requests, "rate_per_s": rate, "tick": tick, "completed_finite": ok,
            "served": s_stats, "one_at_a_time": b_stats,
            "embeds_per_s_ratio": round(s_stats["embeds_per_s"] / b_stats["embeds_per_s"], 2),
+           "kl_over_fp64_oracle": ({"served": _ratio_stats(vs_s), "one_at_a_time": _ratio_stats(vs_b),
+                                    "sample": f"the {oracle_checks} smallest requests of each d"} if check else None),
            "wall_s": {"served": round(t_srv, 2), "one_at_a_time": round(t_base, 2)}}
     log(f"[bench] served: {out}")
     return out
@@ -241,153 +288,235 @@ def knn_leg(sg, n=200_000, k=90):
     return res
 
 
+H_DEFAULT = {1: 0.5, 2: 0.7, 3: 1.2}
+
+
+def front_end(sg, X, k=90, u=30.0):
+    """The P front end on the GPU: exact kNN (certified filter where it
+    applies), FP64 perplexity bisection.  Returns P and the timings."""
+    n = X.shape[0]
+    t0 = time.perf_counter()
+    I, D, fb = sg.knn(X, k, sg.KNN_AUTO)
+    t_knn = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P = sg.perplexityEqualization(I, D, n, k, u)
+    t_perp = time.perf_counter() - t0
+    return P, {"knn": round(t_knn, 2), "knn_rows_recomputed_exactly": fb, "perplexity": round(t_perp, 2)}
+
+
+def kernel_name(d, tune):
+    return (f"attr_csr_loc_kernel<{d},{tune['group']}>" if tune["window"] < 0
+            else f"attr_csr_win_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["window"]
+            else f"attr_csr_blk_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
+            else f"attr_csr_seg2_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
+            else f"attr_csr_seg_kernel<{d},{tune['group']}>")
+
+
+def profile_traffic(profile, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    pj = json.load(open(path))
+    kern = {k.replace(" ", ""): v for k, v in pj["kernels"].items()}
+    hit = kern.get(kernel.replace(" ", ""))
+    return None if hit is None else hit["traffic_bytes_per_launch"]
+
+
+def run_config(sg, P, d, steps, warmup, fmt="CSR_SEG", reorder=2, cpu_seconds=15.0, cpu_max_iters=30,
+               want_cpu=True, want_terms=True, pin=None, kl_check=False, profile=None):
+    """One embedding job from P: `warmup` untimed iterations, then `steps`
+    timed ones (inputs resident); per-phase device times, the rooflines of
+    the attractive term (b) and of the repulsion (c), the FP32-vs-FP64 term
+    errors at the final Y, the CPU baseline on the same P_sym and Y."""
+    n = P.n
+    prm = sg.tsneparams(d=d, format=getattr(sg, "FORMAT_" + fmt), reorder=reorder)
+    ctx = sg.Context(prm)
+    t0 = time.perf_counter()
+    ctx.upload_P(P, symmetrize=True)
+    t_sym = time.perf_counter() - t0
+    if pin:
+        ctx.set_attractive(*pin)
+    nnz = ctx.info()["nnz"]
+    y0 = sg.synth.random_init(n, d, seed=1)
+    ctx.set_Y(y0)
+    y0c = ctx.get_Y() if kl_check else None  # the centred start both runs use
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    if warmup > 0:
+        ctx.step(warmup)
+    t_warm = time.perf_counter() - t0
+    tune = ctx.tuning()
+    Y_snap = ctx.get_Y()
+    it0 = ctx.info()["iter"]
+    t0 = time.perf_counter()
+    ctx.step(steps)  # synchronises the stream at the end
+    t_run = time.perf_counter() - t0
+    ms_per_step = t_run * 1e3 / steps
+    ph = ctx.timing()  # device time per phase summed over the timed steps
+    iters = max(ph[6], 1)
+    phases = {"attractive": ph[0] / iters, "s2g": ph[1] / iters, "conv": ph[2] / iters,
+              "g2s_forces": ph[3] / iters, "update_center": ph[4] / iters, "device_total": ph[5] / iters}
+    log(f"[bench] d={d} {fmt} n={n}: timed {steps} iterations: {ms_per_step:.3f} ms/iter; phases {phases}")
+    Y_end = ctx.get_Y()
+    reorders = ctx.reorders()
+    kl = ctx.kl(1.0)
+    Fa_g, Fr_g, Z_g, grid = ctx.terms()
+
+    # (b) attractive SpMV: 8 nnz + 4 (N + 1) + 8 d N bytes per launch over its
+    # average event-timed duration
+    kern = kernel_name(d, tune)
+    bytes_b = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * d * n
+    t_b = phases["attractive"] * 1e-3
+    ach_b = bytes_b / t_b / 1e9 if t_b > 0 else 0.0
+    roof_b = {"bound": "hbm", "achieved": round(ach_b, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+              "bytes_per_launch": bytes_b, "avg_launch_ms": round(phases["attractive"], 5)}
+    if profile:
+        tr = profile_traffic(profile, kern)
+        if tr is not None:
+            roof_b["traffic"] = tr
+            roof_b["traffic_source"] = f"profiles/{profile} (rocprofv3 --pmc)"
+    # (c) repulsion: 12 d N + 16 (d + 1) ng^d bytes per iteration (Y read by
+    # the spread and the gather, Frep written; the grid written by the spread,
+    # read and written by the convolution, read by the gather) over the
+    # summed device time of spread, convolution and gather + forces; ng is the
+    # grid side at the end of the timed window
+    bytes_c = 12.0 * d * n + 16.0 * (d + 1) * grid ** d
+    t_c = (phases["s2g"] + phases["conv"] + phases["g2s_forces"]) * 1e-3
+    ach_c = bytes_c / t_c / 1e9 if t_c > 0 else 0.0
+    roof_c = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": round(ach_c / HBM_PEAK_GBS, 5), "bytes_per_iteration": bytes_c, "grid_side": grid,
+              "ms_per_iteration": round(t_c * 1e3, 5),
+              "kernels": "spread + grid reduce, FFT convolution passes, gather + forces + Z partials"}
+
+    out = {"n": n, "d": d, "format": fmt.lower(), "nnz_sym": nnz, "ms_per_iter": round(ms_per_step, 4),
+           "timed_iterations": [it0, it0 + steps - 1], "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
+           "roofline": roof_b, "roofline_c": roof_c, "final_grid_side": grid, "final_kl": round(kl, 5),
+           "attractive_tuning": tune, "morton_reorders": reorders, "symmetrize_upload_s": round(t_sym, 2),
+           "full_run_ms": round((t_warm + t_run) * 1e3, 1)}
+    if want_terms or want_cpu or kl_check:
+        from oracle import oracle as O
+
+        P_sym = ctx.get_P()
+        So = O.CSR(P_sym.n, P_sym.rowptr.astype(np.int64), P_sym.col, P_sym.val.astype(np.float64))
+        if want_terms:
+            Yd = Y_end.astype(np.float64)
+            Fa_o = O.attractive(So, Yd)
+            Fr_o, Z_o, ng_o = O.repulsive_interp(Yd.copy(), H_DEFAULT[d], O.ZETA_DIRECT)
+            rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            out["term_error_vs_fp64"] = {"attractive_rel_l2": rel(Fa_g, Fa_o), "repulsive_rel_l2": rel(Fr_g, Fr_o),
+                                         "Z_rel": abs(Z_g - Z_o) / Z_o, "grid_side_gpu": grid,
+                                         "grid_side_oracle": ng_o}
+            log(f"[bench] FP32 vs FP64 oracle terms: {out['term_error_vs_fp64']}")
+        if want_cpu:
+            log(f"[bench] CPU baseline: FP64 oracle from the iteration-{it0} Y ...")
+            ms_cpu, cores, cpu_iters, ms1, it1 = cpu_baseline(P_sym, Y_snap, d, it0, cpu_seconds, cpu_max_iters)
+            out["cpu_baseline"] = {
+                "value": round(ms_cpu, 2), "unit": "ms/iter", "cores": cores, "kind": "port",
+                "sample": f"{cpu_iters} iterations (from iteration {it0}) of the FP64 C/OpenMP oracle on the "
+                          f"same P_sym (nnz={nnz}) and the GPU's iteration-{it0} Y, own FFT (no FFTW on host)",
+                "host_cpu": host_cpu_model(), "one_thread_ms": round(ms1, 1), "one_thread_iters": it1}
+            out["gpu_speedup_vs_cpu"] = round(ms_cpu / ms_per_step, 1)
+            log(f"[bench] CPU {ms_cpu:.1f} ms/iter on {cores} threads -> GPU speed-up {ms_cpu / ms_per_step:.1f}x")
+        if kl_check:
+            iters_all = it0 + steps
+            log(f"[bench] KL check: FP64 oracle, {iters_all} iterations on the host ...")
+            Yo = y0c.astype(np.float64)
+            t0 = time.perf_counter()
+            O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, iters_all, h=H_DEFAULT[d], zmode=O.ZETA_DIRECT)
+            t_o = time.perf_counter() - t0
+            _, Zo, _ = O.repulsive_interp(Yo.copy(), H_DEFAULT[d], O.ZETA_DIRECT)
+            _, Zg, _ = O.repulsive_interp(Y_end.astype(np.float64).copy(), H_DEFAULT[d], O.ZETA_DIRECT)
+            klo = O.kl(So, Yo, Zo)
+            klg = O.kl(So, Y_end.astype(np.float64), Zg)
+            out["kl_check"] = {"iterations": iters_all, "kl_gpu_fp32": round(klg, 6), "kl_oracle_fp64": round(klo, 6),
+                               "rel_diff": round(abs(klg - klo) / klo, 6), "oracle_s": round(t_o, 1),
+                               "oracle_threads": O.get_threads()}
+            log(f"[bench] KL check: {out['kl_check']}")
+        del So, P_sym
+    ctx.close()
+    return out
+
+
 def main_leg(sg, args):
-    """N=1.3M (by default) single-job ms/iter with roofline, CPU baseline and KL check."""
+    """C3: N=1.3M power-law mixture, d=2 (the headline) and, from the same P,
+    d=3 with the 3-D grid FFT."""
     k, u = 90, 30.0
     t_setup = time.perf_counter()
     log(f"[bench] generating {args.workload} ...")
     X = workload(args.workload, args.n)
     n = X.shape[0]
-    log(f"[bench] n={n} dim={X.shape[1]}: exact kNN k={k} on the GPU ...")
-    t0 = time.perf_counter()
-    I, D = sg.knn_exact(X, k)
-    t_knn = time.perf_counter() - t0
-    log(f"[bench] kNN {t_knn:.1f} s; perplexity ...")
-    t0 = time.perf_counter()
-    P = sg.perplexityEqualization(I, D, n, k, u)
-    t_perp = time.perf_counter() - t0
-    del I, D
-    prm = sg.tsneparams(d=args.d, format=getattr(sg, "FORMAT_" + args.format), reorder=args.reorder)
-    ctx = sg.Context(prm)
-    t0 = time.perf_counter()
-    ctx.upload_P(P, symmetrize=True)
-    t_sym = time.perf_counter() - t0
-    if args.pin_attractive:
-        ctx.set_attractive(*[int(x) for x in args.pin_attractive.split(",")])
-    info = ctx.info()
-    nnz = info["nnz"]
-    tune = ctx.tuning()
-    log(f"[bench] attractive-kernel autotuner: {tune}")
-    log(f"[bench] P_sym nnz={nnz} (symmetrise+upload {t_sym:.1f} s); setup {time.perf_counter() - t_setup:.1f} s")
-
-    y0 = sg.synth.random_init(n, args.d, seed=1)
-    ctx.set_Y(y0)
-    y0c = ctx.get_Y() if args.kl_check else None  # the centred start both runs use
-    ctx.set_timing(True)
-    t0 = time.perf_counter()
-    if args.warmup > 0:
-        ctx.step(args.warmup)
-    t_warm = time.perf_counter() - t0
-    log(f"[bench] warmup {args.warmup} iterations: {t_warm * 1e3 / max(args.warmup, 1):.3f} ms/iter")
-    tune = ctx.tuning()  # re-chosen after the first Morton re-order
-    log(f"[bench] attractive kernel in the timed region: {tune}")
-    Y_snap = ctx.get_Y()
-    it0 = ctx.info()["iter"]
-
-    t0 = time.perf_counter()
-    ctx.step(args.steps)  # synchronises the stream at the end
-    t_run = time.perf_counter() - t0
-    ms_per_step = t_run * 1e3 / args.steps
-    ph = ctx.timing()  # device time per phase summed over the timed steps
-    iters = max(ph[6], 1)
-    phases = {"attractive": ph[0] / iters, "s2g": ph[1] / iters, "conv": ph[2] / iters,
-              "g2s_forces": ph[3] / iters, "update_center": ph[4] / iters, "device_total": ph[5] / iters}
-    log(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iter; phases {phases}")
-    Y_end = ctx.get_Y()
-    reorders = ctx.reorders()
-    kl = ctx.kl(1.0)
-    Fa_g, Fr_g, Z_g, grid = ctx.terms()
-    log(f"[bench] final KL {kl:.4f}, grid side {grid}")
-
-    # roofline of the dominant kernel (attractive SpMV): algorithmic bytes
-    # 8 nnz + 4 (N + 1) + 8 d N per launch over its average event-timed duration
-    bytes_b = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * args.d * n
-    t_spmv = phases["attractive"] * 1e-3
-    achieved = bytes_b / t_spmv / 1e9 if t_spmv > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": (f"attr_csr_loc_kernel<{args.d},{tune['group']}>" if tune["window"] < 0
-                           else f"attr_csr_win_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["window"]
-                           else f"attr_csr_blk_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
-                           else f"attr_csr_seg2_kernel<{args.d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
-                           else f"attr_csr_seg_kernel<{args.d},{tune['group']}>"),
-                "bytes_per_launch": bytes_b,
-                "avg_launch_ms": round(phases["attractive"], 5)}
-
-    # HBM traffic per launch of the same kernel on the same workload, from the
-    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain copy)
-    prof = os.path.join(ROOT, "profiles", "attractive_spmv_powerlaw1.3M_d2.json")
-    if os.path.exists(prof) and args.workload == "powerlaw40_50d" and args.d == 2 and n == 1_300_000:
-        pj = json.load(open(prof))
-        kern = {k.replace(" ", ""): v for k, v in pj["kernels"].items()}
-        hit = kern.get(roofline["kernel"].replace(" ", ""))
-        if hit is not None:
-            roofline["traffic"] = hit["traffic_bytes_per_launch"]
-            roofline["traffic_source"] = "profiles/attractive_spmv_powerlaw1.3M_d2.json (rocprofv3 --pmc)"
-
-    cpu = None
-    term_err = None
-    if not args.no_cpu:
-        # FP32 GPU terms against the FP64 oracle at the final embedding
-        from oracle import oracle as O
-
-        P_sym = ctx.get_P()
-        So = O.CSR(P_sym.n, P_sym.rowptr.astype(np.int64), P_sym.col, P_sym.val.astype(np.float64))
-        Yd = Y_end.astype(np.float64)
-        Fa_o = O.attractive(So, Yd)
-        Fr_o, Z_o, ng_o = O.repulsive_interp(Yd.copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d], O.ZETA_DIRECT)
-        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        term_err = {"attractive_rel_l2": rel(Fa_g, Fa_o), "repulsive_rel_l2": rel(Fr_g, Fr_o),
-                    "Z_rel": abs(Z_g - Z_o) / Z_o, "grid_side_gpu": grid, "grid_side_oracle": ng_o}
-        log(f"[bench] FP32 vs FP64 oracle terms: {term_err}")
-        del So
-        log(f"[bench] CPU baseline: FP64 oracle from the iteration-{it0} Y ...")
-        ms_cpu, cores, cpu_iters, ms1, it1 = cpu_baseline(P_sym, Y_snap, args.d, it0, args.cpu_seconds,
-                                                          args.cpu_max_iters)
-        cpu = {"value": round(ms_cpu, 2), "unit": "ms/iter", "cores": cores, "kind": "port",
-               "sample": f"{cpu_iters} iterations (from iteration {it0}) of the FP64 C/OpenMP oracle on the "
-                         f"same P_sym (nnz={nnz}) and the GPU's iteration-{it0} Y, own FFT (no FFTW on host)",
-               "host_cpu": host_cpu_model(), "one_thread_ms": round(ms1, 1), "one_thread_iters": it1}
-        log(f"[bench] CPU {ms_cpu:.1f} ms/iter on {cores} threads -> GPU speed-up {ms_cpu / ms_per_step:.1f}x")
-    kl_check = None
-    if args.kl_check:
-        from oracle import oracle as O
-
-        P_sym = ctx.get_P()
-        So = O.CSR(P_sym.n, P_sym.rowptr.astype(np.int64), P_sym.col, P_sym.val.astype(np.float64))
-        iters = it0 + args.steps
-        log(f"[bench] KL check: FP64 oracle, {iters} iterations on the host ...")
-        Yo = y0c.astype(np.float64)
-        t0 = time.perf_counter()
-        O.run(So, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, iters, zmode=O.ZETA_DIRECT)
-        t_o = time.perf_counter() - t0
-        _, Zo, _ = O.repulsive_interp(Yo.copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d], O.ZETA_DIRECT)
-        _, Zg, _ = O.repulsive_interp(Y_end.astype(np.float64).copy(), {1: 0.5, 2: 0.7, 3: 1.2}[args.d],
-                                      O.ZETA_DIRECT)
-        klo = O.kl(So, Yo, Zo)
-        klg = O.kl(So, Y_end.astype(np.float64), Zg)
-        kl_check = {"iterations": iters, "kl_gpu_fp32": round(klg, 6), "kl_oracle_fp64": round(klo, 6),
-                    "rel_diff": round(abs(klg - klo) / klo, 6), "oracle_s": round(t_o, 1),
-                    "oracle_threads": O.get_threads()}
-        log(f"[bench] KL check: {kl_check}")
-        del So
-    ctx.close()
-
+    log(f"[bench] n={n} dim={X.shape[1]}: kNN k={k} on the GPU ...")
+    P, setup = front_end(sg, X, k, u)
+    del X
+    log(f"[bench] front end {setup}; setup {time.perf_counter() - t_setup:.1f} s")
+    pin = [int(x) for x in args.pin_attractive.split(",")] if args.pin_attractive else None
+    full = args.workload == "powerlaw40_50d" and n == 1_300_000
+    r = run_config(sg, P, args.d, args.steps, args.warmup, fmt=args.format, reorder=args.reorder,
+                   cpu_seconds=args.cpu_seconds, cpu_max_iters=args.cpu_max_iters, want_cpu=not args.no_cpu,
+                   pin=pin, kl_check=args.kl_check,
+                   profile="attractive_spmv_powerlaw1.3M_d2.json" if full and args.d == 2 else None)
+    setup["symmetrize_upload"] = r.pop("symmetrize_upload_s")
+    d3 = None
+    if "d3" in args.configs and args.d == 2:
+        log("[bench] C3 at d=3 from the same P ...")
+        d3 = run_config(sg, P, 3, args.steps, args.warmup, fmt=args.format, reorder=args.reorder,
+                        cpu_seconds=min(args.cpu_seconds, 8.0), cpu_max_iters=10, want_cpu=not args.no_cpu)
     return {
-        "value": round(ms_per_step, 4),
-        "ms_per_step": round(ms_per_step, 4),
+        "value": r["ms_per_iter"],
+        "ms_per_step": r["ms_per_iter"],
         "config": {"workload": f"{args.workload} N={n} d={args.d} k={k} u={u}", "n": n, "d": args.d,
-                   "nnz_sym": nnz, "format": args.format.lower(), "reorder": args.reorder,
-                   "morton_reorders": reorders, "timed_iterations": [it0, it0 + args.steps - 1],
-                   "final_grid_side": grid, "final_kl": round(kl, 5), "attractive_tuning": tune},
-        "phases_ms": {k2: round(v, 5) for k2, v in phases.items()},
-        "full_run_ms": round((t_warm + t_run) * 1e3, 1),
-        "setup_s": {"knn": round(t_knn, 2), "perplexity": round(t_perp, 2), "symmetrize_upload": round(t_sym, 2)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "term_error_vs_fp64": term_err,
-        "kl_check": kl_check,
+                   "nnz_sym": r["nnz_sym"], "format": r["format"], "reorder": args.reorder,
+                   "morton_reorders": r["morton_reorders"], "timed_iterations": r["timed_iterations"],
+                   "final_grid_side": r["final_grid_side"], "final_kl": r["final_kl"],
+                   "attractive_tuning": r["attractive_tuning"]},
+        "phases_ms": r["phases_ms"],
+        "full_run_ms": r["full_run_ms"],
+        "setup_s": setup,
+        "roofline": r["roofline"],
+        "roofline_c": r["roofline_c"],
+        "cpu_baseline": r.get("cpu_baseline"),
+        "term_error_vs_fp64": r.get("term_error_vs_fp64"),
+        "kl_check": r.get("kl_check"),
+        "c3_d3": d3,
     }
+
+
+def c2_leg(sg, args):
+    """C2: N=60,000 sparse 784-D clusters: the hybrid ELL + COO format against
+    COO (and this engine's CSR kernel), GPU against the CPU oracle."""
+    X = workload("sparse784")
+    P, setup = front_end(sg, X)
+    del X
+    out = {"workload": f"sparse784 N={P.n} dim=784 d=2 k=90 u=30", "setup_s": setup, "formats": {}}
+    for fmt in ("HYBRID", "COO", "CSR_SEG"):
+        r = run_config(sg, P, 2, args.steps, args.warmup, fmt=fmt, cpu_seconds=5.0, cpu_max_iters=20,
+                       want_cpu=(fmt == "CSR_SEG" and not args.no_cpu), want_terms=True)
+        out["formats"][fmt.lower()] = {k2: r.get(k2) for k2 in ("ms_per_iter", "phases_ms", "roofline",
+                                                                "term_error_vs_fp64", "final_grid_side")}
+        if "cpu_baseline" in r:
+            out["cpu_baseline"] = r["cpu_baseline"]
+    f = out["formats"]
+    out["hybrid_over_coo_speedup"] = round(f["coo"]["ms_per_iter"] / f["hybrid"]["ms_per_iter"], 3)
+    if out.get("cpu_baseline"):
+        out["gpu_speedup_vs_cpu"] = {k2: round(out["cpu_baseline"]["value"] / v["ms_per_iter"], 1)
+                                     for k2, v in f.items()}
+    log(f"[bench] C2: {out}")
+    return out
+
+
+def c1_leg(sg, args):
+    """C1: N=10,000 isotropic 10-cluster mixture in 50-D, GPU against CPU."""
+    X = workload("mixture10_50d")
+    P, setup = front_end(sg, X)
+    r = run_config(sg, P, 2, args.steps, args.warmup, cpu_seconds=5.0, cpu_max_iters=50,
+                   want_cpu=not args.no_cpu)
+    r["workload"] = f"mixture10_50d N={P.n} dim=50 d=2 k=90 u=30"
+    r["setup_s"] = setup
+    log(f"[bench] C1: {r}")
+    return r
 
 
 def main():
@@ -419,6 +548,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-tick", type=int, default=10)
+    ap.add_argument("--configs", default="d3,c2,c1",
+                    help="extra configurations besides the headline: d3 (C3 at d=3 from the same P), "
+                         "c2 (sparse784 formats), c1 (mixture10_50d)")
     ap.add_argument("--engine-d", type=int, default=3,
                     help="engine leg: the served stream's jobs of this d in one batch")
     ap.add_argument("--no-graph", action="store_true",
@@ -433,6 +565,7 @@ def main():
     import sgtsne_amd as sg
 
     legs = set(args.legs.split(","))
+    args.configs = set(c for c in args.configs.split(",") if c)
     if args.no_batched:
         legs.discard("batched")
     if args.no_served:
@@ -449,6 +582,8 @@ def main():
     batched = batched_leg(sg, args.batch_jobs, args.batch_sample, graph=g) if "batched" in legs else None
     served = served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g) if "served" in legs else None
     knn = knn_leg(sg) if "knn" in legs else None
+    c2 = c2_leg(sg, args) if "c2" in args.configs and "main" in legs else None
+    c1 = c1_leg(sg, args) if "c1" in args.configs and "main" in legs else None
     engine = engine_leg(sg, args.engine_d, graph=g) if "engine" in legs else None
 
     value = m.get("value")
@@ -470,10 +605,14 @@ def main():
         "full_run_ms": m.get("full_run_ms"),
         "setup_s": m.get("setup_s"),
         "roofline": m.get("roofline"),
+        "roofline_c": m.get("roofline_c"),
         "cpu_baseline": m.get("cpu_baseline"),
         "term_error_vs_fp64": m.get("term_error_vs_fp64"),
         "copy_probe": copy,
         "kl_check": m.get("kl_check"),
+        "c3_d3": m.get("c3_d3"),
+        "c2_sparse784": c2,
+        "c1_mixture10": c1,
         "batched": batched,
         "served": served,
         "knn": knn,

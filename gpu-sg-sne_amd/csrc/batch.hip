@@ -206,13 +206,23 @@ int auto_ng_cap(int d, int n) {
 // re-captures it and updates the executable graph in place.
 void invalidate_graph(sgtsne_batch *b) { b->graph_valid = false; }
 
+int stage_commit(sgtsne_batch *b);
+
 // Reserves `bytes` of the staging ring, first waiting for the copies that
 // still read an overlapping span; nullptr when the request exceeds the ring.
+// A span reserved since the last commit has its copy queued but no event
+// yet: when the ring wraps onto one, the group is committed and its event
+// waited for before the memory is written again.
 void *stage_alloc(sgtsne_batch *b, size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes > kStageBytes || !b->stage) return nullptr;
   size_t off = b->stage_head;
   if (off + bytes > kStageBytes) off = 0;
+  for (const auto &sp : b->stage_live)
+    if (!sp.ev && sp.off < off + bytes && off < sp.off + sp.len) {
+      if (stage_commit(b) != SGTSNE_OK) return nullptr;  // the caller then copies synchronously
+      break;
+    }
   for (auto it = b->stage_live.begin(); it != b->stage_live.end();) {
     const bool overlap = it->off < off + bytes && off < it->off + it->len;
     const bool done = it->ev && hipEventQuery(it->ev) == hipSuccess;

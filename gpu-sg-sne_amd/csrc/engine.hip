@@ -193,6 +193,7 @@ int check_params(const sgtsne_params *p) {
             "bad layout");
   SGT_CHECK(p->zeta_mode == SGTSNE_ZETA_DIRECT || p->zeta_mode == SGTSNE_ZETA_ALGEBRAIC, SGTSNE_ERR_ARG,
             "bad zeta_mode");
+  SGT_CHECK(p->spread >= SGTSNE_SPREAD_AUTO && p->spread <= SGTSNE_SPREAD_MFMA, SGTSNE_ERR_ARG, "bad spread");
   return SGTSNE_OK;
 }
 
@@ -226,7 +227,7 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
   const size_t gb = G * m * sizeof(float);
   size_t wb = 0, kb = 0;
   conv_workspace_bytes(c->d, g.ng, g.L, &wb, &kb);
-  const int64_t need = s2g_scratch_need(g, c->n);
+  const int64_t need = s2g_scratch_need(g, c->n, c->p.spread);
   if (gb > c->grid_cap || wb > c->w_cap || kb > c->k_cap || need > c->s2g_cap) {
     SGT_HIP(hipStreamSynchronize(c->stream));
   }
@@ -293,7 +294,7 @@ int repulsive(sgtsne_ctx *c, const GridParams &g, PhaseEvents *pe) {
   const int zdirect = c->p.zeta_mode == SGTSNE_ZETA_DIRECT;
   SGT_TRY(ensure_grid(c, g));
   int64_t need = 0;
-  SGT_TRY(launch_s2g(c->stream, g, c->n, c->Y, c->V, c->s2g_part, c->s2g_cap, &need));
+  SGT_TRY(launch_s2g(c->stream, g, c->n, c->Y, c->V, c->s2g_part, c->s2g_cap, &need, c->p.spread));
   if (pe) SGT_HIP(hipEventRecord(pe->e[2], c->stream));
   SGT_TRY(launch_conv(c->stream, g, zdirect, c->V, c->Phi, c->cb));
   if (pe) SGT_HIP(hipEventRecord(pe->e[3], c->stream));
@@ -895,6 +896,7 @@ int sgtsne_params_default(sgtsne_params *p) {
   p->ell_width = 0;
   p->zeta_mode = SGTSNE_ZETA_DIRECT;
   p->reorder = 2;
+  p->spread = SGTSNE_SPREAD_AUTO;
   return SGTSNE_OK;
 }
 

@@ -72,8 +72,8 @@ int launch_center_minmax(hipStream_t s, int d, int n, float *Y, int apply_mean, 
 // per-block private grids of the LDS path; *needed receives the floats it
 // wants (when larger than scratch_floats the global-atomic path runs).
 int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V, float *scratch,
-               int64_t scratch_floats, int64_t *needed);
-int64_t s2g_scratch_need(const GridParams &g, int n);
+               int64_t scratch_floats, int64_t *needed, int mode = 0);
+int64_t s2g_scratch_need(const GridParams &g, int n, int mode = 0);
 int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
                      float *Frep, int zdirect, RedWork rw, DevScalars *sc);
 int launch_finalize_kl(hipStream_t s, int nblocks, RedWork rw, DevScalars *sc);

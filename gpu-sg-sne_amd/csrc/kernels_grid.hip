@@ -184,6 +184,98 @@ __global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n,
   }
 }
 
+// ---- S2G on the matrix cores (2-D grids of side <= 16 NT) -------------------------
+// For 4 points k at a time, the spread of channel c onto a 16 x 16 tile of
+// nodes is a rank-4 product, one v_mfma_f32_16x16x4f32:
+//   V_c[gx][gy] += sum_k wx_k(gx) * (q_c,k * wy_k(gy))
+// with A[gx][k] = wx_k(gx) (lane l holds gx = l % 16 of point k = l / 16),
+// B[k][gy] = q_c,k wy_k(gy) (lane l holds gy = l % 16 of the same point), and
+// D[gx][gy] in lane l, register v at gx = 4 (l / 16) + v, gy = l % 16.  A
+// lane evaluates one Lagrange factor per axis and tile; nodes off a point's
+// stencil get weight 0.  Nothing is contended: each wave keeps its points'
+// sums in registers whatever the order of the points or how many share a
+// cell, the block's 4 waves are added in LDS in a fixed order, and
+// grid_reduce_kernel adds the blocks' grids in a fixed order (deterministic).
+typedef float mfv4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaBlock = 256;
+constexpr int kMfmaChunk = 2048;  // points per work-group
+
+// Weight of stencil node f + r of a point at f + 1 + dl (the node factor of
+// lane_node), 0 when r is not 0..3.
+__device__ __forceinline__ float lagrange_node(int r, float dl) {
+  const float x = r == 0 ? 1.0f + dl : r == 1 ? dl : r == 2 ? 1.0f - dl : 2.0f - dl;
+  const float w = (r == 1 || r == 2) ? lg1(x) : lg2(x);
+  return (r >= 0 && r <= 3) ? w : 0.0f;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int n, const float *__restrict__ Y,
+                                                              float *__restrict__ part, int chunk) {
+  constexpr int T = 16 * NT;  // side of the tiled grid (>= ng)
+  __shared__ float red[4][3][T * T];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = lane >> 4, node = lane & 15;
+  const int i0 = blockIdx.x * chunk;
+  const int i1 = min(n, i0 + chunk);
+  mfv4 acc[3][NT * NT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int t = 0; t < NT * NT; ++t) acc[c][t] = mfv4{0.0f, 0.0f, 0.0f, 0.0f};
+  const float2 *Y2 = reinterpret_cast<const float2 *>(Y);
+  for (int base = i0 + 4 * wave; base < i1; base += 16) {
+    const int i = base + k;
+    const bool in = i < i1;
+    const float2 y = in ? Y2[i] : make_float2(g.minv[0], g.minv[1]);
+    float t0 = (y.x - g.minv[0]) * g.scale, t1 = (y.y - g.minv[1]) * g.scale;
+    t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
+    t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
+    const int f0 = (int)t0, f1 = (int)t1;
+    const float d0 = t0 - (float)f0, d1 = t1 - (float)f1;
+    float wx[NT], wy[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      wx[t] = in ? lagrange_node(16 * t + node - f0, d0) : 0.0f;
+      wy[t] = lagrange_node(16 * t + node - f1, d1);
+    }
+    const float q1 = y.x - g.center[0], q2 = y.y - g.center[1];
+#pragma unroll
+    for (int ty = 0; ty < NT; ++ty) {
+      const float b0 = wy[ty], b1 = q1 * wy[ty], b2 = q2 * wy[ty];
+#pragma unroll
+      for (int tx = 0; tx < NT; ++tx) {
+        acc[0][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b0, acc[0][tx * NT + ty], 0, 0, 0);
+        acc[1][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b1, acc[1][tx * NT + ty], 0, 0, 0);
+        acc[2][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b2, acc[2][tx * NT + ty], 0, 0, 0);
+      }
+    }
+  }
+  // this wave's sums: node (gx, gy) of channel c at red[wave][c][gy * T + gx]
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int tx = 0; tx < NT; ++tx)
+#pragma unroll
+      for (int ty = 0; ty < NT; ++ty) {
+        float *dst = &red[wave][c][(16 * ty + node) * T + 16 * tx + 4 * k];
+        const mfv4 a = acc[c][tx * NT + ty];
+        dst[0] = a[0];
+        dst[1] = a[1];
+        dst[2] = a[2];
+        dst[3] = a[3];
+      }
+  __syncthreads();
+  // the block's grid (channel-major, x fastest), waves added in a fixed order
+  const int ng = g.ng, G = ng * ng;
+  float *out = part + (int64_t)blockIdx.x * 3 * G;
+  for (int e = threadIdx.x; e < 3 * G; e += blockDim.x) {
+    const int c = e / G, cell = e - c * G;
+    const int gy = cell / ng, gx = cell - gy * ng;
+    const int sidx = gy * T + gx;
+    out[e] = (red[0][c][sidx] + red[1][c][sidx]) + (red[2][c][sidx] + red[3][c][sidx]);
+  }
+}
+
 // ---- S2G with global float atomics (grids too large for LDS) -------------------
 // Same lane-per-node mapping; the grid is large here, so contention is low.
 template <int D>
@@ -522,16 +614,32 @@ int s2g_lds_fits(int d, int ng, size_t *lds_bytes, int *copies) {
   return *lds_bytes <= 65536;
 }
 
-int64_t s2g_scratch_need(const GridParams &g, int n) {
+int64_t s2g_scratch_need(const GridParams &g, int n, int mode) {
   int64_t need = 0;
-  (void)launch_s2g(nullptr, g, -n, nullptr, nullptr, nullptr, 0, &need);
+  (void)launch_s2g(nullptr, g, -n, nullptr, nullptr, nullptr, 0, &need, mode);
   return need;
 }
 
 int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float *V, float *scratch,
-               int64_t scratch_floats, int64_t *needed) {
+               int64_t scratch_floats, int64_t *needed, int mode) {
   const bool query = n < 0;  // size query only
   if (query) n = -n;
+  if (n > 0 && g.d == 2 && g.ng <= 32 && mode != SGTSNE_SPREAD_ATOMIC) {
+    // matrix cores: no contention however the points crowd the few cells
+    const int64_t cells = 3ll * g.ng * g.ng;
+    const int blocks = div_up(n, kMfmaChunk);
+    if (needed) *needed = (int64_t)blocks * cells;
+    if (query) return SGTSNE_OK;
+    SGT_CHECK((int64_t)blocks * cells <= scratch_floats, SGTSNE_ERR_STATE, "spread scratch too small");
+    if (g.ng <= 16)
+      s2g_mfma_kernel<1><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, kMfmaChunk);
+    else
+      s2g_mfma_kernel<2><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, kMfmaChunk);
+    SGT_LAUNCH_CHECK();
+    grid_reduce_kernel<<<div_up(cells, 64), 1024, 0, s>>>(scratch, blocks, (int)cells, V);
+    SGT_LAUNCH_CHECK();
+    return SGTSNE_OK;
+  }
   size_t lds = 0;
   int copies = 1;
   const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);

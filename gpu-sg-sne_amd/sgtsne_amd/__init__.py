@@ -30,6 +30,9 @@ ZETA_ALGEBRAIC = 1
 ERR_FULL = -8
 JOB_RUNNING = 0
 JOB_DONE = 1
+SPREAD_AUTO = 0
+SPREAD_ATOMIC = 1
+SPREAD_MFMA = 2
 KNN_AUTO = 0
 KNN_EXACT_FP64 = 1
 KNN_FILTERED = 2
@@ -64,7 +67,7 @@ class tsneparams(C.Structure):
         ("earlyIter", C.c_int), ("n", C.c_int), ("h", C.c_double), ("dropLeaf", C.c_int),
         ("np", C.c_int), ("format", C.c_int), ("bs", C.c_int), ("ComputeError", C.c_int),
         ("ng", C.c_int), ("grid_mode", C.c_int), ("layout", C.c_int), ("seed", C.c_uint64),
-        ("ell_width", C.c_int), ("zeta_mode", C.c_int), ("reorder", C.c_int),
+        ("ell_width", C.c_int), ("zeta_mode", C.c_int), ("reorder", C.c_int), ("spread", C.c_int),
     ]
 
     def __init__(self, **kw):

@@ -34,7 +34,11 @@ import numpy as np
 from . import Batch, Full, tsneparams
 from .synth import random_init
 
-_POINT_BYTES = {2: 6 * 2 * 4 + 4, 3: 6 * 4 * 4 + 4}  # Y, uY, gains, Fattr, Frep, (spare) + perm
+_POINT_BYTES = {2: 6 * 2 * 4 + 4, 3: 6 * 4 * 4 + 4}
+# the deepest queue of batch iterations the server has been validated with on
+# the GPU (iterations per tick, ticks in flight per engine)
+MAX_TICK = 10
+MAX_DEPTH = 3  # Y, uY, gains, Fattr, Frep, (spare) + perm
 
 
 def footprint_bytes(n, d, k=90):
@@ -72,6 +76,11 @@ class Server:
         self.k, self.u, self.max_iter, self.tick = k, u, max_iter, tick
         # ticks in flight per engine: an int for both, or {d: depth}
         self.depth = depth if isinstance(depth, dict) else {2: depth, 3: depth}
+        if engine_factory is None and (tick > MAX_TICK or max(self.depth.values()) > MAX_DEPTH):
+            # a served sweep faulted the GPU at tick 20 / depth 6 and no cause
+            # has been shown; deeper queues than the tested setting are refused
+            raise ValueError(f"tick <= {MAX_TICK} and depth <= {MAX_DEPTH} ticks in flight are supported "
+                             f"(asked for tick {tick}, depth {depth})")
         self.poll_s = poll_s
         self.engines = {}
         self.budget = {}

@@ -59,6 +59,9 @@ extern "C" {
 #define SGTSNE_ZETA_ALGEBRAIC 1
 
 /* Y layouts: SoA [d][N] (CU path) or AoS [N][d] (HYB path) */
+#define SGTSNE_SPREAD_AUTO 0   /* matrix cores where they apply, else atomics   */
+#define SGTSNE_SPREAD_ATOMIC 1 /* LDS-privatised grids, or global atomics       */
+#define SGTSNE_SPREAD_MFMA 2   /* matrix cores (2-D, grid side <= 32; else AUTO) */
 #define SGTSNE_LAYOUT_SOA 0
 #define SGTSNE_LAYOUT_AOS 1
 
@@ -91,6 +94,9 @@ typedef struct sgtsne_params {
                          Morton order of Y at iterations 50, 150, 300, 600
                          and keeps it when the attractive kernel measures
                          faster (CSR formats)                            (2)   */
+  int spread;         /* SGTSNE_SPREAD_*: how points are spread onto the grid
+                         (AUTO: the matrix-core form for 2-D grids of side
+                         <= 32, else LDS-privatised or atomic)       (AUTO)  */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of

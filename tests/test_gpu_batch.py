@@ -328,3 +328,23 @@ def test_batch_threaded_front_end(sg, O):
         assert st == sg.JOB_DONE and it == 40
         assert np.array_equal(B.get_Y(a), Y_alone)
         assert B.job_left(other) == max(0, 401 - steps - left)
+
+
+def test_batch_staging_ring_wrap_keeps_uploads(sg, O):
+    # a job whose Y fills the 16 MiB pinned staging ring exactly (2^21 points
+    # in 2-D), admitted while another job's iterations are queued: its job
+    # record then wraps onto the start of the ring, where Y's copy may still
+    # be waiting.  The ring commits and waits before reusing that memory, so
+    # the Y read back equals the caller's bit for bit.
+    S_small, _ = _job(sg, O, 3000, 41)
+    n = 1 << 21
+    idx = np.arange(n)
+    rp = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    col = np.stack([(idx - 1) % n, (idx + 1) % n], 1).reshape(-1).astype(np.int32)
+    val = np.full(2 * n, 0.5 / n, np.float32)
+    y0 = np.random.default_rng(9).standard_normal((n, 2)).astype(np.float32)
+    with _batch(sg, max_jobs=4, points=n + 8192, nnz=2 * n + 2_000_000) as B:
+        B.add(S_small, symmetrize=False)
+        B.step(60)  # queued: the big job's copies wait behind these
+        big = B.add(sg.CSR(n, rp, col, val), symmetrize=False, y_in=y0, max_iter=2)
+        assert np.array_equal(B.get_Y(big), y0)

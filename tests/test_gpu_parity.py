@@ -292,6 +292,36 @@ def test_terms_fixed_grid(sg, O, d):
             assert abs(Z - Zo) / Zo < TOL
 
 
+@pytest.mark.parametrize("spread", ["AUTO", "ATOMIC", "MFMA"])
+def test_terms_spread_modes(sg, O, spread):
+    # every spread path against the oracle on 2-D grids of side 16, 20-32
+    # (the matrix-core form: one and two 16-node tiles per axis) and larger
+    # (where the matrix-core request falls back); the start of a run crowds
+    # every point into a few cells, the worst case for atomics
+    n = 6001
+    X = _mixture(n, 20, 5, 77)
+    I, D = O.knn_exact(X, 30)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 10.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(78)
+    cases = [(rng.standard_normal((n, 2)) * 1e-4).astype(np.float32),   # ng 16
+             (rng.standard_normal((n, 2)) * 2.2).astype(np.float32),    # ng 20-32
+             _clustered_embedding(n, 2, rng, spread=4.5),               # ng 20-32
+             _clustered_embedding(n, 2, rng)]                           # ng > 32
+    sides = []
+    with sg.Context(sg.tsneparams(spread=getattr(sg, "SPREAD_" + spread))) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        for Y in cases:
+            ctx.set_Y(Y)
+            Yc = ctx.get_Y()
+            _, Fr, Z, ng = ctx.terms()
+            Fro, Zo, ngo = O.repulsive_interp(Yc.astype(np.float64), 0.7, O.ZETA_DIRECT)
+            assert ng == ngo
+            assert rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL, (spread, ng)
+            sides.append(ng)
+    assert sides[0] == 16 and 16 < sides[1] <= 32 and 16 < sides[2] <= 32 and sides[3] > 32, sides
+
+
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_terms_reference_acceptance_setup(sg, O, d):
     # the reference's repulsion self-test inputs (HY/test_modules.cpp:122-156):

@@ -154,3 +154,13 @@ def test_server_depth_per_engine():
         got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
     assert len(got) == 2
     assert eng[2].max_queued <= 1 * 10 and eng[3].max_queued <= 2 * 10
+
+
+def test_server_refuses_unvalidated_queue_depths():
+    # on real engines the server refuses queues deeper than the setting it was
+    # validated with (a tick 20 / depth 6 sweep faulted the GPU, cause unknown)
+    from sgtsne_amd import serving
+
+    for kw in ({"tick": 20}, {"depth": 6}, {"depth": {2: 3, 3: 4}}):
+        with pytest.raises(ValueError, match="supported"):
+            serving.Server(**kw)
