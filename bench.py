@@ -100,9 +100,12 @@ def job_kl(sg, P, Y, d):
         return ctx.kl(1.0)
 
 
-def oracle_kl_ratios(P, y0, d, Ys, iters=1000):
+def oracle_kl_ratios(P, y0, d, Ys, iters=1000, self_spread=False):
     """Final KL of each embedding in Ys over the FP64 oracle loop's final KL,
-    run from the same P (row-stochastic kNN CSR) and start y0."""
+    run from the same P (row-stochastic kNN CSR) and start y0.  With
+    self_spread, the last ratio is that of the oracle itself started one ulp
+    away (every coordinate moved to the next double): how far the descent's
+    own sensitivity alone moves the final KL."""
     from oracle import oracle as O
 
     S = O.symmetrize(O.CSR(P.n, P.rowptr.astype(np.int64), P.col, P.val.astype(np.float64)))
@@ -116,7 +119,12 @@ def oracle_kl_ratios(P, y0, d, Ys, iters=1000):
     Yo = np.ascontiguousarray(y0, dtype=np.float64).copy()
     O.run(S, Yo, np.zeros_like(Yo), np.ones_like(Yo), 0, iters, h=h, zmode=O.ZETA_DIRECT)
     klo = kl_of(Yo)
-    return [kl_of(Y) / klo for Y in Ys]
+    out = [kl_of(Y) / klo for Y in Ys]
+    if self_spread:
+        Yp = np.nextafter(np.ascontiguousarray(y0, dtype=np.float64), np.inf)
+        O.run(S, Yp, np.zeros_like(Yp), np.ones_like(Yp), 0, iters, h=h, zmode=O.ZETA_DIRECT)
+        out.append(kl_of(Yp) / klo)
+    return out
 
 
 def _ratio_stats(v):
@@ -153,7 +161,7 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
     order = np.argsort(sizes)
     pick = order[np.linspace(0, jobs - 1, sample).round().astype(int)]
     t_plain = 0.0
-    ratios, vs_oracle_b, vs_oracle_p = [], [], []
+    ratios, vs_oracle_b, vs_oracle_p, vs_oracle_self = [], [], [], []
     for i in pick:
         X, _ = synth.gaussian_job(int(sizes[i]), seed=seeds[i])
         y0 = synth.random_init(int(sizes[i]), 2, seed=seeds[i])
@@ -162,9 +170,10 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
         t_plain += time.perf_counter() - t0
         Yb = B.get_Y(int(ids[i]))
         ratios.append(job_kl(sg, P, Yb, 2) / job_kl(sg, P, Yp, 2))
-        rb, rp = oracle_kl_ratios(P, y0, 2, [Yb, Yp])
+        rb, rp, ro = oracle_kl_ratios(P, y0, 2, [Yb, Yp], self_spread=True)
         vs_oracle_b.append(rb)
         vs_oracle_p.append(rp)
+        vs_oracle_self.append(ro)
     B.close()
     t_plain_all = t_plain * jobs / sample
     out = {"jobs": jobs, "points": total, "nnz_sym": info["nnz"], "jobs_done": done,
@@ -177,6 +186,10 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
            "kl_batched_over_plain": {"min": round(float(np.min(ratios)), 5), "mean": round(float(np.mean(ratios)), 5),
                                      "max": round(float(np.max(ratios)), 5)},
            "kl_over_fp64_oracle": {"batched": _ratio_stats(vs_oracle_b), "plain": _ratio_stats(vs_oracle_p),
+                                   "oracle_from_start_one_ulp_away": _ratio_stats(vs_oracle_self),
+                                   "per_job": [{"n": int(sizes[i]), "batched": round(b, 5), "plain": round(p_, 5),
+                                                "oracle_one_ulp": round(o, 5)}
+                                               for i, b, p_, o in zip(pick, vs_oracle_b, vs_oracle_p, vs_oracle_self)],
                                    "sample": "the same sampled jobs, FP64 oracle loop from the same P and start"},
            "graph_replays": info["graph_replays"], "attractive_tuning": tune}
     log(f"[bench] batched: {out}")

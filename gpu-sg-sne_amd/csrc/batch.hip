@@ -457,7 +457,8 @@ int tune(sgtsne_batch *b) {
   const Cand cands[] = {{1, 0, 0},       {2, 0, 0},       {4, 0, 0},       {8, 0, 0},      {16, 0, 0},     {2, 32, 0},
                         {2, 64, 0},
                         {1, 64, 0},      {2, 128, 0},     {1, 128, 0},     {2, 256, 0},    {1, 256, 4096},
-                        {2, 512, 4096},  {1, 512, 8192},  {2, 1024, 8192}};
+                        {2, 512, 4096},  {1, 512, 8192},  {2, 1024, 8192}, {16, 64, 0},     {16, 256, 0},
+                        {8, 256, 0}};
   hipEvent_t e0, e1;
   SGT_HIP(hipEventCreate(&e0));
   SGT_HIP(hipEventCreate(&e1));

@@ -784,7 +784,8 @@ int tune_attractive(sgtsne_ctx *c) {
                         {16, 2, 512, 4096}, {32, 1, 512, 4096}, {16, 1, 512, 8192}, {16, 2, 1024, 8192},
                         {32, 2, 1024, 8192}, {16, 0, 0, -1},    {32, 0, 0, -1},    {64, 0, 0, -1},
                         {16, 8, 0, 0},     {32, 8, 0, 0},      {64, 8, 0, 0},      {16, 16, 0, 0},
-                        {32, 16, 0, 0}};
+                        {32, 16, 0, 0},    {32, 16, 64, 0},    {32, 16, 256, 0},   {16, 16, 128, 0},
+                        {32, 8, 256, 0}};
   if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
   // the column-localised candidates run only when their blocks exist (built
   // when the caller pins that form): on kNN graphs of high-dimensional data

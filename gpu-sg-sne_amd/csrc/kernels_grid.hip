@@ -198,7 +198,9 @@ __global__ __launch_bounds__(kS2GBlock) void s2g_lds_kernel(GridParams g, int n,
 // grid_reduce_kernel adds the blocks' grids in a fixed order (deterministic).
 typedef float mfv4 __attribute__((ext_vector_type(4)));
 constexpr int kMfmaBlock = 256;
-constexpr int kMfmaChunk = 2048;  // points per work-group
+// points per work-group: about 1024 work-groups whatever N, whole batches of
+// 256 (one per wave), between 256 and 4096 points each
+inline int mfma_chunk(int n) { return std::min(4096, std::max(256, div_up(div_up(n, 1024), 256) * 256)); }
 
 // Weight of stencil node f + r of a point at f + 1 + dl (the node factor of
 // lane_node), 0 when r is not 0..3.
@@ -223,30 +225,43 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int 
 #pragma unroll
     for (int t = 0; t < NT * NT; ++t) acc[c][t] = mfv4{0.0f, 0.0f, 0.0f, 0.0f};
   const float2 *Y2 = reinterpret_cast<const float2 *>(Y);
-  for (int base = i0 + 4 * wave; base < i1; base += 16) {
-    const int i = base + k;
-    const bool in = i < i1;
-    const float2 y = in ? Y2[i] : make_float2(g.minv[0], g.minv[1]);
-    float t0 = (y.x - g.minv[0]) * g.scale, t1 = (y.y - g.minv[1]) * g.scale;
-    t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
-    t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
-    const int f0 = (int)t0, f1 = (int)t1;
-    const float d0 = t0 - (float)f0, d1 = t1 - (float)f1;
-    float wx[NT], wy[NT];
+  // each wave takes 64 consecutive points at a time with one coalesced load
+  // (the next 64 are loaded while these are spread); step s of a batch hands
+  // point 4 s + k to lanes 16 k .. 16 k + 15 by cross-lane reads
+  int base = i0 + 64 * wave;
+  float2 nxt = base + lane < i1 ? Y2[base + lane] : make_float2(g.minv[0], g.minv[1]);
+  for (; base < i1; base += 256) {
+    const float2 cur = nxt;
+    const int nb = base + 256;
+    nxt = nb + lane < i1 ? Y2[nb + lane] : make_float2(g.minv[0], g.minv[1]);
+    const int cnt = min(64, i1 - base);
+#pragma unroll 4
+    for (int st = 0; st < 16; ++st) {
+      if (4 * st >= cnt) break;  // wave-uniform
+      const int src = 4 * st + k;
+      const float yx = __shfl(cur.x, src), yy = __shfl(cur.y, src);
+      const bool in = src < cnt;
+      float t0 = (yx - g.minv[0]) * g.scale, t1 = (yy - g.minv[1]) * g.scale;
+      t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
+      t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
+      const int f0 = (int)t0, f1 = (int)t1;
+      const float d0 = t0 - (float)f0, d1 = t1 - (float)f1;
+      float wx[NT], wy[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      wx[t] = in ? lagrange_node(16 * t + node - f0, d0) : 0.0f;
-      wy[t] = lagrange_node(16 * t + node - f1, d1);
-    }
-    const float q1 = y.x - g.center[0], q2 = y.y - g.center[1];
+      for (int t = 0; t < NT; ++t) {
+        wx[t] = in ? lagrange_node(16 * t + node - f0, d0) : 0.0f;
+        wy[t] = lagrange_node(16 * t + node - f1, d1);
+      }
+      const float q1 = yx - g.center[0], q2 = yy - g.center[1];
 #pragma unroll
-    for (int ty = 0; ty < NT; ++ty) {
-      const float b0 = wy[ty], b1 = q1 * wy[ty], b2 = q2 * wy[ty];
+      for (int ty = 0; ty < NT; ++ty) {
+        const float b0 = wy[ty], b1 = q1 * wy[ty], b2 = q2 * wy[ty];
 #pragma unroll
-      for (int tx = 0; tx < NT; ++tx) {
-        acc[0][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b0, acc[0][tx * NT + ty], 0, 0, 0);
-        acc[1][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b1, acc[1][tx * NT + ty], 0, 0, 0);
-        acc[2][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b2, acc[2][tx * NT + ty], 0, 0, 0);
+        for (int tx = 0; tx < NT; ++tx) {
+          acc[0][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b0, acc[0][tx * NT + ty], 0, 0, 0);
+          acc[1][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b1, acc[1][tx * NT + ty], 0, 0, 0);
+          acc[2][tx * NT + ty] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx[tx], b2, acc[2][tx * NT + ty], 0, 0, 0);
+        }
       }
     }
   }
@@ -273,6 +288,102 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int 
     const int gy = cell / ng, gx = cell - gy * ng;
     const int sidx = gy * T + gx;
     out[e] = (red[0][c][sidx] + red[1][c][sidx]) + (red[2][c][sidx] + red[3][c][sidx]);
+  }
+}
+
+// ---- 3-D S2G on the matrix cores (grids of side <= 16) ------------------------------
+// A work-group spreads a chunk of points onto a 16^3 tile grid.  It first sorts
+// its points by the base z-cell of their stencils (a bitonic sort of (z-cell,
+// position) keys in LDS: deterministic), then each of its 4 waves owns 4
+// z-slices.  For 4 points at a time, slice z of channel c gets the rank-4
+// product V_c[gx][gy][z] += sum_k wx_k(gx) (q_c,k wy_k(gy) wz_k(z)) (one
+// v_mfma_f32_16x16x4f32, layouts as in the 2-D form); a slice that none of
+// the 4 points touches is skipped, and in z-sorted order a step touches 4
+// consecutive slices, so a point costs 4 matrix operations over the
+// work-group instead of 64 LDS atomics.  Each wave writes its own slices of
+// the work-group's grid (no reduction between waves); grid_reduce_kernel adds
+// the work-groups' grids in a fixed order.
+constexpr int kMfma3Chunk = 2048;  // points per work-group (a power of two for the sort)
+
+__global__ __launch_bounds__(kMfmaBlock) void s2g_mfma3_kernel(GridParams g, int n, const float *__restrict__ Y,
+                                                               float *__restrict__ part, int chunk) {
+  __shared__ float4 pts[kMfma3Chunk];
+  __shared__ uint32_t key[kMfma3Chunk];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = lane >> 4, node = lane & 15;
+  const int i0 = blockIdx.x * chunk;
+  const int cnt = min(n, i0 + chunk) - i0;
+  const float4 *Y4 = reinterpret_cast<const float4 *>(Y);
+  // load and key: (base z-cell, position); empty slots sort last
+  for (int i = tid; i < kMfma3Chunk; i += kMfmaBlock) {
+    uint32_t kk = 0xFFFFFFFFu;
+    if (i < cnt) {
+      const float4 y = Y4[i0 + i];
+      pts[i] = y;
+      const float t2 = fminf(fmaxf((y.z - g.minv[2]) * g.scale, 0.0f), g.gmax);
+      kk = ((uint32_t)(int)t2 << 16) | (uint32_t)i;
+    }
+    key[i] = kk;
+  }
+  // bitonic sort of the keys, ascending
+  for (int size = 2; size <= kMfma3Chunk; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int i = tid; i < kMfma3Chunk; i += kMfmaBlock) {
+        const int p = i ^ stride;
+        if (p > i) {
+          const uint32_t a = key[i], b = key[p];
+          if ((a > b) == ((i & size) == 0)) {
+            key[i] = b;
+            key[p] = a;
+          }
+        }
+      }
+    }
+  __syncthreads();
+  mfv4 acc[4][4];  // [slice 4 wave + r][channel]
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = mfv4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int st = 0; 4 * st < cnt; ++st) {
+    const int src = 4 * st + k;
+    const bool in = src < cnt;
+    const float4 y = pts[in ? (key[src] & 0xFFFFu) : 0];
+    float t0 = (y.x - g.minv[0]) * g.scale, t1 = (y.y - g.minv[1]) * g.scale, t2 = (y.z - g.minv[2]) * g.scale;
+    t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
+    t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
+    t2 = fminf(fmaxf(t2, 0.0f), g.gmax);
+    const int f0 = (int)t0, f1 = (int)t1, f2 = (int)t2;
+    const float d0 = t0 - (float)f0, d1 = t1 - (float)f1, d2 = t2 - (float)f2;
+    const float wx = in ? lagrange_node(node - f0, d0) : 0.0f;
+    const float wy = lagrange_node(node - f1, d1);
+    const float q1 = y.x - g.center[0], q2 = y.y - g.center[1], q3 = y.z - g.center[2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float wz = in ? lagrange_node(4 * wave + r - f2, d2) : 0.0f;
+      if (__ballot(wz != 0.0f) == 0ull) continue;  // no point of the step touches this slice
+      const float b = wy * wz;
+      acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, b, acc[r][0], 0, 0, 0);
+      acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q1 * b, acc[r][1], 0, 0, 0);
+      acc[r][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q2 * b, acc[r][2], 0, 0, 0);
+      acc[r][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q3 * b, acc[r][3], 0, 0, 0);
+    }
+  }
+  // this wave's slices of the work-group's grid (channel-major, x fastest)
+  const int ng = g.ng, G = ng * ng * ng;
+  float *out = part + (int64_t)blockIdx.x * 4 * G;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int z = 4 * wave + r;
+    if (z >= ng) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gx = 4 * k + v, gy = node;
+        if (gx < ng && gy < ng) out[(int64_t)c * G + (z * ng + gy) * ng + gx] = acc[r][c][v];
+      }
   }
 }
 
@@ -627,14 +738,27 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   if (n > 0 && g.d == 2 && g.ng <= 32 && mode != SGTSNE_SPREAD_ATOMIC) {
     // matrix cores: no contention however the points crowd the few cells
     const int64_t cells = 3ll * g.ng * g.ng;
-    const int blocks = div_up(n, kMfmaChunk);
+    const int chunk = mfma_chunk(n);
+    const int blocks = div_up(n, chunk);
     if (needed) *needed = (int64_t)blocks * cells;
     if (query) return SGTSNE_OK;
     SGT_CHECK((int64_t)blocks * cells <= scratch_floats, SGTSNE_ERR_STATE, "spread scratch too small");
     if (g.ng <= 16)
-      s2g_mfma_kernel<1><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, kMfmaChunk);
+      s2g_mfma_kernel<1><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, chunk);
     else
-      s2g_mfma_kernel<2><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, kMfmaChunk);
+      s2g_mfma_kernel<2><<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, chunk);
+    SGT_LAUNCH_CHECK();
+    grid_reduce_kernel<<<div_up(cells, 64), 1024, 0, s>>>(scratch, blocks, (int)cells, V);
+    SGT_LAUNCH_CHECK();
+    return SGTSNE_OK;
+  }
+  if (n > 0 && g.d == 3 && g.ng <= 16 && mode != SGTSNE_SPREAD_ATOMIC) {
+    const int64_t cells = 4ll * g.ng * g.ng * g.ng;
+    const int blocks = div_up(n, kMfma3Chunk);
+    if (needed) *needed = (int64_t)blocks * cells;
+    if (query) return SGTSNE_OK;
+    SGT_CHECK((int64_t)blocks * cells <= scratch_floats, SGTSNE_ERR_STATE, "spread scratch too small");
+    s2g_mfma3_kernel<<<blocks, kMfmaBlock, 0, s>>>(g, n, Y, scratch, kMfma3Chunk);
     SGT_LAUNCH_CHECK();
     grid_reduce_kernel<<<div_up(cells, 64), 1024, 0, s>>>(scratch, blocks, (int)cells, V);
     SGT_LAUNCH_CHECK();

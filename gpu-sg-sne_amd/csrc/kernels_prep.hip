@@ -307,6 +307,109 @@ __global__ void knn_center_kernel(const double *__restrict__ X, int n, int dim, 
   atomicMax(rmax, (unsigned long long)__double_as_longlong(r));
 }
 
+// ---- block-centred queries ----------------------------------------------------
+// The filter's error bound grows with the size of the vectors whose dot
+// product it takes, (|a| + |b|)^2, while neighbour gaps are set by local
+// distances: with the points centred once globally, clusters far from the
+// mean certify no row.  So queries are grouped into blocks of kF2Q that lie
+// close together (in the order of a Morton code of a 3-D projection of the
+// centred points), and each block is centred at its own mean c_b: the queries
+// as fl32(x - c_b), the references of every tile as fl32(fl32(x - mu) -
+// fl32(c_b - mu)), with their norms summed in the filter.  The certificate
+// then involves only the query's distance to its block's mean.
+__device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// +-1/sqrt(dim) entries of three fixed projection directions (a hash of the
+// axis and the dimension), so every projection is at most the point's norm.
+__device__ __forceinline__ float proj_sign(int axis, int l) {
+  uint32_t h = (uint32_t)l * 2654435761u + (uint32_t)axis * 40503u + 0x9E3779B9u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  return (h & 1u) ? 1.0f : -1.0f;
+}
+
+__global__ void knn_morton_key_kernel(const float *__restrict__ Xc, int n, int dim, int dp,
+                                      const unsigned long long *__restrict__ rmax, uint32_t *__restrict__ key,
+                                      int32_t *__restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float R = (float)__longlong_as_double((long long)*rmax);
+  const float scale = R > 0.0f ? 512.0f / (R * sqrtf((float)dim)) : 0.0f;
+  float p[3] = {0.0f, 0.0f, 0.0f};
+  for (int l = 0; l < dim; ++l) {
+    const float v = Xc[(size_t)i * dp + l];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] += v * proj_sign(a, l);
+  }
+  uint32_t k = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float u = fminf(fmaxf(p[a] * scale + 512.0f, 0.0f), 1023.0f);
+    k |= spread_bits10((uint32_t)u) << a;
+  }
+  key[i] = k;
+  idx[i] = i;
+}
+
+// One work-group per query block: the block's mean c_b (FP64) over its valid
+// slots, delta_b = fl32(c_b - mu) padded to dp, and |delta_b|.
+__global__ __launch_bounds__(256) void knn_block_centre_kernel(const double *__restrict__ X, int n, int dim, int dp,
+                                                               const int32_t *__restrict__ qorder,
+                                                               const double *__restrict__ mu,
+                                                               double *__restrict__ cb, float *__restrict__ delta,
+                                                               double *__restrict__ dn, int bq) {
+  __shared__ double part[4];
+  const int b = blockIdx.x;
+  const int s0 = b * bq, s1 = min(n, s0 + bq);
+  double ss = 0.0;
+  for (int l = threadIdx.x; l < dp; l += blockDim.x) {
+    float dv = 0.0f;
+    if (l < dim) {
+      double c = 0.0;
+      for (int sl = s0; sl < s1; ++sl) c += X[(size_t)qorder[sl] * dim + l];
+      c /= (double)(s1 - s0);
+      cb[(size_t)b * dim + l] = c;
+      dv = (float)(c - mu[l]);
+    }
+    delta[(size_t)b * dp + l] = dv;
+    ss += (double)dv * (double)dv;
+  }
+  ss = wave_sum(ss);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) dn[b] = sqrt(part[0] + part[1] + part[2] + part[3]);
+}
+
+// Queries in slot order, centred at their block's mean: Xq[slot] =
+// fl32(x_q - c_b) (zero-padded), its squared norm (FP32 of the FP64 sum) by
+// slot, and by query id its norm |a| and |delta_b| for the certificate.
+__global__ void knn_query_prep_kernel(const double *__restrict__ X, int n, int dim, int dp,
+                                      const int32_t *__restrict__ qorder, const double *__restrict__ cb,
+                                      const double *__restrict__ dn, float *__restrict__ Xq,
+                                      float *__restrict__ qn32, double *__restrict__ qa, double *__restrict__ qdn,
+                                      int bq) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= n) return;
+  const int q = qorder[sl], b = sl / bq;
+  double ss = 0.0;
+  for (int l = 0; l < dp; ++l) {
+    const float v = l < dim ? (float)(X[(size_t)q * dim + l] - cb[(size_t)b * dim + l]) : 0.0f;
+    Xq[(size_t)sl * dp + l] = v;
+    ss += (double)v * (double)v;
+  }
+  qn32[sl] = (float)ss;
+  qa[q] = sqrt(ss);
+  qdn[q] = dn[b];
+}
+
 // Candidate lists: a max-heap of kp (approximate distance, index) keys per
 // query in global memory (lists longer than the LDS version holds).  Wave w
 // owns queries q0 + 16w .. q0 + 16w + 15.  The points have 4 NST dimensions.
@@ -417,66 +520,106 @@ __global__ __launch_bounds__(256) void knn_filter_kernel(const float *__restrict
 constexpr int kF2Q = 32;
 constexpr int kF2MaxKp = 128;
 
+// A reference tile shifted into a query block's frame: rt holds
+// fl32(x~ - delta) transposed, and rnp[t] the partial squared norm of the
+// row this thread stored (rows t % 64: two threads cover every row, so a
+// row's norm is rnp[j] + rnp[j + 64]).
+template <int NST>
+__device__ __forceinline__ void tile_store_shift(float *rt, float *rnp, const float4 (&buf)[kFiltR * NST / 128],
+                                                 const float4 (&dl)[kFiltR * NST / 128]) {
+  float ss = 0.0f;
+#pragma unroll
+  for (int u = 0; u < kFiltR * NST / 128; ++u) {
+    const int t = threadIdx.x + u * 128;
+    const int j = t % kFiltR, c4 = t / kFiltR;
+    const float4 v = make_float4(buf[u].x - dl[u].x, buf[u].y - dl[u].y, buf[u].z - dl[u].z, buf[u].w - dl[u].w);
+    float *col = rt + 4 * c4 * kFiltRS + j;
+    col[0] = v.x;
+    col[kFiltRS] = v.y;
+    col[2 * kFiltRS] = v.z;
+    col[3 * kFiltRS] = v.w;
+    ss = fmaf(v.x, v.x, ss);
+    ss = fmaf(v.y, v.y, ss);
+    ss = fmaf(v.z, v.z, ss);
+    ss = fmaf(v.w, v.w, ss);
+  }
+  rnp[threadIdx.x] = ss;
+}
+
 template <int NST>
 __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__restrict__ Xc,
-                                                             const float *__restrict__ n32, int n, int kp,
+                                                             const float *__restrict__ Xq,
+                                                             const float *__restrict__ qn32,
+                                                             const int32_t *__restrict__ qorder,
+                                                             const float *__restrict__ delta, int n, int kp,
                                                              float *__restrict__ hd, int *__restrict__ hi) {
+  static_assert((kFiltR * NST) % 128 == 0, "a tile must split evenly over the work-group");
   extern __shared__ float dyn[];  // heap keys [kF2Q][kp] | heap indices [kF2Q][kp]
   float *hk = dyn;
   int *hx = reinterpret_cast<int *>(hk + (size_t)kF2Q * kp);
   __shared__ float rt[4 * NST * kFiltRS];
   __shared__ float dist[kF2Q][kFiltR];
   __shared__ unsigned long long mask[kF2Q];
-  __shared__ float rn[kFiltR];
+  __shared__ float rnp[128];
   __shared__ float qn[kF2Q];
+  __shared__ int qid[kF2Q];
 
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4;
-  const int q0 = blockIdx.x * kF2Q;
+  const int q0 = blockIdx.x * kF2Q;  // first slot of this query block
   float areg[NST];
   {
-    const int qa = q0 + 16 * w + (lane & 15);
+    const int sa = q0 + 16 * w + (lane & 15);
 #pragma unroll
-    for (int st = 0; st < NST; ++st) areg[st] = qa < n ? Xc[(size_t)qa * 4 * NST + 4 * st + (lane >> 4)] : 0.0f;
+    for (int st = 0; st < NST; ++st) areg[st] = sa < n ? Xq[(size_t)sa * 4 * NST + 4 * st + (lane >> 4)] : 0.0f;
+  }
+  // this thread's float4 columns of the block's shift
+  float4 dl[kFiltR * NST / 128];
+  {
+    const float4 *D4 = reinterpret_cast<const float4 *>(delta) + (size_t)blockIdx.x * NST;
+#pragma unroll
+    for (int u = 0; u < kFiltR * NST / 128; ++u) dl[u] = D4[(threadIdx.x + u * 128) / kFiltR];
   }
   for (int t = tid; t < kF2Q * kp; t += 128) {
     hk[t] = INFINITY;
     hx[t] = INT_MAX;
   }
-  if (tid < kF2Q) qn[tid] = q0 + tid < n ? n32[q0 + tid] : 0.0f;
+  if (tid < kF2Q) {
+    qn[tid] = q0 + tid < n ? qn32[q0 + tid] : 0.0f;
+    qid[tid] = q0 + tid < n ? qorder[q0 + tid] : -1;
+  }
   float4 buf[kFiltR * NST / 128];
   float rnv;
-  tile_load<128, NST>(Xc, n32, n, 0, buf, rnv);
+  tile_load<128, NST>(Xc, Xc, n, 0, buf, rnv);
   for (int r0 = 0; r0 < n; r0 += kFiltR) {
     __syncthreads();  // the previous tile's merge is done
-    tile_store<128, NST>(rt, rn, buf, rnv);
+    tile_store_shift<NST>(rt, rnp, buf, dl);
     __syncthreads();
-    if (r0 + kFiltR < n) tile_load<128, NST>(Xc, n32, n, r0 + kFiltR, buf, rnv);
+    if (r0 + kFiltR < n) tile_load<128, NST>(Xc, Xc, n, r0 + kFiltR, buf, rnv);
     fv4 acc[4];
     tile_dots<NST>(areg, rt, lane, acc);
     // this lane's four query rows 16 w + 4 g + v: norms and current worst keys
     float qv[4], thd[4];
-    int thi[4];
-    bool qok[4];
+    int thi[4], qq[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int ql = 16 * w + 4 * g + v;
       qv[v] = qn[ql];
       thd[v] = hk[ql * kp];
       thi[v] = hx[ql * kp];
-      qok[v] = q0 + ql < n;
+      qq[v] = qid[ql];
     }
     unsigned long long mq[4] = {0ull, 0ull, 0ull, 0ull};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int jl = 16 * t + (lane & 15), r = r0 + jl;
-      const float rv = rn[jl];
+      const float rv = rnp[jl] + rnp[jl + 64];
       const bool rok = r < n;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int ql = 16 * w + 4 * g + v;
         const float dd = (qv[v] + rv) - 2.0f * acc[t][v];
         dist[ql][jl] = dd;
-        const bool cand = qok[v] && rok && r != q0 + ql && key_lessf(dd, r, thd[v], thi[v]);
+        const bool cand = qq[v] >= 0 && rok && r != qq[v] && key_lessf(dd, r, thd[v], thi[v]);
         const unsigned long long b = __ballot(cand);
         mq[v] |= ((b >> (16 * g)) & 0xFFFFull) << (16 * t);
       }
@@ -515,8 +658,8 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__rest
   }
   __syncthreads();
   for (int t = tid; t < kF2Q * kp; t += 128) {
-    const int ql = t / kp, q = q0 + ql;
-    if (q < n) {
+    const int ql = t / kp, q = qid[ql];
+    if (q >= 0) {
       hd[(size_t)q * kp + (t - ql * kp)] = hk[t];
       hi[(size_t)q * kp + (t - ql * kp)] = hx[t];
     }
@@ -524,11 +667,21 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__rest
 }
 
 // Step counts (dimensions / 4, rounded up) the filters are built for.
+// The block-centred arrays of the LDS-heap filter (kp <= kF2MaxKp).
+struct BlockCentred {
+  const float *Xq;         // queries in slot order, centred at their block's mean
+  const float *qn32;       // their squared norms
+  const int32_t *qorder;   // slot -> query id
+  const float *delta;      // per block: fl32(c_b - mu), padded
+};
+
 template <int NST>
-int launch_filter(hipStream_t s, const float *Xc, const float *n32, int n, int kp, float *hd, int *hi) {
+int launch_filter(hipStream_t s, const float *Xc, const float *n32, int n, int kp, const BlockCentred &bc,
+                  float *hd, int *hi) {
   if (kp <= kF2MaxKp) {
     const size_t lds = (sizeof(float) + sizeof(int)) * (size_t)kF2Q * kp;
-    knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(Xc, n32, n, kp, hd, hi);
+    knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(Xc, bc.Xq, bc.qn32, bc.qorder, bc.delta, n, kp,
+                                                                 hd, hi);
   } else {
     if constexpr ((kFiltR * NST) % 256 == 0) {
       knn_filter_kernel<NST><<<div_up(n, kFiltQ), 256, 0, s>>>(Xc, n32, n, kp, hd, hi);
@@ -551,15 +704,16 @@ int filter_steps(int dim, int kp) {
   return nst <= 4 ? 4 : nst <= 8 ? 8 : nst <= 12 ? 12 : nst <= 16 ? 16 : nst <= 24 ? 24 : 32;
 }
 
-int launch_filter_for(hipStream_t s, int nst, const float *Xc, const float *n32, int n, int kp, float *hd, int *hi) {
+int launch_filter_for(hipStream_t s, int nst, const float *Xc, const float *n32, int n, int kp,
+                      const BlockCentred &bc, float *hd, int *hi) {
   switch (nst) {
-    case 4: return launch_filter<4>(s, Xc, n32, n, kp, hd, hi);
-    case 8: return launch_filter<8>(s, Xc, n32, n, kp, hd, hi);
-    case 12: return launch_filter<12>(s, Xc, n32, n, kp, hd, hi);
-    case 14: return launch_filter<14>(s, Xc, n32, n, kp, hd, hi);
-    case 16: return launch_filter<16>(s, Xc, n32, n, kp, hd, hi);
-    case 24: return launch_filter<24>(s, Xc, n32, n, kp, hd, hi);
-    default: return launch_filter<32>(s, Xc, n32, n, kp, hd, hi);
+    case 4: return launch_filter<4>(s, Xc, n32, n, kp, bc, hd, hi);
+    case 8: return launch_filter<8>(s, Xc, n32, n, kp, bc, hd, hi);
+    case 12: return launch_filter<12>(s, Xc, n32, n, kp, bc, hd, hi);
+    case 14: return launch_filter<14>(s, Xc, n32, n, kp, bc, hd, hi);
+    case 16: return launch_filter<16>(s, Xc, n32, n, kp, bc, hd, hi);
+    case 24: return launch_filter<24>(s, Xc, n32, n, kp, bc, hd, hi);
+    default: return launch_filter<32>(s, Xc, n32, n, kp, bc, hd, hi);
   }
 }
 
@@ -591,14 +745,22 @@ __device__ void bitonic_sort(K *key, int *id) {
 // One work-group per query: certify the candidate list, recompute the
 // candidates' distances as knn_kernel does (diff = x_q - x_c in FP64,
 // acc = fma(diff, diff, acc) over the dimensions in order) and write the row.
-// cE, ce: the filter's error in squared distance is at most cE (|x_q| + R)^2
-// and the FP32 rounding of the centred points moves a distance by at most
-// ce (|x_q| + R), with R the largest centred norm.
+//
+// The filter compares a = fl32(x_q - c) with b = fl32(fl32(x_c - mu) -
+// fl32(c - mu)) for the frame c of the query's block (c = mu, no shift, when
+// qdn is 0): F = |a|^2 + |b|^2 - 2 a.b is within E = cE (|a| + |b|)^2 of
+// |a - b|^2, and |a - b| is within e = ce (|a| + R + |c - mu| + |b|) of the
+// true distance T^(1/2), R the largest |x - mu|.  The k nearest candidates
+// (F <= F_k) lie within hi = sqrt(F_k + E) + e with |b| <= |a| + sqrt(F_k) + 1;
+// a point outside the list within hi would have |b| <= |a| + hi + e + 1 and
+// so F <= (hi + e)^2 + E, yet every non-candidate has F >= F_kp: when F_kp
+// exceeds that bound (with a 1e-12 relative margin for the FP64 rounding of
+// the recomputed distances), the list holds the k nearest.
 __global__ __launch_bounds__(kRerank) void knn_rerank_kernel(
     const double *__restrict__ X, int n, int dim, int k, int kp, const float *__restrict__ hd,
-    const int *__restrict__ hi, const double *__restrict__ nrm, const unsigned long long *__restrict__ rmax,
-    double cE, double ce, int32_t *__restrict__ I, double *__restrict__ D, int *__restrict__ fb_list,
-    int *__restrict__ fb_cnt) {
+    const int *__restrict__ hi, const double *__restrict__ qa, const double *__restrict__ qdn,
+    const unsigned long long *__restrict__ rmax, double cE, double ce, int32_t *__restrict__ I,
+    double *__restrict__ D, int *__restrict__ fb_list, int *__restrict__ fb_cnt) {
   __shared__ float sk[kRerank];
   __shared__ double sd[kRerank];
   __shared__ int si[kRerank];
@@ -611,11 +773,15 @@ __global__ __launch_bounds__(kRerank) void knn_rerank_kernel(
     bool good = kp >= n - 1;  // every other point is a candidate
     if (!good) {
       const double R = __longlong_as_double((long long)*rmax);
-      const double sn = nrm[q] + R;
-      const double E = cE * sn * sn, e = ce * sn;
-      const double lo = sqrt(fmax((double)sk[kp - 1] - E, 0.0)) - e;  // any non-candidate is at least this far
-      const double hi_k = sqrt(fmax((double)sk[k - 1] + E, 0.0)) + e;  // the k nearest candidates are within this
-      good = lo > hi_k * (1.0 + 1e-12) + 1e-300;
+      const double a = qa[q], dl = qdn[q];
+      const double fk = fmax((double)sk[k - 1], 0.0), fkp = (double)sk[kp - 1];
+      const double bk = a + sqrt(fk) + 1.0;
+      const double ek = ce * (a + R + dl + bk);
+      const double hik = (sqrt(fk + cE * (a + bk) * (a + bk)) + ek) * (1.0 + 1e-12) + 1e-300;
+      const double bn = a + hik + ek + 1.0;
+      const double en = ce * (a + R + dl + bn);
+      const double fmaxn = (hik + en) * (hik + en) + cE * (a + bn) * (a + bn);
+      good = fkp > fmaxn;  // false for NaN keys too
     }
     ok = good;
   }
@@ -955,9 +1121,14 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
             kRerank - kFiltSlack, kFiltMaxDp, kFiltQ);
   const int kp = std::min(k + kFiltSlack, n - 1);
   const int nst = filter_steps(dim, kp), dp = 4 * nst;
-  double *mu = nullptr, *nrm = nullptr;
-  float *Xc = nullptr, *n32 = nullptr, *hd = nullptr;
+  const bool blocked = kp <= kF2MaxKp;  // the LDS-heap filter centres each query block
+  const int nb = div_up(n, kF2Q);
+  double *mu = nullptr, *nrm = nullptr, *cb = nullptr, *dn = nullptr, *qa = nullptr, *qdn = nullptr;
+  float *Xc = nullptr, *n32 = nullptr, *hd = nullptr, *Xq = nullptr, *qn32 = nullptr, *delta = nullptr;
   int *hi = nullptr, *fb = nullptr;
+  int32_t *kidx = nullptr, *qorder = nullptr;
+  uint32_t *key0 = nullptr, *key1 = nullptr;
+  void *tmp = nullptr;
   unsigned long long *rmax = nullptr;
   SGT_HIP(hipMallocAsync((void **)&mu, sizeof(double) * dim, s));
   SGT_HIP(hipMallocAsync((void **)&nrm, sizeof(double) * (size_t)n, s));
@@ -967,21 +1138,50 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   SGT_HIP(hipMallocAsync((void **)&hi, sizeof(int) * (size_t)n * kp, s));
   SGT_HIP(hipMallocAsync((void **)&fb, sizeof(int) * ((size_t)n + 1), s));  // [0]: count, then rows
   SGT_HIP(hipMallocAsync((void **)&rmax, sizeof(unsigned long long), s));
+  SGT_HIP(hipMallocAsync((void **)&qdn, sizeof(double) * (size_t)n, s));
   SGT_HIP(hipMemsetAsync(rmax, 0, sizeof(unsigned long long), s));
   SGT_HIP(hipMemsetAsync(fb, 0, sizeof(int), s));
   knn_mean_kernel<<<dim, 256, 0, s>>>(X, n, dim, mu);
   SGT_LAUNCH_CHECK();
   knn_center_kernel<<<div_up(n, 256), 256, 0, s>>>(X, n, dim, dp, mu, Xc, n32, nrm, rmax);
   SGT_LAUNCH_CHECK();
-  SGT_TRY(launch_filter_for(s, nst, Xc, n32, n, kp, hd, hi));
+  BlockCentred bc{};
+  if (blocked) {
+    SGT_HIP(hipMallocAsync((void **)&key0, sizeof(uint32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&key1, sizeof(uint32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&kidx, sizeof(int32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&qorder, sizeof(int32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&cb, sizeof(double) * (size_t)nb * dim, s));
+    SGT_HIP(hipMallocAsync((void **)&delta, sizeof(float) * (size_t)nb * dp, s));
+    SGT_HIP(hipMallocAsync((void **)&dn, sizeof(double) * (size_t)nb, s));
+    SGT_HIP(hipMallocAsync((void **)&Xq, sizeof(float) * (size_t)n * dp, s));
+    SGT_HIP(hipMallocAsync((void **)&qn32, sizeof(float) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&qa, sizeof(double) * (size_t)n, s));
+    // query order: Morton code of a 3-D projection of the centred points
+    knn_morton_key_kernel<<<div_up(n, 256), 256, 0, s>>>(Xc, n, dim, dp, rmax, key0, kidx);
+    SGT_LAUNCH_CHECK();
+    size_t tb = 0;
+    SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0, key1, kidx, qorder, n, 0, 30, s));
+    SGT_HIP(hipMallocAsync(&tmp, tb, s));
+    SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key0, key1, kidx, qorder, n, 0, 30, s));
+    knn_block_centre_kernel<<<nb, 256, 0, s>>>(X, n, dim, dp, qorder, mu, cb, delta, dn, kF2Q);
+    SGT_LAUNCH_CHECK();
+    knn_query_prep_kernel<<<div_up(n, 256), 256, 0, s>>>(X, n, dim, dp, qorder, cb, dn, Xq, qn32, qa, qdn, kF2Q);
+    SGT_LAUNCH_CHECK();
+    bc = BlockCentred{Xq, qn32, qorder, delta};
+  } else {
+    SGT_HIP(hipMemsetAsync(qdn, 0, sizeof(double) * (size_t)n, s));  // frame of every query: mu
+  }
+  SGT_TRY(launch_filter_for(s, nst, Xc, n32, n, kp, bc, hd, hi));
   // error constants (unit roundoff u = 2^-24): the squared distance from the
-  // filter is within (2 dp + 3) u (|x| + |y|)^2 of that of the FP32 points
+  // filter is within (2 dp + 3) u (|a| + |b|)^2 of that of the FP32 vectors
   // under round-to-nearest; allowing twice that covers directed rounding in
-  // the accumulation, and 1e-30 covers underflow.  Rounding the centred points
-  // to FP32 moves a distance by at most 1.01 u (|x| + |y|).
+  // the accumulation, and 1e-30 covers underflow.  The FP32 roundings of the
+  // centred points move a distance by at most 1.01 u (|a| + R + |c - mu| + |b|).
   const double u = std::ldexp(1.0, -24);
   const double cE = 2.0 * (2.0 * dp + 8.0) * u, ce = 1.05 * u;
-  knn_rerank_kernel<<<n, kRerank, 0, s>>>(X, n, dim, k, kp, hd, hi, nrm, rmax, cE, ce, I, D, fb + 1, fb);
+  knn_rerank_kernel<<<n, kRerank, 0, s>>>(X, n, dim, k, kp, hd, hi, blocked ? qa : nrm, qdn, rmax, cE, ce, I, D,
+                                          fb + 1, fb);
   SGT_LAUNCH_CHECK();
   int nfb = 0;
   SGT_HIP(hipMemcpyAsync(&nfb, fb, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -989,14 +1189,9 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   int st = SGTSNE_OK;
   if (nfb > 0) st = knn_exact_rows(s, X, n, dim, k, fb + 1, nfb, I, D);
   if (fallback_rows) *fallback_rows = nfb;
-  (void)hipFreeAsync(mu, s);
-  (void)hipFreeAsync(nrm, s);
-  (void)hipFreeAsync(Xc, s);
-  (void)hipFreeAsync(n32, s);
-  (void)hipFreeAsync(hd, s);
-  (void)hipFreeAsync(hi, s);
-  (void)hipFreeAsync(fb, s);
-  (void)hipFreeAsync(rmax, s);
+  void *bufs[] = {mu, nrm, Xc, n32, hd, hi, fb, rmax, qdn, key0, key1, kidx, qorder, cb, delta, dn, Xq, qn32, qa, tmp};
+  for (void *p : bufs)
+    if (p) (void)hipFreeAsync(p, s);
   return st;
 }
 

@@ -22,18 +22,23 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// The pair term with every rounding spelled out (explicit fused multiply-adds,
+// the first square rounded on its own): the compiler's choice of which
+// product to fuse would otherwise depend on the code around each inlined copy,
+// and the kernel forms, which add the same entries in the same order, would
+// differ in the last bit.
 template <int D>
 __device__ __forceinline__ void accum_pair(const float (&yi)[D], const float (&yj)[D], float p,
                                            float (&acc)[D]) {
-  float df[D], dist = 0.0f;
+  float df[D];
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    df[a] = yi[a] - yj[a];
-    dist += df[a] * df[a];
-  }
+  for (int a = 0; a < D; ++a) df[a] = yi[a] - yj[a];
+  float dist = df[0] * df[0];
+#pragma unroll
+  for (int a = 1; a < D; ++a) dist = __builtin_fmaf(df[a], df[a], dist);
   const float pq = p / (1.0f + dist);
 #pragma unroll
-  for (int a = 0; a < D; ++a) acc[a] += pq * df[a];
+  for (int a = 0; a < D; ++a) acc[a] = __builtin_fmaf(pq, df[a], acc[a]);
 }
 
 // Format 4 (default): a group of G lanes per row walks the row's contiguous
@@ -67,6 +72,56 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg_kernel(int n, const int32
   }
 }
 
+// One lane's share of a row: entries t0, t0 + G, ... < e, added in that
+// order.  U >= 8: masked steps, a lane's next U entries (short rows: all of
+// them) have their column and value loads, then their gathers, in flight
+// together.  U < 8: U strides per step, then the remainder one by one.
+// (col, val) are streamed once per iteration with nontemporal loads and do
+// not evict Y from the caches.
+template <int D, int G, int U>
+__device__ __forceinline__ void row_accum(int t, const int e, const int32_t *__restrict__ col,
+                                          const float *__restrict__ val, const float *__restrict__ Y,
+                                          const float (&yi)[D], float (&acc)[D]) {
+  if constexpr (U >= 8) {
+    for (; t < e; t += U * G) {
+      int c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool in = t + u * G < e;
+        c[u] = in ? __builtin_nontemporal_load(col + t + u * G) : -1;
+        v[u] = in ? __builtin_nontemporal_load(val + t + u * G) : 0.0f;
+      }
+      float yj[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c[u] >= 0) load_pt<D>(Y, c[u], yj[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c[u] >= 0) accum_pair<D>(yi, yj[u], v[u], acc);
+    }
+  } else {
+    for (; t + (U - 1) * G < e; t += U * G) {
+      int c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
+      float yj[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
+    }
+    for (; t < e; t += G) {
+      float yj[D];
+      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
+    }
+  }
+}
+
 // Format 4, tuned form: as above, but each lane issues the (col, val) loads of
 // U strides before the U neighbour gathers, so U independent loads per lane
 // are in flight; (col, val) are streamed once per iteration with nontemporal
@@ -86,49 +141,7 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg2_kernel(int n, const int3
     float acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) acc[a] = 0.0f;
-    const int e = rowptr[row + 1];
-    int t = rowptr[row] + lane;
-    if constexpr (U >= 8) {
-      // masked steps: a lane's next U entries (short rows: all of them) have
-      // their column and value loads, then their gathers, in flight together;
-      // the entries are added in the same order as below
-      for (; t < e; t += U * G) {
-        int c[U];
-        float v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool in = t + u * G < e;
-          c[u] = in ? __builtin_nontemporal_load(col + t + u * G) : -1;
-          v[u] = in ? __builtin_nontemporal_load(val + t + u * G) : 0.0f;
-        }
-        float yj[U][D];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (c[u] >= 0) load_pt<D>(Y, c[u], yj[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (c[u] >= 0) accum_pair<D>(yi, yj[u], v[u], acc);
-      }
-    } else {
-      for (; t + (U - 1) * G < e; t += U * G) {
-        int c[U];
-        float v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
-        float yj[U][D];
-#pragma unroll
-        for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
-      }
-      for (; t < e; t += G) {
-        float yj[D];
-        load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
-        accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
-      }
-    }
+    row_accum<D, G, U>(rowptr[row] + lane, rowptr[row + 1], col, val, Y, yi, acc);
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -165,26 +178,7 @@ __global__ __launch_bounds__(kBlock) void attr_csr_blk_kernel(int n, int rows_pe
     float acc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) acc[a] = 0.0f;
-    const int e = rowptr[row + 1];
-    int t = rowptr[row] + lane;
-    for (; t + (U - 1) * G < e; t += U * G) {
-      int c[U];
-      float v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(col + t + u * G);
-#pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
-      float yj[U][D];
-#pragma unroll
-      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
-    }
-    for (; t < e; t += G) {
-      float yj[D];
-      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
-      accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
-    }
+    row_accum<D, G, U>(rowptr[row] + lane, rowptr[row + 1], col, val, Y, yi, acc);
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -552,8 +546,13 @@ int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_pe
   using I16 = std::integral_constant<int, 16>;
   using I32 = std::integral_constant<int, 32>;
   using I64 = std::integral_constant<int, 64>;
+  using I8 = std::integral_constant<int, 8>;
   auto per_g = [&](auto dd, auto uu) { go(dd, I16{}, uu); go(dd, I32{}, uu); go(dd, I64{}, uu); };
-  auto per_d = [&](auto dd) { per_g(dd, I1{}); per_g(dd, I2{}); };
+  auto per_d = [&](auto dd) {
+    per_g(dd, I1{});
+    per_g(dd, I2{});
+    go(dd, I16{}, I8{}); go(dd, I32{}, I8{}); go(dd, I16{}, I16{}); go(dd, I32{}, I16{});
+  };
   per_d(I1{});
   per_d(I2{});
   per_d(I3{});

@@ -68,7 +68,7 @@ def test_batch_terms_vs_oracle(sg, O, d):
 # of Y in LDS; batch.hip tune()); unroll 8 and 16 are the masked-step forms
 _BATCH_CANDIDATES = [(1, 0, 0), (2, 0, 0), (4, 0, 0), (8, 0, 0), (16, 0, 0), (2, 32, 0), (2, 64, 0), (1, 64, 0),
                      (2, 128, 0), (1, 128, 0), (2, 256, 0), (1, 256, 4096), (2, 512, 4096), (1, 512, 8192),
-                     (2, 1024, 8192)]
+                     (2, 1024, 8192), (16, 64, 0), (16, 256, 0), (8, 256, 0)]
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
@@ -343,7 +343,7 @@ def test_batch_staging_ring_wrap_keeps_uploads(sg, O):
     col = np.stack([(idx - 1) % n, (idx + 1) % n], 1).reshape(-1).astype(np.int32)
     val = np.full(2 * n, 0.5 / n, np.float32)
     y0 = np.random.default_rng(9).standard_normal((n, 2)).astype(np.float32)
-    with _batch(sg, max_jobs=4, points=n + 8192, nnz=2 * n + 2_000_000) as B:
+    with _batch(sg, max_jobs=4, points=n + 8192, nnz=2 * n + 2_000_000, layout=sg.LAYOUT_AOS) as B:
         B.add(S_small, symmetrize=False)
         B.step(60)  # queued: the big job's copies wait behind these
         big = B.add(sg.CSR(n, rp, col, val), symmetrize=False, y_in=y0, max_iter=2)

@@ -29,7 +29,8 @@ def _clustered(n, d, rng, spread=30.0, clusters=12):
 
 
 def _check_terms(O, ctx, So, Y, h, tag):
-    ctx.set_Y(Y)
+    if Y is not None:  # else the context's current iterate
+        ctx.set_Y(Y)
     Yc = ctx.get_Y()
     Fa, Fr, Z, ng = ctx.terms()
     Yd = Yc.astype(np.float64)
@@ -87,6 +88,11 @@ def test_c3_knn_rows_sample(sg, O, c3):
 
 @pytest.mark.parametrize("d", [2, 3])
 def test_c3_terms_full_size(sg, O, c3, d):
+    # the terms at a spread-out start and at the run's own iterates (after the
+    # exaggeration phase and at the end of a 1000-iteration run), each from
+    # the same P_sym and Y as the FP64 oracle
+    from sgtsne_amd import synth
+
     X, _, _, P = c3
     n = X.shape[0]
     h = {2: 0.7, 3: 1.2}[d]
@@ -97,8 +103,12 @@ def test_c3_terms_full_size(sg, O, c3, d):
         assert abs(S.val.sum(dtype=np.float64) - 1.0) < 1e-4
         So = O.CSR(n, S.rowptr.astype(np.int64), S.col, S.val.astype(np.float64))
         del S
-        for t, Y in enumerate([(rng.standard_normal((n, d)) * 3.0).astype(np.float32), _clustered(n, d, rng)]):
-            _check_terms(O, ctx, So, Y, h, f"C3 d={d} case {t}")
+        _check_terms(O, ctx, So, (rng.standard_normal((n, d)) * 3.0).astype(np.float32), h, f"C3 d={d} spread start")
+        ctx.set_Y(synth.random_init(n, d, seed=1))
+        ctx.step(300)
+        _check_terms(O, ctx, So, None, h, f"C3 d={d} iteration 300")
+        ctx.step(700)
+        _check_terms(O, ctx, So, None, h, f"C3 d={d} iteration 1000")
 
 
 # ---------------------------------------------------------------- C5

@@ -293,6 +293,32 @@ def test_terms_fixed_grid(sg, O, d):
 
 
 @pytest.mark.parametrize("spread", ["AUTO", "ATOMIC", "MFMA"])
+def test_terms_spread_modes_3d(sg, O, spread):
+    # 3-D: the matrix-core form on grids of side 16, and the fallback above
+    n = 6001
+    X = _mixture(n, 20, 5, 79)
+    I, D = O.knn_exact(X, 30)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 10.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(80)
+    cases = [(rng.standard_normal((n, 3)) * 1e-4).astype(np.float32),   # ng 16
+             (rng.standard_normal((n, 3)) * 2.0).astype(np.float32),    # ng 16
+             _clustered_embedding(n, 3, rng, spread=20.0)]              # ng > 16
+    sides = []
+    with sg.Context(sg.tsneparams(d=3, spread=getattr(sg, "SPREAD_" + spread))) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        for Y in cases:
+            ctx.set_Y(Y)
+            Yc = ctx.get_Y()
+            _, Fr, Z, ng = ctx.terms()
+            Fro, Zo, ngo = O.repulsive_interp(Yc.astype(np.float64), 1.2, O.ZETA_DIRECT)
+            assert ng == ngo
+            assert rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL, (spread, ng)
+            sides.append(ng)
+    assert sides[0] == 16 and sides[1] == 16 and sides[2] > 16, sides
+
+
+@pytest.mark.parametrize("spread", ["AUTO", "ATOMIC", "MFMA"])
 def test_terms_spread_modes(sg, O, spread):
     # every spread path against the oracle on 2-D grids of side 16, 20-32
     # (the matrix-core form: one and two 16-node tiles per axis) and larger
@@ -507,7 +533,7 @@ _SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (64, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0
            (32, 1, 512, 4096), (16, 0, 0, -1), (32, 0, 0, -1), (64, 0, 0, -1),
            (16, 8, 0, 0), (32, 8, 0, 0), (64, 8, 0, 0), (16, 16, 0, 0), (32, 16, 0, 0),
            (32, 2, 0, 0), (32, 4, 0, 0), (64, 2, 0, 0), (32, 1, 64, 0), (32, 2, 64, 0), (32, 2, 1024, 8192), (64, 1, 256, 0),
-           (16, 1, 512, 8192)]
+           (16, 1, 512, 8192), (32, 16, 64, 0), (32, 16, 256, 0), (16, 16, 128, 0), (32, 8, 256, 0)]
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
