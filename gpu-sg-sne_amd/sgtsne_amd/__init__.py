@@ -52,7 +52,7 @@ EXPORTED = [
     "sgtsne_batch_set_graph", "sgtsne_batch_set_attractive", "sgtsne_copy_probe", "sgtsne_locality_order",
     "sgtsne_free", "sgtsne_read_X_text", "sgtsne_read_P_mtx", "sgtsne_write_embedding_text",
     "sgtsne_write_embedding_text_f64", "sgtsne_write_embedding_bin", "sgtsne_write_time_info",
-    "sgtsne_simulation_data", "sgtsne_random_start",
+    "sgtsne_simulation_data", "sgtsne_random_start", "sgtsne_grid_conv_noalloc",
 ]
 
 
@@ -159,6 +159,7 @@ def lib():
         "sgtsne_write_time_info": (C.c_int, [C.c_char_p, f64p, C.c_int, C.c_int, C.c_int]),
         "sgtsne_simulation_data": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p]),
         "sgtsne_random_start": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p]),
+        "sgtsne_grid_conv_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -152,6 +152,18 @@ int sgtsne_perplexity_equalize(const int32_t *I, const double *D, int n, int k, 
  * (1 + h^2 r^2)^-1.  4 <= ng <= 512. */
 int sgtsne_grid_conv(int d, int ng, double h, int zdirect, const float *V, float *Phi);
 
+/* sgtsne_grid_conv without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  V and Phi
+ * are DEVICE arrays of the caller (same layouts as sgtsne_grid_conv);
+ * `stream` is a hipStream_t (NULL: the null stream) and the call returns once
+ * the work is queued on it.  The FFT workspace comes from one 32 MiB device
+ * array of the library per GPU, held until the process ends, which fits every
+ * 1-D and 2-D grid (ng <= 512) and 3-D grids of side ng <= 64; larger grids
+ * return SGTSNE_ERR_ARG.  Every call on a GPU shares that array, so calls
+ * that overlap on one GPU (other streams or threads) race: order them. */
+int sgtsne_grid_conv_noalloc(int d, int ng, double h, int zdirect, const float *V, float *Phi, void *stream);
+
 /* ---- drop-in entry -------------------------------------------------- */
 
 /* Runs the whole method: lambda rescaling (if lambda != 1), P + P^T and
