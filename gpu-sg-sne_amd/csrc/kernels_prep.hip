@@ -1222,12 +1222,20 @@ int launch_rowids(hipStream_t s, int n, const int32_t *rowptr, int32_t *row) {
 // Keeps memory freed to the device's stream-ordered pool cached in the pool:
 // with the default release threshold (0) every synchronisation returns it to
 // the driver, and the next hipMallocAsync blocks the host to map it again.
+// Cross-stream reuse is limited to what stream order and recorded events
+// prove: a block freed on one stream is handed to another only through an
+// event dependency the caller made, never because the runtime saw the free
+// complete (opportunistic) or inserted a wait of its own (internal
+// dependencies).
 void retain_pool_memory() {
   int dev = 0;
   hipMemPool_t pool = nullptr;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
   uint64_t keep = UINT64_MAX;
   (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  int off = 0;
+  (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowOpportunistic, &off);
+  (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &off);
 }
 
 int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n) {
