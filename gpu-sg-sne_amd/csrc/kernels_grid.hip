@@ -292,35 +292,39 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int 
 }
 
 // ---- 3-D S2G on the matrix cores (grids of side <= 16) ------------------------------
-// A work-group spreads a chunk of points onto a 16^3 tile grid.  It first sorts
-// its points by the base z-cell of their stencils (a bitonic sort of (z-cell,
-// position) keys in LDS: deterministic), then each of its 4 waves owns 4
-// z-slices.  For 4 points at a time, slice z of channel c gets the rank-4
-// product V_c[gx][gy][z] += sum_k wx_k(gx) (q_c,k wy_k(gy) wz_k(z)) (one
-// v_mfma_f32_16x16x4f32, layouts as in the 2-D form); a slice that none of
-// the 4 points touches is skipped, and in z-sorted order a step touches 4
-// consecutive slices, so a point costs 4 matrix operations over the
-// work-group instead of 64 LDS atomics.  Each wave writes its own slices of
-// the work-group's grid (no reduction between waves); grid_reduce_kernel adds
-// the work-groups' grids in a fixed order.
+// A work-group spreads a chunk of points onto its own copy of the grid in LDS.
+// It first sorts its points by the base z-cell of their stencils (a bitonic
+// sort of (z-cell, position) keys: deterministic) and gives each of its 4
+// waves a contiguous quarter of the sorted points, so the work is even however
+// the points crowd a few cells.  A wave walks its points in runs of equal base
+// z-cell f, 4 at a time; such a step touches slices f .. f + 3 only, and slice
+// f + r of channel c gets the rank-4 product
+//   V_c[gx][gy][f + r] += sum_k wx_k(gx) (q_c,k wy_k(gy) wz_k(r))
+// (one v_mfma_f32_16x16x4f32, layouts as in the 2-D form) into a window of 4
+// slice accumulators.  When the base cell moves up, the slices below it are
+// final for this wave: they are stored into the LDS grid, which no other wave
+// stores to (the waves' runs are disjoint), and the window slides.  The 4
+// windows left at the end are added in wave order after a barrier, so the grid
+// does not depend on timing; grid_reduce_kernel adds the work-groups' grids in
+// a fixed order.
 constexpr int kMfma3Chunk = 2048;  // points per work-group (a power of two for the sort)
 
 __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma3_kernel(GridParams g, int n, const float *__restrict__ Y,
                                                                float *__restrict__ part, int chunk) {
-  __shared__ float4 pts[kMfma3Chunk];
+  __shared__ float grid[4 * 16 * 16 * 16];  // [channel][z][y][x], side ng
   __shared__ uint32_t key[kMfma3Chunk];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = lane >> 4, node = lane & 15;
+  const int ng = g.ng, G = ng * ng * ng;
   const int i0 = blockIdx.x * chunk;
   const int cnt = min(n, i0 + chunk) - i0;
-  const float4 *Y4 = reinterpret_cast<const float4 *>(Y);
-  // load and key: (base z-cell, position); empty slots sort last
+  const float4 *__restrict__ Y4 = reinterpret_cast<const float4 *>(Y) + i0;
+  for (int e = tid; e < 4 * G; e += kMfmaBlock) grid[e] = 0.0f;
+  // keys: (base z-cell, position); empty slots sort last
   for (int i = tid; i < kMfma3Chunk; i += kMfmaBlock) {
     uint32_t kk = 0xFFFFFFFFu;
     if (i < cnt) {
-      const float4 y = Y4[i0 + i];
-      pts[i] = y;
-      const float t2 = fminf(fmaxf((y.z - g.minv[2]) * g.scale, 0.0f), g.gmax);
+      const float t2 = fminf(fmaxf((Y4[i].z - g.minv[2]) * g.scale, 0.0f), g.gmax);
       kk = ((uint32_t)(int)t2 << 16) | (uint32_t)i;
     }
     key[i] = kk;
@@ -341,50 +345,110 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma3_kernel(GridParams g, int
       }
     }
   __syncthreads();
-  mfv4 acc[4][4];  // [slice 4 wave + r][channel]
+  const int pa = (cnt * wave) >> 2, pb = (cnt * (wave + 1)) >> 2;  // this wave's sorted positions
+  mfv4 acc[4][4];  // [slice zb + r][channel]
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = mfv4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int st = 0; 4 * st < cnt; ++st) {
-    const int src = 4 * st + k;
-    const bool in = src < cnt;
-    const float4 y = pts[in ? (key[src] & 0xFFFFu) : 0];
-    float t0 = (y.x - g.minv[0]) * g.scale, t1 = (y.y - g.minv[1]) * g.scale, t2 = (y.z - g.minv[2]) * g.scale;
-    t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
-    t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
-    t2 = fminf(fmaxf(t2, 0.0f), g.gmax);
-    const int f0 = (int)t0, f1 = (int)t1, f2 = (int)t2;
-    const float d0 = t0 - (float)f0, d1 = t1 - (float)f1, d2 = t2 - (float)f2;
-    const float wx = in ? lagrange_node(node - f0, d0) : 0.0f;
-    const float wy = lagrange_node(node - f1, d1);
-    const float q1 = y.x - g.center[0], q2 = y.y - g.center[1], q3 = y.z - g.center[2];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float wz = in ? lagrange_node(4 * wave + r - f2, d2) : 0.0f;
-      if (__ballot(wz != 0.0f) == 0ull) continue;  // no point of the step touches this slice
-      const float b = wy * wz;
-      acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, b, acc[r][0], 0, 0, 0);
-      acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q1 * b, acc[r][1], 0, 0, 0);
-      acc[r][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q2 * b, acc[r][2], 0, 0, 0);
-      acc[r][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q3 * b, acc[r][3], 0, 0, 0);
-    }
-  }
-  // this wave's slices of the work-group's grid (channel-major, x fastest)
-  const int ng = g.ng, G = ng * ng * ng;
-  float *out = part + (int64_t)blockIdx.x * 4 * G;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int z = 4 * wave + r;
-    if (z >= ng) continue;
+  // slice r of the window -> grid slice z (lane l holds x = 4 (l / 16) + v, y = l % 16)
+  auto put = [&](int r, int z, bool add) {
+    const int gx0 = 4 * k, gy = node;
+    if (z >= ng || gy >= ng) return;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int gx = 4 * k + v, gy = node;
-        if (gx < ng && gy < ng) out[(int64_t)c * G + (z * ng + gy) * ng + gx] = acc[r][c][v];
+      for (int v = 0; v < 4; ++v)
+        if (gx0 + v < ng) {
+          float *cell = &grid[((c * ng + z) * ng + gy) * ng + gx0 + v];
+          *cell = add ? *cell + acc[r][c][v] : acc[r][c][v];
+        }
+  };
+  auto clear = [&](int r) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = mfv4{0.0f, 0.0f, 0.0f, 0.0f};
+  };
+  int zb = pa < pb ? (int)(key[pa] >> 16) : 0;  // window: slices zb .. zb + 3
+  int p0 = pa;
+  while (p0 < pb) {  // wave-uniform
+    const int fz = (int)(key[p0] >> 16);
+    if (fz != zb) {  // slices zb .. fz - 1 are final for this wave
+      const int sh = fz - zb;
+      if (sh == 1) {
+        put(0, zb, false);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { acc[0][c] = acc[1][c]; acc[1][c] = acc[2][c]; acc[2][c] = acc[3][c]; }
+        clear(3);
+      } else if (sh == 2) {
+        put(0, zb, false);
+        put(1, zb + 1, false);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { acc[0][c] = acc[2][c]; acc[1][c] = acc[3][c]; }
+        clear(2);
+        clear(3);
+      } else if (sh == 3) {
+        put(0, zb, false);
+        put(1, zb + 1, false);
+        put(2, zb + 2, false);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[0][c] = acc[3][c];
+        clear(1);
+        clear(2);
+        clear(3);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          put(r, zb + r, false);
+          clear(r);
+        }
       }
+      zb = fz;
+    }
+    // end of the run of base cell fz within this wave's positions
+    int lo = p0 + 1, hi = pb;
+    const uint32_t nextz = (uint32_t)(fz + 1) << 16;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (key[m] < nextz) lo = m + 1;
+      else hi = m;
+    }
+    const int re = lo;
+    for (int ps = p0; ps < re; ps += 4) {
+      const int src = ps + k;
+      const bool in = src < re;
+      const float4 y = Y4[key[in ? src : ps] & 0xFFFFu];
+      float t0 = (y.x - g.minv[0]) * g.scale, t1 = (y.y - g.minv[1]) * g.scale, t2 = (y.z - g.minv[2]) * g.scale;
+      t0 = fminf(fmaxf(t0, 0.0f), g.gmax);  // NaN -> 0: never leaves the grid
+      t1 = fminf(fmaxf(t1, 0.0f), g.gmax);
+      t2 = fminf(fmaxf(t2, 0.0f), g.gmax);
+      const int f0 = (int)t0, f1 = (int)t1;
+      const float d0 = t0 - (float)f0, d1 = t1 - (float)f1, d2 = t2 - (float)(int)t2;
+      const float wx = in ? lagrange_node(node - f0, d0) : 0.0f;
+      const float wy = lagrange_node(node - f1, d1);
+      const float wz[4] = {lg2(1.0f + d2), lg1(d2), lg1(1.0f - d2), lg2(2.0f - d2)};
+      const float q1 = y.x - g.center[0], q2 = y.y - g.center[1], q3 = y.z - g.center[2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float b = wy * wz[r];
+        acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, b, acc[r][0], 0, 0, 0);
+        acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q1 * b, acc[r][1], 0, 0, 0);
+        acc[r][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q2 * b, acc[r][2], 0, 0, 0);
+        acc[r][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, q3 * b, acc[r][3], 0, 0, 0);
+      }
+    }
+    p0 = re;  // the next run starts right after this one
   }
+  // the windows left open, added in wave order
+  for (int w = 0; w < 4; ++w) {
+    __syncthreads();
+    if (wave == w && pa < pb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) put(r, zb + r, true);
+    }
+  }
+  __syncthreads();
+  float *out = part + (int64_t)blockIdx.x * 4 * G;
+  for (int e = tid; e < 4 * G; e += kMfmaBlock) out[e] = grid[e];
 }
 
 // ---- S2G with global float atomics (grids too large for LDS) -------------------
@@ -470,8 +534,97 @@ __global__ __launch_bounds__(kBlock) void g2s_force_kernel(GridParams g, int n,
     for (int c = 0; c < C; ++c) ph[c] = 0.0f;
     for_stencil<D>(ng, f, w, [&](int idx, float ww) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) ph[c] += Phi[c * G + idx] * ww;
+      for (int c = 0; c < C; ++c) ph[c] = __builtin_fmaf(Phi[c * G + idx], ww, ph[c]);
     });
+    float fr[D];
+    double ysq = 0.0, ydot = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const float yt = y[a] - g.center[a];
+      fr[a] = yt * ph[0] - ph[a + 1];
+      ysq += (double)yt * (double)yt;
+      ydot += (double)yt * (double)ph[a + 1];
+    }
+    store_pt<D>(Frep, i, fr);
+    if constexpr (ZDIRECT) neu_add(z, (double)ph[D + 1]);
+    else neu_add(z, (1.0 + 2.0 * ysq) * (double)ph[0] - 2.0 * ydot);
+  }
+  z = wave_neu(z);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { lds[wid * 2] = z.s; lds[wid * 2 + 1] = z.c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Neu t{lds[0], lds[1]};
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) neu_merge(t, Neu{lds[2 * k], lds[2 * k + 1]});
+    part[(size_t)blockIdx.x * 8 + 0] = t.s;
+    part[(size_t)blockIdx.x * 8 + 1] = t.c;
+  }
+}
+
+// ---- G2S + forces + Z with the grid in LDS (2-D side <= 64, 3-D side <= 16) --------
+// The same arithmetic, in the same order, as g2s_force_kernel, with the 4^D
+// stencil reads of a point served from LDS instead of L1/L2: each work-group
+// (one per CU, 16 waves) first copies the grid into LDS, channels 0..3 of a
+// node as one float4 and the fifth (Z-direct in 3-D) in a second array, then
+// strides over the points.
+constexpr int kG2sLdsCells = 4096;
+constexpr int kG2sLdsBlock = 1024;
+
+inline bool g2s_lds_fits(int d, int ng) { return (d == 2 && ng <= 64) || (d == 3 && ng <= 16); }
+
+template <int D, bool ZDIRECT>
+__global__ __launch_bounds__(kG2sLdsBlock) void g2s_lds_kernel(GridParams g, int n, const float *__restrict__ Y,
+                                                               const float *__restrict__ Phi,
+                                                               float *__restrict__ Frep, double *part) {
+  constexpr int C = ZDIRECT ? D + 2 : D + 1;
+  static_assert(C >= 3 && C <= 5, "2-D or 3-D only");
+  __shared__ float4 q4[kG2sLdsCells];
+  __shared__ float q5[C == 5 ? kG2sLdsCells : 1];
+  __shared__ double lds[2 * (kG2sLdsBlock / 64)];
+  const int ng = g.ng;
+  const int G = grid_cells<D>(ng);
+  for (int e = threadIdx.x; e < G; e += blockDim.x) {
+    float4 v;
+    v.x = Phi[e];
+    v.y = Phi[G + e];
+    v.z = Phi[2 * G + e];
+    v.w = C > 3 ? Phi[3 * G + e] : 0.0f;
+    q4[e] = v;
+    if constexpr (C == 5) q5[e] = Phi[4 * G + e];
+  }
+  __syncthreads();
+  Neu z{0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float y[D];
+    load_pt<D>(Y, i, y);
+    int f[D];
+    float w[D][4];
+    stencil<D>(g, y, f, w);
+    float ph[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) ph[c] = 0.0f;
+    auto node = [&](int idx, float ww) {
+      const float4 v = q4[idx];
+      ph[0] = __builtin_fmaf(v.x, ww, ph[0]);
+      ph[1] = __builtin_fmaf(v.y, ww, ph[1]);
+      ph[2] = __builtin_fmaf(v.z, ww, ph[2]);
+      if constexpr (C > 3) ph[3] = __builtin_fmaf(v.w, ww, ph[3]);
+      if constexpr (C == 5) ph[4] = __builtin_fmaf(q5[idx], ww, ph[4]);
+    };
+    if constexpr (D == 3) {
+      // for_stencil's order and weights, one z-plane of 16 nodes at a time
+      // (the whole stencil unrolled would spill)
+#pragma unroll 1
+      for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+        for (int i1 = 0; i1 < 4; ++i1) {
+          const float w21 = w[2][i2] * w[1][i1];
+#pragma unroll
+          for (int i0 = 0; i0 < 4; ++i0) node(((f[2] + i2) * ng + f[1] + i1) * ng + f[0] + i0, w21 * w[0][i0]);
+        }
+    } else {
+      for_stencil<D>(ng, f, w, node);
+    }
     float fr[D];
     double ysq = 0.0, ydot = 0.0;
 #pragma unroll
@@ -804,6 +957,19 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
 
 int launch_g2s_force(hipStream_t s, const GridParams &g, int n, const float *Y, const float *Phi,
                      float *Frep, int zdirect, RedWork rw, DevScalars *sc) {
+  if (g2s_lds_fits(g.d, g.ng)) {
+    // one 16-wave work-group per CU (the grid copy takes half of its LDS)
+    const int lb = std::max(1, std::min({rw.max_blocks, 256, div_up(n > 0 ? n : 1, kG2sLdsBlock)}));
+    if (g.d == 2) {
+      if (zdirect) g2s_lds_kernel<2, true><<<lb, kG2sLdsBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part);
+      else g2s_lds_kernel<2, false><<<lb, kG2sLdsBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part);
+    } else {
+      if (zdirect) g2s_lds_kernel<3, true><<<lb, kG2sLdsBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part);
+      else g2s_lds_kernel<3, false><<<lb, kG2sLdsBlock, 0, s>>>(g, n, Y, Phi, Frep, rw.part);
+    }
+    SGT_LAUNCH_CHECK();
+    return finalize(s, FIN_ZNEU, lb, g.d, n, 0, rw, sc);
+  }
   const int blocks = grid_blocks(n, std::min(rw.max_blocks, 2048));
 #define G2S_CASE(DD)                                                                               \
   case DD:                                                                                         \
