@@ -317,8 +317,15 @@ def front_end(sg, X, k=90, u=30.0):
     return P, {"knn": round(t_knn, 2), "knn_rows_recomputed_exactly": fb, "perplexity": round(t_perp, 2)}
 
 
-def kernel_name(d, tune):
-    return (f"attr_csr_loc_kernel<{d},{tune['group']}>" if tune["window"] < 0
+def kernel_name(d, tune, fmt="CSR_SEG"):
+    if fmt == "COO":
+        return f"attr_coo_kernel<{d}>"
+    if fmt == "HYBRID":
+        return f"attr_ell_kernel<{d}> + attr_coo_kernel<{d}>"
+    if fmt == "CSR":
+        return f"attr_csr_row_kernel<{d}>"
+    return (f"attr_csr_cc_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["window"] <= -2
+            else f"attr_csr_loc_kernel<{d},{tune['group']}>" if tune["window"] < 0
             else f"attr_csr_win_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["window"]
             else f"attr_csr_blk_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["rows_per_block"]
             else f"attr_csr_seg2_kernel<{d},{tune['group']},{tune['unroll']}>" if tune["unroll"]
@@ -379,7 +386,7 @@ def run_config(sg, P, d, steps, warmup, fmt="CSR_SEG", reorder=2, cpu_seconds=15
 
     # (b) attractive SpMV: 8 nnz + 4 (N + 1) + 8 d N bytes per launch over its
     # average event-timed duration
-    kern = kernel_name(d, tune)
+    kern = kernel_name(d, tune, fmt)
     bytes_b = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * d * n
     t_b = phases["attractive"] * 1e-3
     ach_b = bytes_b / t_b / 1e9 if t_b > 0 else 0.0

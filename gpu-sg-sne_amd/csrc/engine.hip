@@ -91,12 +91,17 @@ struct sgtsne_ctx {
   int seg_unroll = 0;  // 0: plain segmented kernel; else the unrolled form
   int seg_rpb = 0;     // > 0: blocked form with this many rows per work-group
   int seg_win = 0;     // > 0: windowed blocked form, this many points of Y staged in LDS;
-                       // -1: column-localised blocks (loc_* below)
+                       // -1: column-localised blocks (loc_* below); <= -64: chunk-cached
+                       // blocked form with chunks of -seg_win points (cc_* below)
   // column-localised blocks: row ranges, their distinct columns, 16-bit local indices
   int loc_nblk = 0, loc_cap = 0;
   int32_t *loc_brow = nullptr, *loc_bu = nullptr, *loc_ucol = nullptr;
   uint16_t *loc_lidx = nullptr;
   double loc_reuse = 0.0;  // nonzeros per staged column
+  // chunk-cached blocked form: per range of cc_rpb rows, the cc_nslots chunks
+  // of 2^cc_chs points of Y its neighbours fall in most, staged in LDS
+  int cc_rpb = 0, cc_chs = 0, cc_nslots = 0;
+  int32_t *cc_slots = nullptr;
   std::vector<double> tune_ms;  // autotuner timings (ms per candidate)
   std::vector<std::array<int, 4>> tune_cands;
   bool tune_pinned = false;  // the caller fixed the attractive kernel: no re-tuning
@@ -158,11 +163,64 @@ void free_loc(sgtsne_ctx *c) {
   free_dev(c->loc_ucol);
   free_dev(c->loc_lidx);
   c->loc_nblk = 0;
-  if (c->seg_win < 0) c->seg_win = 0;
+  if (c->seg_win == -1) c->seg_win = 0;
+}
+
+void free_cc(sgtsne_ctx *c) {
+  free_dev(c->cc_slots);
+  c->cc_rpb = c->cc_chs = c->cc_nslots = 0;
+  if (c->seg_win <= -2) c->seg_win = 0;
+}
+
+// LDS of the chunk-cached form: 128 KB of cached points (plus a byte per chunk)
+constexpr size_t kCcPointBytes = 131072;
+
+int cc_slots_for(const sgtsne_ctx *c, int chs) {
+  return (int)std::min<size_t>(127, (kCcPointBytes / ((size_t)c->ds * sizeof(float))) >> chs);
+}
+
+bool cc_fits(const sgtsne_ctx *c, int chs) {
+  const int ns = cc_slots_for(c, chs);
+  return ns >= 1 && ((size_t)ns << chs) * c->ds * sizeof(float) + (size_t)div_up(c->n, 1 << chs) <= 147456;
+}
+
+int log2_exact(int v) {
+  int k = 0;
+  while ((1 << k) < v) ++k;
+  return (1 << k) == v ? k : -1;
+}
+
+// The chunk table of the chunk-cached form for ranges of rpb rows and chunks of
+// 2^chs points, for the current order of P (kept while the parameters match).
+int build_chunk_cache(sgtsne_ctx *c, int rpb, int chs) {
+  if (c->cc_slots && c->cc_rpb == rpb && c->cc_chs == chs) return SGTSNE_OK;
+  free_dev(c->cc_slots);
+  c->cc_rpb = c->cc_chs = c->cc_nslots = 0;
+  SGT_CHECK(rpb > 0 && chs >= 6 && chs <= 16 && cc_fits(c, chs), SGTSNE_ERR_ARG,
+            "chunk cache with chunks of %d points does not fit LDS", 1 << chs);
+  const int nranges = div_up(c->n, rpb), nchunks = div_up(c->n, 1 << chs), nslots = cc_slots_for(c, chs);
+  int32_t *counts = nullptr;
+  SGT_TRY(alloc_dev((void **)&counts, sizeof(int32_t) * (size_t)nranges * nchunks));
+  int st = alloc_dev((void **)&c->cc_slots, sizeof(int32_t) * (size_t)nranges * nslots);
+  if (st == SGTSNE_OK) st = launch_chunk_cache(c->stream, c->n, rpb, chs, nslots, c->rowptr, c->col, counts, c->cc_slots);
+  if (st == SGTSNE_OK && hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_error("building the chunk table failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  free_dev(counts);
+  if (st != SGTSNE_OK) {
+    free_dev(c->cc_slots);
+    return st;
+  }
+  c->cc_rpb = rpb;
+  c->cc_chs = chs;
+  c->cc_nslots = nslots;
+  return SGTSNE_OK;
 }
 
 void free_P(sgtsne_ctx *c) {
   free_loc(c);
+  free_cc(c);
   free_dev(c->rowptr);
   free_dev(c->col);
   free_dev(c->val);
@@ -263,6 +321,9 @@ int ensure_grid(sgtsne_ctx *c, const GridParams &g) {
 int attractive(sgtsne_ctx *c) {
   switch (c->p.format) {
     case SGTSNE_FORMAT_CSR_SEG:
+      if (c->seg_win <= -2)
+        return launch_attr_csr_cc(c->stream, c->d, c->seg_group, c->seg_unroll, c->cc_rpb, c->cc_chs, c->cc_nslots,
+                                  c->cc_slots, c->n, c->rowptr, c->col, c->val, c->Y, c->Fattr);
       if (c->seg_win < 0)
         return launch_attr_csr_loc(c->stream, c->d, c->seg_group, c->loc_nblk, c->loc_cap, c->loc_brow, c->loc_bu,
                                    c->loc_ucol, c->rowptr, c->loc_lidx, c->val, c->Y, c->Fattr);
@@ -356,12 +417,13 @@ int build_localised(sgtsne_ctx *c);
 int time_attractive(sgtsne_ctx *c, int32_t *rp, int32_t *col, float *val, float *Y, float *ms) {
   int32_t *k_rp = c->rowptr, *k_col = c->col;
   float *k_val = c->val, *k_Y = c->Y;
-  // the column-localised blocks belong to one order: compare orders with the
-  // plain kernel of the same lanes per row
+  // the column-localised blocks and the chunk table belong to one order:
+  // compare orders with the grid-stride kernel of the same lanes per row
+  // (and, for the chunk-cached form, the same steps)
   const int k_win = c->seg_win, k_unroll = c->seg_unroll, k_rpb = c->seg_rpb;
   if (c->seg_win < 0) {
+    if (c->seg_win == -1) c->seg_unroll = 0;
     c->seg_win = 0;
-    c->seg_unroll = 0;
     c->seg_rpb = 0;
   }
   c->rowptr = rp;
@@ -470,10 +532,16 @@ int spatial_reorder(sgtsne_ctx *c) {
   c->permuted = true;
   c->reorders++;
   // the column-localised blocks follow the order
-  const bool had_loc = c->loc_nblk > 0, using_loc = c->seg_win < 0;
+  const bool had_loc = c->loc_nblk > 0, using_loc = c->seg_win == -1;
   if (had_loc) {
     SGT_TRY(build_localised(c));
     if (using_loc && c->loc_nblk > 0) c->seg_win = -1;
+  }
+  // so does the chunk table
+  if (c->cc_slots) {
+    const int rpb = c->cc_rpb, chs = c->cc_chs;
+    free_dev(c->cc_slots);
+    SGT_TRY(build_chunk_cache(c, rpb, chs));
   }
   // the first re-order changes the gather pattern most: choose the kernel again
   if (c->reorders == 1 && c->p.format == SGTSNE_FORMAT_CSR_SEG && !c->tune_pinned) SGT_TRY(tune_attractive(c));
@@ -785,7 +853,7 @@ int tune_attractive(sgtsne_ctx *c) {
                         {32, 2, 1024, 8192}, {16, 0, 0, -1},    {32, 0, 0, -1},    {64, 0, 0, -1},
                         {16, 8, 0, 0},     {32, 8, 0, 0},      {64, 8, 0, 0},      {16, 16, 0, 0},
                         {32, 16, 0, 0},    {32, 16, 64, 0},    {32, 16, 256, 0},   {16, 16, 128, 0},
-                        {32, 8, 256, 0}};
+                        {32, 8, 256, 0},   {32, 16, 2048, -512}, {64, 8, 2048, -512}};
   if (c->n < 4096) return SGTSNE_OK;  // the heuristic choice is fine for small jobs
   // the column-localised candidates run only when their blocks exist (built
   // when the caller pins that form): on kNN graphs of high-dimensional data
@@ -800,11 +868,19 @@ int tune_attractive(sgtsne_ctx *c) {
   int st = SGTSNE_OK;
   for (const Cand &k : cands) {
     if (k.win > 0 && (size_t)k.win * c->ds * sizeof(float) > 65536) continue;  // window must fit LDS
-    if (k.win < 0 && c->loc_nblk == 0) continue;                                // no localised blocks
+    if (k.win == -1 && c->loc_nblk == 0) continue;                             // no localised blocks
+    const int chs = k.win <= -2 ? log2_exact(-k.win) : 0;
+    if (k.win <= -2) {  // chunk-cached: its table for these ranges and chunks
+      if (!cc_fits(c, chs) || k.rpb >= c->n) continue;
+      st = build_chunk_cache(c, k.rpb, chs);
+      if (st != SGTSNE_OK) break;
+    }
     float ms = 0.f, tot = 0.f;
     for (int rep = 0; rep < 3 && st == SGTSNE_OK; rep++) {
       (void)hipEventRecord(e0, c->stream);
-      st = k.win < 0   ? launch_attr_csr_loc(c->stream, c->d, k.g, c->loc_nblk, c->loc_cap, c->loc_brow, c->loc_bu,
+      st = k.win <= -2 ? launch_attr_csr_cc(c->stream, c->d, k.g, k.u, k.rpb, chs, c->cc_nslots, c->cc_slots, c->n,
+                                            c->rowptr, c->col, c->val, c->Y, c->Fattr)
+           : k.win < 0   ? launch_attr_csr_loc(c->stream, c->d, k.g, c->loc_nblk, c->loc_cap, c->loc_brow, c->loc_bu,
                                              c->loc_ucol, c->rowptr, c->loc_lidx, c->val, c->Y, c->Fattr)
            : k.win > 0 ? launch_attr_csr_win(c->stream, c->d, k.g, k.u, k.rpb, k.win, c->n, c->rowptr, c->col,
                                              c->val, c->Y, c->Fattr)
@@ -826,6 +902,9 @@ int tune_attractive(sgtsne_ctx *c) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   SGT_TRY(st);
+  // the winner's chunk table (the last one built may belong to another candidate)
+  if (bw <= -2) SGT_TRY(build_chunk_cache(c, brpb, log2_exact(-bw)));
+  else free_dev(c->cc_slots), c->cc_rpb = c->cc_chs = c->cc_nslots = 0;
   c->seg_group = bg;
   c->seg_unroll = bu;
   c->seg_rpb = brpb;
@@ -1238,10 +1317,14 @@ int sgtsne_ctx_set_attractive(sgtsne_ctx *c, int group, int unroll, int rows_per
   SGT_CHECK(c != nullptr, SGTSNE_ERR_ARG, "ctx is NULL");
   SGT_CHECK(c->have_P && c->p.format == SGTSNE_FORMAT_CSR_SEG, SGTSNE_ERR_STATE,
             "pinning the attractive kernel needs P uploaded in the CSR_SEG format");
-  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 && window >= -1 && (window <= 0 || rows_per_block > 0) &&
-                (window >= 0 || (unroll == 0 && rows_per_block == 0)),
+  const int chs = window <= -2 ? log2_exact(-window) : 0;
+  SGT_CHECK(unroll >= 0 && rows_per_block >= 0 &&
+                (window >= 0    ? (window == 0 || rows_per_block > 0)
+                 : window == -1 ? (unroll == 0 && rows_per_block == 0)
+                                : (rows_per_block > 0 && chs >= 6 && chs <= 16)),
             SGTSNE_ERR_ARG, "bad attractive kernel shape");
-  if (window < 0 && c->loc_nblk == 0) {
+  if (window <= -2) SGT_TRY(build_chunk_cache(c, rows_per_block, chs));
+  if (window == -1 && c->loc_nblk == 0) {
     SGT_TRY(build_localised(c));
     SGT_CHECK(c->loc_nblk > 0, SGTSNE_ERR_ARG, "a row is wider than the column-localised blocks");
   }

@@ -47,6 +47,14 @@ int launch_attr_csr_seg2(hipStream_t s, int d, int group, int unroll, int n, con
 int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_per_block, int n,
                         const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
 // windowed blocked form: win points of Y around each row range staged in LDS
+// chunk-cached blocked form: per row range, nslots chunks of 2^chs points of Y
+// staged in LDS (slots[range * nslots + k] = chunk id or -1)
+int launch_attr_csr_cc(hipStream_t s, int d, int group, int unroll, int rows_per_block, int chs, int nslots,
+                       const int32_t *slots, int n, const int32_t *rowptr, const int32_t *col, const float *val,
+                       const float *Y, float *F);
+// builds slots: counts is scratch of div_up(n, rows_per_block) * div_up(n, 2^chs) ints
+int launch_chunk_cache(hipStream_t s, int n, int rows_per_block, int chs, int nslots, const int32_t *rowptr,
+                       const int32_t *col, int32_t *counts, int32_t *slots);
 int launch_attr_csr_win(hipStream_t s, int d, int group, int unroll, int rows_per_block, int win, int n,
                         const int32_t *rowptr, const int32_t *col, const float *val, const float *Y, float *F);
 // column-localised form: per row block, Y at its distinct columns staged in LDS

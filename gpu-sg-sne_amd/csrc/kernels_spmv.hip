@@ -78,10 +78,10 @@ __global__ __launch_bounds__(kBlock) void attr_csr_seg_kernel(int n, const int32
 // together.  U < 8: U strides per step, then the remainder one by one.
 // (col, val) are streamed once per iteration with nontemporal loads and do
 // not evict Y from the caches.
-template <int D, int G, int U>
-__device__ __forceinline__ void row_accum(int t, const int e, const int32_t *__restrict__ col,
-                                          const float *__restrict__ val, const float *__restrict__ Y,
-                                          const float (&yi)[D], float (&acc)[D]) {
+template <int D, int G, int U, class Gather>
+__device__ __forceinline__ void row_accum_g(int t, const int e, const int32_t *__restrict__ col,
+                                            const float *__restrict__ val, Gather gather,
+                                            const float (&yi)[D], float (&acc)[D]) {
   if constexpr (U >= 8) {
     for (; t < e; t += U * G) {
       int c[U];
@@ -95,7 +95,7 @@ __device__ __forceinline__ void row_accum(int t, const int e, const int32_t *__r
       float yj[U][D];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (c[u] >= 0) load_pt<D>(Y, c[u], yj[u]);
+        if (c[u] >= 0) gather(c[u], yj[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (c[u] >= 0) accum_pair<D>(yi, yj[u], v[u], acc);
@@ -110,16 +110,24 @@ __device__ __forceinline__ void row_accum(int t, const int e, const int32_t *__r
       for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(val + t + u * G);
       float yj[U][D];
 #pragma unroll
-      for (int u = 0; u < U; ++u) load_pt<D>(Y, c[u], yj[u]);
+      for (int u = 0; u < U; ++u) gather(c[u], yj[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u) accum_pair<D>(yi, yj[u], v[u], acc);
     }
     for (; t < e; t += G) {
       float yj[D];
-      load_pt<D>(Y, __builtin_nontemporal_load(col + t), yj);
+      gather(__builtin_nontemporal_load(col + t), yj);
       accum_pair<D>(yi, yj, __builtin_nontemporal_load(val + t), acc);
     }
   }
+}
+
+// The plain form: every neighbour from Y (global memory).
+template <int D, int G, int U>
+__device__ __forceinline__ void row_accum(int t, const int e, const int32_t *__restrict__ col,
+                                          const float *__restrict__ val, const float *__restrict__ Y,
+                                          const float (&yi)[D], float (&acc)[D]) {
+  row_accum_g<D, G, U>(t, e, col, val, [&](int c, float (&yj)[D]) { load_pt<D>(Y, c, yj); }, yi, acc);
 }
 
 // Format 4, tuned form: as above, but each lane issues the (col, val) loads of
@@ -184,6 +192,141 @@ __global__ __launch_bounds__(kBlock) void attr_csr_blk_kernel(int n, int rows_pe
 #pragma unroll
       for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
     if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
+// Format 4, chunk-cached form: as the blocked form, with 1024-thread work-
+// groups (one per CU) that first copy into LDS the chunks of Y (runs of 2^chs
+// consecutive points) their rows' neighbours fall in most often, chosen per
+// row range at upload (chunk_select_kernel), plus a chunk -> slot table.  A
+// neighbour in a cached chunk is read from LDS, any other from Y.  Each lane
+// sums the same entries in the same order as the other forms with G lanes per
+// row (only the source of y_j changes), so the result is theirs bit for bit.
+constexpr int kCcBlock = 1024;
+constexpr size_t kCcLdsMax = 147456;  // cached points + slot table
+
+template <int D, int G, int U>
+__global__ __launch_bounds__(kCcBlock) void attr_csr_cc_kernel(int n, int rows_per_block, int nranges, int chs,
+                                                               int nslots, int nchunks,
+                                                               const int32_t *__restrict__ slots,
+                                                               const int32_t *__restrict__ rowptr,
+                                                               const int32_t *__restrict__ col,
+                                                               const float *__restrict__ val,
+                                                               const float *__restrict__ Y,
+                                                               float *__restrict__ F) {
+  using V = typename VecT<D>::T;
+  extern __shared__ float4 craw[];
+  V *cy = reinterpret_cast<V *>(craw);
+  const int cpts = nslots << chs;
+  int8_t *lut = reinterpret_cast<int8_t *>(cy + cpts);
+  const V *__restrict__ Yv = reinterpret_cast<const V *>(Y);
+  const int per_xcd = (gridDim.x + kXcds - 1) / kXcds;
+  const int range = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (range >= nranges) return;  // the whole work-group leaves together
+  const int32_t *__restrict__ sl = slots + (int64_t)range * nslots;
+  for (int i = threadIdx.x; i < nchunks; i += kCcBlock) lut[i] = -1;
+  __syncthreads();
+  if ((int)threadIdx.x < nslots) {
+    const int ch = sl[threadIdx.x];
+    if (ch >= 0) lut[ch] = (int8_t)threadIdx.x;
+  }
+  // the cached chunks, eight loads in flight per thread
+  const int cm = (1 << chs) - 1;
+  for (int e0 = 0; e0 < cpts; e0 += kCcBlock * 8) {
+    V tmp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = e0 + k * kCcBlock + (int)threadIdx.x;
+      if (e < cpts) {
+        const int ch = sl[e >> chs];
+        const int j = (ch << chs) + (e & cm);
+        tmp[k] = (ch >= 0 && j < n) ? Yv[j] : V{};
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = e0 + k * kCcBlock + (int)threadIdx.x;
+      if (e < cpts) cy[e] = tmp[k];
+    }
+  }
+  __syncthreads();
+  auto gather = [&](int c, float (&yj)[D]) {
+    const int sl_of = lut[c >> chs];
+    const V v = sl_of >= 0 ? cy[(sl_of << chs) | (c & cm)] : Yv[c];
+    if constexpr (D == 1) {
+      yj[0] = v;
+    } else if constexpr (D == 2) {
+      yj[0] = v.x; yj[1] = v.y;
+    } else {
+      yj[0] = v.x; yj[1] = v.y; yj[2] = v.z;
+    }
+  };
+  const int r0 = range * rows_per_block;
+  const int r1 = min(n, r0 + rows_per_block);
+  const int grp = threadIdx.x / G;
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kCcBlock / G;
+  for (int row = r0 + grp; row < r1; row += NG) {
+    float yi[D];
+    load_pt<D>(Y, row, yi);
+    float acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0f;
+    row_accum_g<D, G, U>(rowptr[row] + lane, rowptr[row + 1], col, val, gather, yi, acc);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, G);
+    if (lane == 0) store_pt<D>(F, row, acc);
+  }
+}
+
+// Chunk histogram of the nonzeros per row range (one wave per row).
+__global__ __launch_bounds__(kBlock) void chunk_hist_kernel(int n, int rows_per_block, int chs, int nchunks,
+                                                            const int32_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ col,
+                                                            int32_t *__restrict__ counts) {
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < n; row += nw) {
+    int32_t *h = counts + (int64_t)(row / rows_per_block) * nchunks;
+    for (int t = rowptr[row] + lane; t < rowptr[row + 1]; t += 64) atomicAdd(&h[col[t] >> chs], 1);
+  }
+}
+
+// Per row range, the nslots chunks with the most nonzeros (ties: lower chunk
+// first; chunks without nonzeros are not taken, their slots get -1).  One
+// work-group per range; the range's counts are consumed in place.
+__global__ __launch_bounds__(kBlock) void chunk_select_kernel(int nranges, int nchunks, int nslots,
+                                                              int32_t *__restrict__ counts,
+                                                              int32_t *__restrict__ slots) {
+  __shared__ int64_t best[kBlock / 64];
+  const int range = blockIdx.x;
+  if (range >= nranges) return;
+  int32_t *h = counts + (int64_t)range * nchunks;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int s = 0; s < nslots; ++s) {
+    // key: count in the high word, inverted chunk index in the low word
+    int64_t k = -1;
+    for (int i = threadIdx.x; i < nchunks; i += blockDim.x) {
+      const int64_t v = h[i] > 0 ? ((int64_t)h[i] << 32) | (int64_t)(0x7FFFFFFF - i) : -1;
+      k = v > k ? v : k;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int64_t other = __shfl_xor(k, o, 64);
+      k = other > k ? other : k;
+    }
+    if (lane == 0) best[wid] = k;
+    __syncthreads();
+    int64_t b = best[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = best[w] > b ? best[w] : b;
+    const int ch = b < 0 ? -1 : 0x7FFFFFFF - (int)(b & 0xFFFFFFFF);
+    if (threadIdx.x == 0) {
+      slots[(int64_t)range * nslots + s] = ch;
+      if (ch >= 0) h[ch] = 0;
+    }
+    __syncthreads();
   }
 }
 
@@ -557,6 +700,59 @@ int launch_attr_csr_blk(hipStream_t s, int d, int group, int unroll, int rows_pe
   per_d(I2{});
   per_d(I3{});
   SGT_CHECK(ok, SGTSNE_ERR_ARG, "no blocked attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_attr_csr_cc(hipStream_t s, int d, int group, int unroll, int rows_per_block, int chs, int nslots,
+                       const int32_t *slots, int n, const int32_t *rowptr, const int32_t *col, const float *val,
+                       const float *Y, float *F) {
+  if (n <= 0) return SGTSNE_OK;
+  const int ds = d == 3 ? 4 : d;
+  const int nchunks = div_up(n, 1 << chs);
+  const size_t lds = ((size_t)nslots << chs) * ds * sizeof(float) + (size_t)nchunks;
+  SGT_CHECK(slots != nullptr && rows_per_block > 0 && chs >= 6 && chs <= 16 && nslots >= 1 && nslots <= 127 &&
+                lds <= kCcLdsMax,
+            SGTSNE_ERR_ARG, "chunk cache of %d x 2^%d points does not fit LDS", nslots, chs);
+  const int nranges = div_up(n, rows_per_block);
+  const int blocks = div_up(nranges, kXcds) * kXcds;
+  bool ok = false;
+  auto go = [&](auto dd, auto gg, auto uu) {
+    constexpr int DD = decltype(dd)::value, GG = decltype(gg)::value, UU = decltype(uu)::value;
+    if (d == DD && group == GG && unroll == UU) {
+      attr_csr_cc_kernel<DD, GG, UU><<<blocks, kCcBlock, lds, s>>>(n, rows_per_block, nranges, chs, nslots, nchunks,
+                                                                  slots, rowptr, col, val, Y, F);
+      ok = true;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I8 = std::integral_constant<int, 8>;
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  auto per_d = [&](auto dd) {
+    go(dd, I16{}, I16{}); go(dd, I32{}, I8{}); go(dd, I32{}, I16{}); go(dd, I64{}, I8{});
+  };
+  per_d(I1{});
+  per_d(I2{});
+  per_d(I3{});
+  SGT_CHECK(ok, SGTSNE_ERR_ARG, "no chunk-cached attractive kernel for d=%d group=%d unroll=%d", d, group, unroll);
+  SGT_LAUNCH_CHECK();
+  return SGTSNE_OK;
+}
+
+int launch_chunk_cache(hipStream_t s, int n, int rows_per_block, int chs, int nslots, const int32_t *rowptr,
+                       const int32_t *col, int32_t *counts, int32_t *slots) {
+  if (n <= 0) return SGTSNE_OK;
+  const int nranges = div_up(n, rows_per_block);
+  const int nchunks = div_up(n, 1 << chs);
+  SGT_HIP(hipMemsetAsync(counts, 0, sizeof(int32_t) * (size_t)nranges * nchunks, s));
+  chunk_hist_kernel<<<(int)std::min<int64_t>(div_up((int64_t)n * 64, kBlock), 65536), kBlock, 0, s>>>(
+      n, rows_per_block, chs, nchunks, rowptr, col, counts);
+  SGT_LAUNCH_CHECK();
+  chunk_select_kernel<<<nranges, kBlock, 0, s>>>(nranges, nchunks, nslots, counts, slots);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }

@@ -420,9 +420,9 @@ class Context:
 
     def tuning(self):
         g, u, rpb, win, cnt = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        ms = np.zeros(32)
+        ms = np.zeros(64)
         _check(lib().sgtsne_ctx_tuning(self._h, C.byref(g), C.byref(u), C.byref(rpb), C.byref(win), ms.ctypes.data,
-                                       32, C.byref(cnt)))
+                                       64, C.byref(cnt)))
         return {"group": g.value, "unroll": u.value, "rows_per_block": rpb.value, "window": win.value,
                 "candidates_ms": [round(float(x), 4) for x in ms[:cnt.value]]}
 

@@ -61,7 +61,7 @@ extern "C" {
 /* Y layouts: SoA [d][N] (CU path) or AoS [N][d] (HYB path) */
 #define SGTSNE_SPREAD_AUTO 0   /* matrix cores where they apply, else atomics   */
 #define SGTSNE_SPREAD_ATOMIC 1 /* LDS-privatised grids, or global atomics       */
-#define SGTSNE_SPREAD_MFMA 2   /* matrix cores (2-D, grid side <= 32; else AUTO) */
+#define SGTSNE_SPREAD_MFMA 2   /* matrix cores (2-D side <= 32, 3-D side <= 16; else AUTO) */
 #define SGTSNE_LAYOUT_SOA 0
 #define SGTSNE_LAYOUT_AOS 1
 
@@ -96,7 +96,8 @@ typedef struct sgtsne_params {
                          faster (CSR formats)                            (2)   */
   int spread;         /* SGTSNE_SPREAD_*: how points are spread onto the grid
                          (AUTO: the matrix-core form for 2-D grids of side
-                         <= 32, else LDS-privatised or atomic)       (AUTO)  */
+                         <= 32 and 3-D grids of side <= 16, else
+                         LDS-privatised or atomic)                  (AUTO)  */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of
@@ -230,8 +231,10 @@ int sgtsne_ctx_timing(sgtsne_ctx *ctx, double *out7);
  * format, and again after the first Morton re-order): lanes per row, unroll
  * (0 = plain form), rows per work-group (0 = grid-stride form), points of Y
  * staged in LDS per work-group (0 = none; -1 = column-localised blocks: Y at
- * each row block's distinct columns staged in LDS), and the measured ms of each
- * candidate (up to max; *count receives how many were timed). */
+ * each row block's distinct columns staged in LDS; -2^k, k = 6..16 =
+ * chunk-cached blocks: per row block, the chunks of 2^k consecutive points its
+ * neighbours fall in most, 128 KB of them, staged in LDS), and the measured ms
+ * of each candidate (up to max; *count receives how many were timed). */
 int sgtsne_ctx_tuning(sgtsne_ctx *ctx, int *group, int *unroll, int *rows_per_block, int *window, double *ms,
                       int max, int *count);
 

@@ -533,7 +533,11 @@ _SHAPES = [(16, 0, 0, 0), (32, 0, 0, 0), (64, 0, 0, 0), (16, 4, 0, 0), (16, 2, 0
            (32, 1, 512, 4096), (16, 0, 0, -1), (32, 0, 0, -1), (64, 0, 0, -1),
            (16, 8, 0, 0), (32, 8, 0, 0), (64, 8, 0, 0), (16, 16, 0, 0), (32, 16, 0, 0),
            (32, 2, 0, 0), (32, 4, 0, 0), (64, 2, 0, 0), (32, 1, 64, 0), (32, 2, 64, 0), (32, 2, 1024, 8192), (64, 1, 256, 0),
-           (16, 1, 512, 8192), (32, 16, 64, 0), (32, 16, 256, 0), (16, 16, 128, 0), (32, 8, 256, 0)]
+           (16, 1, 512, 8192), (32, 16, 64, 0), (32, 16, 256, 0), (16, 16, 128, 0), (32, 8, 256, 0),
+           # chunk-cached: 127 chunks of 64 points (8128 < n: the rest from Y) or,
+           # in 3-D, 8192 cached points; ranges not dividing n
+           (32, 16, 2048, -512), (16, 16, 512, -64), (64, 8, 1024, -128), (32, 8, 4096, -1024),
+           (32, 16, 1000, -64)]
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
