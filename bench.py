@@ -483,7 +483,8 @@ def main_leg(sg, args):
     if "d3" in args.configs and args.d == 2:
         log("[bench] C3 at d=3 from the same P ...")
         d3 = run_config(sg, P, 3, args.steps, args.warmup, fmt=args.format, reorder=args.reorder,
-                        cpu_seconds=min(args.cpu_seconds, 8.0), cpu_max_iters=10, want_cpu=not args.no_cpu)
+                        cpu_seconds=min(args.cpu_seconds, 8.0), cpu_max_iters=10, want_cpu=not args.no_cpu,
+                        profile="attractive_spmv_powerlaw1.3M_d2.json" if full else None)
     return {
         "value": r["ms_per_iter"],
         "ms_per_step": r["ms_per_iter"],
