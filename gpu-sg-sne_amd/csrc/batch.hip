@@ -314,6 +314,38 @@ void release_deferred(sgtsne_batch *b, bool all = false) {
 // fields) so that a running job's iteration state is left alone.
 int write_static(sgtsne_batch *b, int j) { return upload(b, b->jobs_d + j, &b->jobs_h[j], offsetof(BatchJob, iter)); }
 
+// Host-side check of the tables a step reads, before they are uploaded: the
+// running jobs' point and nonzero ranges lie inside the arena and do not
+// overlap, and every work item covers at most kBatchChunk points of its own
+// job's range, the items of a job being contiguous from its item0.
+int validate_tables(const sgtsne_batch *b, const std::vector<BatchItem> &items) {
+  std::vector<std::pair<int64_t, int64_t>> pr, zr;
+  for (int j : b->arena) {
+    const HostJob &h = b->hj[j];
+    SGT_CHECK(j >= 0 && j < b->max_jobs && h.live, SGTSNE_ERR_STATE, "arena lists job %d, not live", j);
+    SGT_CHECK(h.off >= 0 && h.n > 0 && (int64_t)h.off + h.n <= b->max_points, SGTSNE_ERR_STATE,
+              "job %d: points [%d, %d) outside the arena of %lld", j, h.off, h.off + h.n, (long long)b->max_points);
+    SGT_CHECK(h.nnz_off >= 0 && h.nnz >= 0 && h.nnz_off + h.nnz <= b->max_nnz, SGTSNE_ERR_STATE,
+              "job %d: nonzeros outside the arena", j);
+    pr.push_back({h.off, (int64_t)h.off + h.n});
+    if (h.nnz > 0) zr.push_back({h.nnz_off, h.nnz_off + h.nnz});
+    const BatchJob &J = b->jobs_h[j];
+    for (int i = J.item0; i < J.item0 + J.nitems; ++i) {
+      SGT_CHECK(i >= 0 && i < (int)items.size(), SGTSNE_ERR_STATE, "job %d: item %d outside the table", j, i);
+      const BatchItem &it = items[i];
+      SGT_CHECK(it.job == j && it.begin >= h.off && it.begin < it.end && it.end <= h.off + h.n &&
+                    it.end - it.begin <= kBatchChunk,
+                SGTSNE_ERR_STATE, "job %d: item %d [%d, %d) outside its points", j, i, it.begin, it.end);
+    }
+  }
+  for (auto *v : {&pr, &zr}) {
+    std::sort(v->begin(), v->end());
+    for (size_t i = 1; i < v->size(); ++i)
+      SGT_CHECK((*v)[i - 1].second <= (*v)[i].first, SGTSNE_ERR_STATE, "two running jobs share arena memory");
+  }
+  return SGTSNE_OK;
+}
+
 int rebuild_items(sgtsne_batch *b) {
   std::vector<BatchItem> items;
   for (int j : b->arena) {
@@ -327,6 +359,7 @@ int rebuild_items(sgtsne_batch *b) {
     J.n = h.n;
   }
   SGT_CHECK((int)items.size() <= b->items_cap, SGTSNE_ERR_ARG, "work items exceed capacity");
+  SGT_TRY(validate_tables(b, items));
   b->nitems = (int)items.size();
   // stream-ordered after the iterations already queued, which still read the
   // old table; staged, so nothing here waits for them
