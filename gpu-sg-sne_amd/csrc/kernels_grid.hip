@@ -293,8 +293,8 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int 
 
 // ---- 3-D S2G on the matrix cores (grids of side <= 16) ------------------------------
 // A work-group spreads a chunk of points onto its own copy of the grid in LDS.
-// It first sorts its points by the base z-cell of their stencils (a bitonic
-// sort of (z-cell, position) keys: deterministic) and gives each of its 4
+// It first sorts its points by the base z-cell of their stencils (a stable
+// counting sort of (z-cell, position) keys: deterministic) and gives each of its 4
 // waves a contiguous quarter of the sorted points, so the work is even however
 // the points crowd a few cells.  A wave walks its points in runs of equal base
 // z-cell f, 4 at a time; such a step touches slices f .. f + 3 only, and slice
@@ -307,12 +307,14 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma_kernel(GridParams g, int 
 // windows left at the end are added in wave order after a barrier, so the grid
 // does not depend on timing; grid_reduce_kernel adds the work-groups' grids in
 // a fixed order.
-constexpr int kMfma3Chunk = 2048;  // points per work-group (a power of two for the sort)
+constexpr int kMfma3Chunk = 2048;  // points per work-group (a multiple of the work-group size)
 
 __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma3_kernel(GridParams g, int n, const float *__restrict__ Y,
                                                                float *__restrict__ part, int chunk) {
   __shared__ float grid[4 * 16 * 16 * 16];  // [channel][z][y][x], side ng
   __shared__ uint32_t key[kMfma3Chunk];
+  __shared__ int zcnt[kMfma3Chunk / kMfmaBlock][4][16];
+  __shared__ int zbase[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = lane >> 4, node = lane & 15;
   const int ng = g.ng, G = ng * ng * ng;
@@ -320,30 +322,51 @@ __global__ __launch_bounds__(kMfmaBlock) void s2g_mfma3_kernel(GridParams g, int
   const int cnt = min(n, i0 + chunk) - i0;
   const float4 *__restrict__ Y4 = reinterpret_cast<const float4 *>(Y) + i0;
   for (int e = tid; e < 4 * G; e += kMfmaBlock) grid[e] = 0.0f;
-  // keys: (base z-cell, position); empty slots sort last
-  for (int i = tid; i < kMfma3Chunk; i += kMfmaBlock) {
-    uint32_t kk = 0xFFFFFFFFu;
-    if (i < cnt) {
-      const float t2 = fminf(fmaxf((Y4[i].z - g.minv[2]) * g.scale, 0.0f), g.gmax);
-      kk = ((uint32_t)(int)t2 << 16) | (uint32_t)i;
+  // keys (base z-cell, position) of the chunk's points in ascending order, by
+  // a stable counting sort over the at most 16 base z-cells: point i = 256 r
+  // + 64 w + lane is ranked among the equal cells of its wave by a ballot,
+  // and the (cell, round, wave) counts give its place
+  constexpr int kRounds = kMfma3Chunk / kMfmaBlock;
+  int fzr[kRounds], rk[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int i = r * kMfmaBlock + tid;
+    fzr[r] = -1;
+    if (i < cnt) fzr[r] = (int)fminf(fmaxf((Y4[i].z - g.minv[2]) * g.scale, 0.0f), g.gmax);
+    rk[r] = 0;
+    for (int v = 0; v < 16; ++v) {
+      const unsigned long long m = __ballot(fzr[r] == v);
+      if (fzr[r] == v) rk[r] = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) zcnt[r][wave][v] = __popcll(m);
     }
-    key[i] = kk;
   }
-  // bitonic sort of the keys, ascending
-  for (int size = 2; size <= kMfma3Chunk; size <<= 1)
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
-      for (int i = tid; i < kMfma3Chunk; i += kMfmaBlock) {
-        const int p = i ^ stride;
-        if (p > i) {
-          const uint32_t a = key[i], b = key[p];
-          if ((a > b) == ((i & size) == 0)) {
-            key[i] = b;
-            key[p] = a;
-          }
-        }
+  __syncthreads();
+  if (tid < 16) {  // per cell: exclusive prefix over (round, wave), then the cell total
+    int run = 0;
+    for (int r = 0; r < kRounds; ++r)
+      for (int w = 0; w < 4; ++w) {
+        const int c = zcnt[r][w][tid];
+        zcnt[r][w][tid] = run;
+        run += c;
       }
+    zbase[tid] = run;
+  }
+  __syncthreads();
+  if (tid == 0) {  // cell totals -> first position of each cell
+    int run = 0;
+    for (int v = 0; v < 16; ++v) {
+      const int c = zbase[v];
+      zbase[v] = run;
+      run += c;
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const int i = r * kMfmaBlock + tid;
+    const int v = fzr[r];
+    if (v >= 0) key[zbase[v] + zcnt[r][wave][v] + rk[r]] = ((uint32_t)v << 16) | (uint32_t)i;
+  }
   __syncthreads();
   const int pa = (cnt * wave) >> 2, pb = (cnt * (wave + 1)) >> 2;  // this wave's sorted positions
   mfv4 acc[4][4];  // [slice zb + r][channel]
