@@ -22,7 +22,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <atomic>
 #include <random>
+#include <thread>
 #include <vector>
 
 namespace sgt {
@@ -638,34 +640,52 @@ void unpack_y(const std::vector<float> &in, int layout, int n, int d, int ds, co
 // of indices, so the gathers of consecutive rows share cache lines.
 namespace {
 
-// Breadth-first levels from `root` over unvisited vertices (mark != stamp);
-// returns the depth and leaves the last level in [lo, hi) of `q`.
+// Breadth-first levels from `root` over unvisited vertices (mark != stamp),
+// level by level on `nthreads` host threads: a vertex joins the next level
+// on the thread whose atomic exchange of its mark succeeds.  Returns the
+// depth and leaves the last level (as a set: its order depends on timing) in
+// `last`; both are what the sequential sweep gives.
 int bfs_levels(int root, const int32_t *rp, const int32_t *col, std::vector<int32_t> &mark, int stamp,
-               std::vector<int32_t> &q, size_t &lo, size_t &hi) {
-  q.clear();
-  q.push_back(root);
+               std::vector<int32_t> &last, int nthreads) {
+  std::vector<int32_t> front{root}, next;
   mark[root] = stamp;
-  size_t head = 0;
   int depth = 0;
-  lo = 0;
-  hi = 1;
-  while (head < q.size()) {
-    const size_t end = q.size();
-    lo = head;
-    hi = end;
-    for (; head < end; ++head) {
-      const int v = q[head];
-      for (int t = rp[v]; t < rp[v + 1]; t++) {
-        const int u = col[t];
-        if (mark[u] != stamp) {
-          mark[u] = stamp;
-          q.push_back(u);
+  std::vector<std::vector<int32_t>> part(nthreads);
+  for (;;) {
+    const size_t nf = front.size();
+    // small levels: one thread (spawning costs more than they take)
+    const int T = nf < 4096 ? 1 : nthreads;
+    auto work = [&](int t) {
+      std::vector<int32_t> &out = part[t];
+      out.clear();
+      const size_t b0 = nf * t / T, b1 = nf * (t + 1) / T;
+      for (size_t h = b0; h < b1; ++h) {
+        const int v = front[h];
+        for (int e = rp[v]; e < rp[v + 1]; e++) {
+          const int u = col[e];
+          if (__atomic_load_n(&mark[u], __ATOMIC_RELAXED) != stamp &&
+              __atomic_exchange_n(&mark[u], stamp, __ATOMIC_RELAXED) != stamp)
+            out.push_back(u);
         }
       }
+    };
+    if (T == 1) {
+      work(0);
+    } else {
+      std::vector<std::thread> pool;
+      for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+      work(0);
+      for (auto &th : pool) th.join();
     }
-    if (q.size() > end) ++depth;
+    next.clear();
+    for (int t = 0; t < T; ++t) next.insert(next.end(), part[t].begin(), part[t].end());
+    if (next.empty()) {
+      last.swap(front);
+      return depth;
+    }
+    ++depth;
+    front.swap(next);
   }
-  return depth;
 }
 
 }  // namespace
@@ -673,7 +693,8 @@ int bfs_levels(int root, const int32_t *rp, const int32_t *col, std::vector<int3
 void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t> &perm) {
   perm.clear();
   perm.reserve(n);
-  std::vector<int32_t> deg(n), mark(n, 0), q;
+  std::vector<int32_t> deg(n), mark(n, 0), last;
+  const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<uint8_t> placed(n, 0);
   for (int i = 0; i < n; i++) deg[i] = rp[i + 1] - rp[i];
   // candidate starts in increasing degree (ties by index)
@@ -686,13 +707,12 @@ void bfs_order(int n, const int32_t *rp, const int32_t *col, std::vector<int32_t
     if (placed[cand]) continue;
     // pseudo-peripheral vertex of cand's component
     int root = cand;
-    size_t lo = 0, hi = 0;
-    int depth = bfs_levels(root, rp, col, mark, ++stamp, q, lo, hi);
+    int depth = bfs_levels(root, rp, col, mark, ++stamp, last, nthreads);
     for (int sweep = 0; sweep < 4; sweep++) {
-      int best = q[lo];
-      for (size_t t = lo; t < hi; t++)
-        if (deg[q[t]] < deg[best] || (deg[q[t]] == deg[best] && q[t] < best)) best = q[t];
-      const int d2 = bfs_levels(best, rp, col, mark, ++stamp, q, lo, hi);
+      int best = last[0];
+      for (const int v : last)
+        if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+      const int d2 = bfs_levels(best, rp, col, mark, ++stamp, last, nthreads);
       if (d2 <= depth) break;
       depth = d2;
       root = best;

@@ -128,6 +128,72 @@ def test_locality_order_is_rcm(sg):
     assert ours <= 1.25 * ref and ours < rand / 5, (ours, ref, rand)
 
 
+def _rcm_sequential(n, rp, col):
+    # the order the library computes, one vertex at a time: per component, a
+    # pseudo-peripheral root (George-Liu: up to 4 re-sweeps from the
+    # lowest-degree vertex of the last level while the depth grows), a
+    # Cuthill-McKee sweep taking unplaced neighbours by (degree, index), reversed
+    deg = np.diff(rp)
+    placed = np.zeros(n, bool)
+    perm = []
+
+    def levels(root):
+        seen = {root}
+        front, depth = [root], 0
+        while True:
+            nxt = []
+            for v in front:
+                for u in col[rp[v]:rp[v + 1]]:
+                    if u not in seen:
+                        seen.add(u)
+                        nxt.append(int(u))
+            if not nxt:
+                return depth, front
+            depth, front = depth + 1, nxt
+
+    for cand in sorted(range(n), key=lambda i: (deg[i], i)):
+        if placed[cand]:
+            continue
+        root = cand
+        depth, last = levels(root)
+        for _ in range(4):
+            best = min(last, key=lambda v: (deg[v], v))
+            d2, last2 = levels(best)
+            if d2 <= depth:
+                break
+            depth, root, last = d2, best, last2
+        head = len(perm)
+        perm.append(root)
+        placed[root] = True
+        while head < len(perm):
+            v = perm[head]
+            nb = []
+            for u in col[rp[v]:rp[v + 1]]:
+                if not placed[u]:
+                    placed[u] = True
+                    nb.append((int(deg[u]), int(u)))
+            perm.extend(u for _, u in sorted(nb))
+            head += 1
+    return np.array(perm[::-1], np.int64)
+
+
+def test_locality_order_matches_sequential_sweep(sg):
+    # levels of thousands of vertices take the multi-threaded sweep; the
+    # order must be the one-vertex-at-a-time order bit for bit
+    from scipy.sparse import csr_matrix
+
+    rng = np.random.default_rng(3)
+    n, k = 30000, 10
+    A = csr_matrix((np.ones(n * k), rng.integers(0, n, n * k), np.arange(0, n * k + 1, k)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    A.setdiag(0)
+    A.eliminate_zeros()
+    A.sort_indices()
+    rp, col = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    perm = sg.locality_order(sg.CSR(n, rp, col, np.ones(col.size, np.float32)))
+    assert np.array_equal(perm, _rcm_sequential(n, rp, col))
+
+
 def test_locality_order_components_and_isolated(sg):
     # two components and isolated vertices are all placed exactly once
     rp = np.array([0, 1, 2, 2, 3, 4, 4], np.int32)
