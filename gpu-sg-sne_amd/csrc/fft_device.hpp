@@ -39,8 +39,10 @@ constexpr int kMaxStages = 10;
 constexpr int kPerThread = kLdsElems / kFftThreads;  // elements of a work-group's share per thread
 // two ping-pong buffers and the twiddle table of the line length
 constexpr size_t kLdsBytes = (2 * (size_t)kLdsElems + kMaxL) * sizeof(float2);
-// LDS of one pass's work-group (nl lines of length L)
-__host__ __device__ inline size_t pass_lds_bytes(int nl, int L) { return (2 * (size_t)nl * L + L) * sizeof(float2); }
+// LDS of one pass's work-group (nl lines of length L held Lp elements apart)
+__host__ __device__ inline size_t pass_lds_bytes(int nl, int L, int Lp) {
+  return (2 * (size_t)nl * Lp + L) * sizeof(float2);
+}
 
 // Even lengths whose only prime factors are 2, 3 and 5.
 __host__ __device__ inline bool fft_smooth(int L) {
@@ -97,7 +99,7 @@ enum PassKind {
 };
 
 struct PassArgs {
-  int L, nl;             // FFT length, lines per work-group
+  int L, nl, Lp;         // FFT length, lines per work-group, LDS distance of lines
   int nst, rad[kMaxStages];  // Stockham stages and their radices
   int nlines, ninner;    // line = outer * ninner + inner
   int64_t i_os, i_is, i_es, pair_in;
@@ -180,7 +182,7 @@ __device__ __forceinline__ void butterfly(float2 (&v)[R], float sg) {
 // butterfly position j and applies it to every G-th line, so the position's
 // index arithmetic and twiddles are computed once per stage, not per line.
 template <int R>
-__device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L, int nl, int Ns,
+__device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L, int Lp, int nl, int Ns,
                                                const float2 *__restrict__ tw, bool inv) {
   const int LR = L / R;
   const int stride = L / (Ns * R);
@@ -200,8 +202,8 @@ __device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L
     }
     const int base = q * Ns * R + k;
     for (int l = g; l < nl; l += G) {
-      const float2 *src = x + l * L;
-      float2 *dst = y + l * L;
+      const float2 *src = x + l * Lp;
+      float2 *dst = y + l * Lp;
       float2 v[R];
 #pragma unroll
       for (int t = 0; t < R; ++t) v[t] = src[j + t * LR];
@@ -220,10 +222,10 @@ __device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int 
   int Ns = 1;
   for (int s = 0; s < a.nst; ++s) {
     const int R = a.rad[s];
-    if (R == 4) stockham_stage<4>(x, y, a.L, nl, Ns, tw, inv);
-    else if (R == 2) stockham_stage<2>(x, y, a.L, nl, Ns, tw, inv);
-    else if (R == 3) stockham_stage<3>(x, y, a.L, nl, Ns, tw, inv);
-    else stockham_stage<5>(x, y, a.L, nl, Ns, tw, inv);
+    if (R == 4) stockham_stage<4>(x, y, a.L, a.Lp, nl, Ns, tw, inv);
+    else if (R == 2) stockham_stage<2>(x, y, a.L, a.Lp, nl, Ns, tw, inv);
+    else if (R == 3) stockham_stage<3>(x, y, a.L, a.Lp, nl, Ns, tw, inv);
+    else stockham_stage<5>(x, y, a.L, a.Lp, nl, Ns, tw, inv);
     __syncthreads();
     float2 *tmp = x;
     x = y;
@@ -274,25 +276,145 @@ __device__ __forceinline__ void for_elements(int nl, int L, int line0, int ninne
   }
 }
 
+// Elements of one line a thread visits in a strided pass (lines across
+// threads): at most ceil(L / (kFftThreads / nl)) over the supported lengths,
+// checked at compile time below.
+constexpr int kLineElems = 10;
+
+// Line layout of a pass: `padded` passes (lines across threads) keep lines
+// L + 1 elements apart in LDS, so the threads of a wave, which take adjacent
+// lines at the same element, fall on different banks.
+__host__ __device__ inline void set_lines(PassArgs &a, bool padded) {
+  a.Lp = padded ? a.L + 1 : a.L;
+  int nl = kLdsElems / a.Lp;
+  if (nl < 1) nl = 1;
+  if (nl > 64) nl = 64;
+  a.nl = nl;
+}
+
+constexpr bool line_elems_fit() {
+  for (int L = 2; L <= kMaxL; L += 2) {
+    int m = L;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    while (m % 5 == 0) m /= 5;
+    if (m != 1) continue;
+    int nl = kLdsElems / (L + 1);
+    nl = nl < 1 ? 1 : nl > 64 ? 64 : nl;
+    const int G = kFftThreads / nl;
+    if ((L + G - 1) / G > kLineElems) return false;
+  }
+  return true;
+}
+static_assert(line_elems_fit(), "kLineElems below the elements per thread of a strided pass");
+
 // One work-group's share of one pass: lines [blk * nl, blk * nl + nl) of
 // channel pair p.  Every thread of the work-group must call it (barriers);
-// the work-group has kFftThreads threads and nl * L <= kLdsElems.
+// the work-group has kFftThreads threads and nl * Lp <= kLdsElems.
 // The global reads a work-group needs (its lines, the twiddle table and,
 // for a convolution, the kernel spectrum) are all issued before the first
 // barrier, so their latencies overlap instead of adding up stage by stage.
 template <int IN, int OUT, int MID, bool ECONTIG>
 __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds) {
   const int L = a.L;
+  const int Lp = a.Lp;
   const int nl = a.nl;
   // line indices stay below 2^22 (at most 1024^2 lines), where fdiv is exact
   const float invL = 1.0f / (float)L, invni = 1.0f / (float)a.ninner;
   float2 *b0 = lds;
-  float2 *b1 = lds + nl * L;
-  float2 *twl = lds + 2 * nl * L;
+  float2 *b1 = lds + nl * Lp;
+  float2 *twl = lds + 2 * nl * Lp;
   const int line0 = blk * nl;
-  const int total = nl * L;
 
   for (int t = threadIdx.x; t < L; t += kFftThreads) twl[t] = a.tw[t];
+
+  if constexpr (!ECONTIG) {
+    // Lines across threads (the pass axis is strided in memory): thread
+    // (g, l) owns elements e = g, g + G, ... of line l, for the load, the
+    // spectrum multiply and the store alike, so each of those walks one
+    // pointer by a fixed step.
+    static_assert(IN == IN_W || IN == IN_KT, "strided passes read W or kernel half-spectra");
+    static_assert(OUT == OUT_W || OUT == OUT_KT, "strided passes write W or kernel half-spectra");
+    const int G = kFftThreads / nl;
+    const int s = threadIdx.x;
+    const bool act = s < G * nl;
+    const int g = act ? fdiv(s, nl, 1.0f / (float)nl) : 0;
+    const int l = s - g * nl;
+    const int line = line0 + l;
+    const bool lv = act && line < a.nlines;
+    const int outer = lv ? fdiv(line, a.ninner, invni) : 0;
+    const int inner = line - outer * a.ninner;
+    float2 kreg[MID == MID_CONV ? kLineElems : 1];
+    if (act) {
+      float2 *dl = b0 + l * Lp;
+      if (IN == IN_W) {
+        const float2 *src = static_cast<const float2 *>(a.in) + (int64_t)p * a.pair_in + outer * a.i_os +
+                            inner * a.i_is + g * a.i_es;
+        const int64_t step = (int64_t)G * a.i_es;
+        const int ne = lv ? a.n_in : 0;
+        for (int e = g; e < L; e += G, src += step) dl[e] = e < ne ? *src : make_float2(0.f, 0.f);
+      } else {  // IN_KT: (K2, K1) half-spectra, mirrored along the pass axis
+        const float2 *src = static_cast<const float2 *>(a.in) + outer * a.i_os + inner * a.i_is;
+        const int es = (int)a.i_es;
+        const int ne = lv ? a.n_in : 0;
+        for (int e = g; e < L; e += G) dl[e] = e < ne ? src[mirror(e, L) * es] : make_float2(0.f, 0.f);
+      }
+      if (MID == MID_CONV) {
+        // kernel spectrum at (mirror(e), mirror(outer), mirror(inner)):
+        // index mirror(e) * ks + kb
+        int ks = 1, kb = 0;
+        if (a.kdim == 2) {
+          ks = a.H;
+          kb = mirror(inner, L);
+        } else if (a.kdim == 3) {
+          ks = a.H * a.H;
+          kb = mirror(outer, L) * a.H + mirror(inner, L);
+        }
+        const bool cplx = a.kcplx[p] != 0;
+#pragma unroll
+        for (int k = 0; k < kLineElems; ++k) {
+          const int e = g + k * G;
+          kreg[k] = make_float2(0.f, 0.f);
+          if (lv && e < L) {
+            const float2 kv = a.Kh[mirror(e, L) * ks + kb];
+            kreg[k] = make_float2(kv.x, cplx ? kv.y : 0.0f);
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    float2 *r = stockham(b0, b1, a, nl, MID == MID_INV, twl);
+    if (MID == MID_CONV) {
+      // multiply by the kernel spectrum, then inverse transform
+      if (act) {
+        float2 *rl = r + l * Lp;
+#pragma unroll
+        for (int k = 0; k < kLineElems; ++k) {
+          const int e = g + k * G;
+          if (e < L) {
+            const float kv = kreg[k].x, kvi = kreg[k].y;
+            const float2 v = rl[e];
+            rl[e] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
+          }
+        }
+      }
+      __syncthreads();
+      float2 *other = (r == b0) ? b1 : b0;
+      r = stockham(r, other, a, nl, true, twl);
+    }
+
+    if (lv) {
+      const float2 *rl = r + l * Lp;
+      float2 *dst = static_cast<float2 *>(a.out) + (OUT == OUT_W ? (int64_t)p * a.pair_out : 0) + outer * a.o_os +
+                    inner * a.o_is + g * a.o_es;
+      const int64_t step = (int64_t)G * a.o_es;
+      for (int e = g; e < a.n_out; e += G, dst += step) *dst = rl[e];
+    }
+    __syncthreads();  // LDS is reused by the caller's next share
+    return;
+  } else {
+  const int total = nl * L;
   // kernel spectrum of each element this thread multiplies (real part, and
   // the imaginary part for the complex K2^ + i K1^ spectrum)
   float2 kreg[MID == MID_CONV ? kPerThread : 1];
@@ -354,7 +476,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
         v = K[outer * a.i_os + inner * a.i_is + me * a.i_es];
       }
     }
-    b0[l * L + e] = v;
+    b0[l * Lp + e] = v;
   });
   __syncthreads();
 
@@ -365,9 +487,10 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
     for (int t = 0; t < kPerThread; ++t) {
       const int idx = threadIdx.x + t * kFftThreads;
       if (idx < total) {
+        const int l = fdiv(idx, L, invL), e = idx - l * L;
         const float kv = kreg[t].x, kvi = kreg[t].y;
-        const float2 v = r[idx];
-        r[idx] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
+        const float2 v = r[l * Lp + e];
+        r[l * Lp + e] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
       }
     }
     __syncthreads();
@@ -379,7 +502,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
   for_elements<ECONTIG>(nl, L, line0, a.ninner, invni, [&](int l, int e, int line, int outer, int inner) {
     if (line >= a.nlines || e >= a.n_out) return;
     const int64_t off = outer * a.o_os + inner * a.o_is + e * a.o_es;
-    const float2 v = r[l * L + e];
+    const float2 v = r[l * Lp + e];
     if (OUT == OUT_W) {
       float2 *W = static_cast<float2 *>(a.out);
       W[(int64_t)p * a.pair_out + off] = v;
@@ -393,6 +516,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
     }
   });
   __syncthreads();  // LDS is reused by the caller's next share
+  }
 }
 
 // Work-groups of one pass (per channel pair).
@@ -452,10 +576,7 @@ __host__ __device__ inline Plan make_plan(int d, bool zdirect) {
 __host__ __device__ inline void base_args(PassArgs &a, const GridParams &g, const Plan &pl, const ConvPtrs &b) {
   a = PassArgs{};
   a.L = g.L;
-  int nl = kLdsElems / g.L;
-  if (nl < 1) nl = 1;
-  if (nl > 64) nl = 64;
-  a.nl = nl;
+  set_lines(a, false);
   // radix-4 stages first, then 2, 3, 5
   int m = g.L, ns = 0;
   while (m % 4 == 0 && ns < kMaxStages) { a.rad[ns++] = 4; m /= 4; }
@@ -484,8 +605,9 @@ __host__ __device__ inline void base_args(PassArgs &a, const GridParams &g, cons
 __host__ __device__ inline int conv_npasses(int d) { return d == 1 ? 2 : d == 2 ? 5 : 8; }
 
 // Arguments of pass i (in the order of pass_kinds) of one convolution,
-// written in place; returns the channel pairs the pass runs over.
-__host__ __device__ inline int conv_pass(const GridParams &g, bool zdirect, const ConvPtrs &b, int i, PassArgs &a) {
+// written in place, lines unpadded; returns the channel pairs the pass runs over.
+__host__ __device__ inline int conv_pass_shape(const GridParams &g, bool zdirect, const ConvPtrs &b, int i,
+                                               PassArgs &a) {
   const int d = g.d, ng = g.ng, L = g.L, H = L / 2 + 1;
   const Plan pl = make_plan(d, zdirect);
   const int64_t G = d == 1 ? ng : d == 2 ? (int64_t)ng * ng : (int64_t)ng * ng * ng;
@@ -580,6 +702,21 @@ __host__ __device__ inline int conv_pass(const GridParams &g, bool zdirect, cons
       a.scale = 1.0f / ((float)L * (float)L * (float)L);
       return pl.pairs;
   }
+}
+
+// Whether a pass kind walks a strided axis (lines across threads).
+__host__ __device__ inline bool pass_strided(int kind) {
+  return kind == PK_KT_T || kind == PK_W_W_CONV || kind == PK_W_W_FWD || kind == PK_W_W_INV;
+}
+
+// Arguments of pass i (in the order of pass_kinds) of one convolution,
+// written in place; returns the channel pairs the pass runs over.
+__host__ __device__ inline int conv_pass(const GridParams &g, bool zdirect, const ConvPtrs &b, int i, PassArgs &a) {
+  const int pairs = conv_pass_shape(g, zdirect, b, i, a);
+  int kinds[kMaxPasses];
+  pass_kinds(g.d, kinds);
+  set_lines(a, pass_strided(kinds[i]));
+  return pairs;
 }
 
 // Arguments of every pass of one convolution (in the order of pass_kinds),

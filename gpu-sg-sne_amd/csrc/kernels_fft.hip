@@ -42,7 +42,7 @@ template <int IN, int OUT, int MID, bool EC>
 int run_pass(hipStream_t s, const PassArgs &a, int pairs) {
   const int blocks = pass_blocks(a);
   if (blocks == 0) return SGTSNE_OK;
-  const size_t lds = pass_lds_bytes(a.nl, a.L);
+  const size_t lds = pass_lds_bytes(a.nl, a.L, a.Lp);
   dim3 grid(blocks, pairs);
   hipLaunchKernelGGL((fft_pass_kernel<IN, OUT, MID, EC>), grid, dim3(kFftThreads), lds, s, a);
   SGT_LAUNCH_CHECK();
