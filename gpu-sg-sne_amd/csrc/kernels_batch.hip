@@ -52,10 +52,29 @@ constexpr int kGridSizesCount = sizeof(kGridSizesHost) / sizeof(kGridSizesHost[0
 // in the job's locality order) usually cover a small window of the grid: the
 // window is accumulated in LDS and added to the job's grid with one global
 // atomic per non-zero cell; items whose window does not fit add directly.
+// threads of the spread: in 3-D a work unit (a quarter item, at most 256
+// points) is split over 8 waves, each with one batch of at most 64 points
 template <int D>
-__global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ jobs,
-                                                   const BatchItem *__restrict__ items, int nitems,
-                                                   const float *__restrict__ Y, int lds_slots) {
+constexpr int s2g_threads() { return D == 3 ? 512 : kBT; }
+static_assert((kBatchChunk / 4 + s2g_threads<3>() / 64 - 1) / (s2g_threads<3>() / 64) <= 64,
+              "a 3-D spread wave takes one batch of at most 64 points");
+
+// min and max over the wave's 64 lanes
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int D>
+__global__ __launch_bounds__(s2g_threads<D>()) void bs2g_kernel(const BatchJob *__restrict__ jobs,
+                                                                const BatchItem *__restrict__ items, int nitems,
+                                                                const float *__restrict__ Y, int lds_slots) {
   extern __shared__ unsigned long long win[];
   __shared__ int sbox[6];
   constexpr int M = D + 1;
@@ -83,6 +102,24 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
     unsigned long long *V = reinterpret_cast<unsigned long long *>(J->vfix);
     const int ng = g.ng;
     const int G = grid_cells<D>(ng);
+    // each point slot of each wave takes a contiguous run of the work unit's
+    // points; a lane adds its points' integer contributions in registers while
+    // they land on the same node (consecutive points mostly share a cell) and
+    // issues one atomic per run.  Integer sums are exact, so the grid is the
+    // same bits as with one atomic per point.  A slot's points arrive NODES at
+    // a time: lane `node` loads point b + node of the run (one coalesced load
+    // for the slot) and every lane of the slot reads them in order from the
+    // loading lanes, so no lane waits on a global load per point.  In 3-D a
+    // wave's run is one such batch, loaded once for the window and the spread.
+    const int len = itm.end - itm.begin;
+    const int per_wave = (len + nwaves - 1) / nwaves, per_slot = (per_wave + PPW - 1) / PPW;
+    const int r0 = min(itm.end, itm.begin + wave * per_wave + slot * per_slot);
+    const int r1 = min(min(itm.end, itm.begin + (wave + 1) * per_wave), r0 + per_slot);
+    const int cnt = r1 - r0;  // this slot's points (<= 0: none)
+    float y3[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) y3[a] = 0.0f;
+    if (D == 3 && node < cnt) load_pt<D>(Y, r0 + node, y3);
     if (threadIdx.x < 3) {
       sbox[threadIdx.x] = INT_MAX;
       sbox[3 + threadIdx.x] = INT_MIN;
@@ -92,19 +129,33 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
       int lo[D], hi[D];
 #pragma unroll
       for (int a = 0; a < D; ++a) { lo[a] = INT_MAX; hi[a] = INT_MIN; }
-      for (int i = itm.begin + threadIdx.x; i < itm.end; i += blockDim.x) {
-        float y[D];
-        load_pt<D>(Y, i, y);
+      auto widen = [&](const float (&y)[D]) {
         int f[D];
         float w[D][4];
         stencil<D>(g, y, f, w);
 #pragma unroll
         for (int a = 0; a < D; ++a) { lo[a] = min(lo[a], f[a]); hi[a] = max(hi[a], f[a]); }
+      };
+      if constexpr (D == 3) {
+        if (node < cnt) widen(y3);
+      } else {
+        for (int i = itm.begin + threadIdx.x; i < itm.end; i += blockDim.x) {
+          float y[D];
+          load_pt<D>(Y, i, y);
+          widen(y);
+        }
       }
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        atomicMin(&sbox[a], lo[a]);
-        atomicMax(&sbox[3 + a], hi[a]);
+        lo[a] = wave_min_i(lo[a]);
+        hi[a] = wave_max_i(hi[a]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          atomicMin(&sbox[a], lo[a]);
+          atomicMax(&sbox[3 + a], hi[a]);
+        }
       }
     }
     __syncthreads();
@@ -118,23 +169,39 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
     }
     const bool in_lds = (int64_t)wcells * M <= (int64_t)lds_slots;
     __syncthreads();  // sbox is reset by the next item
-    // each point slot of each wave takes a contiguous run of the work unit's
-    // points; a lane adds its points' integer contributions in registers while
-    // they land on the same node (consecutive points mostly share a cell) and
-    // issues one atomic per run.  Integer sums are exact, so the grid is the
-    // same bits as with one atomic per point.
-    const int len = itm.end - itm.begin;
-    const int per_wave = (len + nwaves - 1) / nwaves, per_slot = (per_wave + PPW - 1) / PPW;
-    const int r0 = min(itm.end, itm.begin + wave * per_wave + slot * per_slot);
-    const int r1 = min(min(itm.end, itm.begin + (wave + 1) * per_wave), r0 + per_slot);
+    auto for_points = [&](auto &&visit) {
+      if constexpr (D == 3) {  // one slot per wave: the source lane is uniform
+        for (int k = 0; k < cnt; ++k) {
+          float y[D];
+#pragma unroll
+          for (int a = 0; a < D; ++a) y[a] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y3[a]), k));
+          visit(y);
+        }
+      } else {
+        for (int b = 0; b < per_slot; b += NODES) {
+          float yv[D];
+          if (b + node < cnt) {
+            load_pt<D>(Y, r0 + b + node, yv);
+          } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) yv[a] = 0.0f;
+          }
+          const int kn = min(NODES, per_slot - b);
+          for (int k = 0; k < kn; ++k) {
+            float y[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) y[a] = __shfl(yv[a], slot * NODES + k, 64);
+            if (b + k < cnt) visit(y);
+          }
+        }
+      }
+    };
     unsigned long long acc[M];
     int cur = -1;
     if (in_lds) {
       for (int e = threadIdx.x; e < wcells * M; e += blockDim.x) win[e] = 0ull;
       __syncthreads();
-      for (int i = r0; i < r1; ++i) {
-        float y[D];
-        load_pt<D>(Y, i, y);
+      for_points([&](const float (&y)[D]) {
         int f[D];
         float ww;
         lane_node<D>(g, y, a0, a1, a2, f, ww);
@@ -158,7 +225,7 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
         acc[0] += fix_round(ww, fsf);
 #pragma unroll
         for (int a = 0; a < D; ++a) acc[a + 1] += fix_round((y[a] - g.center[a]) * ww, fsf);
-      }
+      });
       if (cur >= 0) {
 #pragma unroll
         for (int m = 0; m < M; ++m) atomicAdd(&win[m * wcells + cur], acc[m]);
@@ -185,9 +252,7 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
       }
       __syncthreads();  // the window is cleared by the next item
     } else {
-      for (int i = r0; i < r1; ++i) {
-        float y[D];
-        load_pt<D>(Y, i, y);
+      for_points([&](const float (&y)[D]) {
         int gi;
         float ww;
         node_of_lane<D>(g, y, node, gi, ww);
@@ -203,7 +268,7 @@ __global__ __launch_bounds__(kBT) void bs2g_kernel(const BatchJob *__restrict__ 
         acc[0] += fix_round(ww, fsf);
 #pragma unroll
         for (int a = 0; a < D; ++a) acc[a + 1] += fix_round((y[a] - g.center[a]) * ww, fsf);
-      }
+      });
       if (cur >= 0) {
 #pragma unroll
         for (int m = 0; m < M; ++m) atomicAdd(&V[(int64_t)m * G + cur], acc[m]);
@@ -652,7 +717,7 @@ __global__ void offset_copy_kernel(const int32_t *__restrict__ in, int32_t *__re
 template <int D>
 int bs2g_dispatch(hipStream_t s, const BatchJob *jobs, const BatchItem *items, int nitems, const float *Y, int blocks,
                   int lds_bytes) {
-  bs2g_kernel<D><<<blocks, kBT, lds_bytes, s>>>(jobs, items, nitems, Y, lds_bytes / 8);
+  bs2g_kernel<D><<<blocks, s2g_threads<D>(), lds_bytes, s>>>(jobs, items, nitems, Y, lds_bytes / 8);
   SGT_LAUNCH_CHECK();
   return SGTSNE_OK;
 }
