@@ -43,7 +43,7 @@ struct BatchJob {
   double h;              // grid spacing (h-driven grid size)
   float mean0[4];        // subtracted from Y at admission
   long long *vfix;       // spread grid, fixed point [d+1][ng^d] (ng <= ng_cap)
-  float *phi;            // potentials [d+2][ng^d]
+  float *phi;            // potentials: [ng^d] float4 of channels 0-3, then channel 4 [ng^d] (3-D)
   float2 *W, *Kt1, *Kt2, *Kh;
   // dynamic
   int iter, status;

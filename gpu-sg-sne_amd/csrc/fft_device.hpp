@@ -83,7 +83,10 @@ __device__ __forceinline__ int fdiv(int x, int d, float inv) {
 // converts it and clears every element it reads, so the grid is zero again
 // for the next spread.
 enum InKind { IN_VPAIR = 0, IN_W = 1, IN_KGEN = 2, IN_KT = 3, IN_VFIX = 4 };
-enum OutKind { OUT_W = 0, OUT_PHIPAIR = 1, OUT_KT = 2 };
+// OUT_PHIQ: potentials node-major for the batched gather, channels 0-3 of a
+// node as one float4 (Phi[4 node + c]) and channel 4 as a plane after them
+// (Phi[4 G + node]).
+enum OutKind { OUT_W = 0, OUT_PHIPAIR = 1, OUT_KT = 2, OUT_PHIQ = 3 };
 enum MidKind { MID_FWD = 0, MID_INV = 1, MID_CONV = 2 };
 
 // The pass shapes of one convolution (template arguments of the pass kernel).
@@ -510,6 +513,11 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
       float *Phi = static_cast<float *>(a.out);
       Phi[(int64_t)a.out_c[p][0] * a.pair_out + off] = v.x * a.scale;
       if (a.out_c[p][1] >= 0) Phi[(int64_t)a.out_c[p][1] * a.pair_out + off] = v.y * a.scale;
+    } else if (OUT == OUT_PHIQ) {
+      float *Phi = static_cast<float *>(a.out);
+      const int c0 = a.out_c[p][0], c1 = a.out_c[p][1];
+      Phi[c0 < 4 ? off * 4 + c0 : 4 * a.pair_out + off] = v.x * a.scale;
+      if (c1 >= 0) Phi[c1 < 4 ? off * 4 + c1 : 4 * a.pair_out + off] = v.y * a.scale;
     } else {  // the spectra of real even kernels are real: keep (Re, Im) = (K2^, K1^)
       float2 *K = static_cast<float2 *>(a.out);
       K[off] = v;

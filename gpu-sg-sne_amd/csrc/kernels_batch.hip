@@ -312,9 +312,12 @@ __global__ __launch_bounds__(g2s_threads<D>()) void bg2s_kernel(const BatchJob *
     const BatchJob *J = jobs + itm.job;
     if (J->status != BJ_RUN) continue;
     const GridParams g = J->g;
-    const float *Phi = J->phi;
+    // potentials node-major: channels 0-3 of a node as one float4, channel 4
+    // (the direct Z term in 3-D) as a plane after them
+    const float4 *Phi4 = reinterpret_cast<const float4 *>(J->phi);
     const int ng = g.ng;
     const int G = grid_cells<D>(ng);
+    const float *Phi1 = J->phi + 4 * (size_t)G;
     Neu z{0.0, 0.0};
     for (int i = itm.begin + threadIdx.x; i < itm.end; i += blockDim.x) {
       float y[D];
@@ -326,8 +329,10 @@ __global__ __launch_bounds__(g2s_threads<D>()) void bg2s_kernel(const BatchJob *
 #pragma unroll
       for (int c = 0; c < C; ++c) ph[c] = 0.0f;
       for_stencil<D>(ng, f, w, [&](int idx, float ww) {
+        const float4 q = Phi4[idx];
+        const float qc[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-        for (int c = 0; c < C; ++c) ph[c] += Phi[c * G + idx] * ww;
+        for (int c = 0; c < C; ++c) ph[c] += (c < 4 ? qc[c] : Phi1[idx]) * ww;
       });
       float fr[D];
       double ysq = 0.0, ydot = 0.0;
@@ -738,7 +743,7 @@ size_t batch_grid_bytes(int d, int ng_cap, size_t *vfix_b, size_t *phi_b, size_t
   const int L = fft_length(2 * ng_cap - 1);  // FFT lengths grow with ng: the largest one
   const size_t G = d == 1 ? (size_t)ng_cap : d == 2 ? (size_t)ng_cap * ng_cap : (size_t)ng_cap * ng_cap * ng_cap;
   *vfix_b = G * (size_t)(d + 1) * sizeof(long long);
-  *phi_b = G * (size_t)(d + 2) * sizeof(float);
+  *phi_b = G * (size_t)(d == 3 ? 5 : 4) * sizeof(float);  // node-major float4 (+ a plane in 3-D)
   conv_workspace_bytes(d, ng_cap, L, w_b, k_b);
   if (*w_b == 0) *w_b = sizeof(float2);
   return *vfix_b + *phi_b + *w_b + 3 * *k_b;
@@ -776,8 +781,8 @@ int launch_bconv(hipStream_t s, int d, int zdirect, const void *args_v, const in
       case PK_KT_T: SGT_TRY((bpass<IN_KT, OUT_KT, MID_FWD, false>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       case PK_V_W: SGT_TRY((bpass<IN_VFIX, OUT_W, MID_FWD, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       case PK_W_W_CONV: SGT_TRY((bpass<IN_W, OUT_W, MID_CONV, false>(s, args, prefix, njobs, np, p, pr, blocks))); break;
-      case PK_W_PHI: SGT_TRY((bpass<IN_W, OUT_PHIPAIR, MID_INV, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
-      case PK_V_PHI: SGT_TRY((bpass<IN_VFIX, OUT_PHIPAIR, MID_CONV, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
+      case PK_W_PHI: SGT_TRY((bpass<IN_W, OUT_PHIQ, MID_INV, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
+      case PK_V_PHI: SGT_TRY((bpass<IN_VFIX, OUT_PHIQ, MID_CONV, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       case PK_W_W_FWD: SGT_TRY((bpass<IN_W, OUT_W, MID_FWD, false>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       case PK_W_W_INV: SGT_TRY((bpass<IN_W, OUT_W, MID_INV, false>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       default: SGT_CHECK(false, SGTSNE_ERR_ARG, "bad FFT pass kind");
