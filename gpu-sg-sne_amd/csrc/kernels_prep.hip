@@ -336,9 +336,49 @@ __device__ __forceinline__ float proj_sign(int axis, int l) {
   return (h & 1u) ? 1.0f : -1.0f;
 }
 
+// Cells of the point set: the nearest of P pivots (points (p n) / P, P <= 64;
+// FP32 distances of the centred points, ties to the lower pivot), so that
+// points of one cluster share cells.  Used only to order the points; any
+// assignment gives the same kNN rows.
+constexpr int kKnnMaxCells = 64;
+
+__global__ __launch_bounds__(256) void knn_cell_kernel(const float *__restrict__ Xc, int n, int dp, int P,
+                                                       int32_t *__restrict__ cell) {
+  extern __shared__ float piv[];  // [P][dp]
+  for (int t = threadIdx.x; t < P * dp; t += blockDim.x) {
+    const int p = t / dp, l = t - p * dp;
+    piv[t] = Xc[(size_t)(((int64_t)p * n) / P) * dp + l];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 *x4 = reinterpret_cast<const float4 *>(Xc + (size_t)i * dp);
+  float best = INFINITY;
+  int bi = 0;
+  for (int p = 0; p < P; ++p) {
+    const float *pp = piv + p * dp;
+    float d2 = 0.0f;
+    for (int c = 0; c < dp / 4; ++c) {
+      const float4 v = x4[c];
+      const float a = v.x - pp[4 * c], b = v.y - pp[4 * c + 1], e = v.z - pp[4 * c + 2], f = v.w - pp[4 * c + 3];
+      d2 = fmaf(a, a, d2);
+      d2 = fmaf(b, b, d2);
+      d2 = fmaf(e, e, d2);
+      d2 = fmaf(f, f, d2);
+    }
+    if (d2 < best) {
+      best = d2;
+      bi = p;
+    }
+  }
+  cell[i] = bi;
+}
+
+// Sort keys: the cell in the top 6 bits (when cells are given), then the
+// Morton code of a 3-D projection of the centred point.
 __global__ void knn_morton_key_kernel(const float *__restrict__ Xc, int n, int dim, int dp,
-                                      const unsigned long long *__restrict__ rmax, uint32_t *__restrict__ key,
-                                      int32_t *__restrict__ idx) {
+                                      const unsigned long long *__restrict__ rmax, const int32_t *__restrict__ cell,
+                                      uint32_t *__restrict__ key, int32_t *__restrict__ idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float R = (float)__longlong_as_double((long long)*rmax);
@@ -355,8 +395,78 @@ __global__ void knn_morton_key_kernel(const float *__restrict__ Xc, int n, int d
     const float u = fminf(fmaxf(p[a] * scale + 512.0f, 0.0f), 1023.0f);
     k |= spread_bits10((uint32_t)u) << a;
   }
-  key[i] = k;
+  key[i] = cell ? ((uint32_t)cell[i] << 24) | (k >> 6) : k;
   idx[i] = i;
+}
+
+// Points per cell (the sorted order groups each cell's points together).
+__global__ void knn_cell_count_kernel(const int32_t *__restrict__ cell, int n, int *__restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[cell[i]], 1);
+}
+
+// First sorted position of each cell, and its first reference slot: every
+// cell starts a fresh tile of kFiltR slots (the unused slots are padding).
+__global__ void knn_cell_offsets_kernel(const int *__restrict__ cnt, int P, int *__restrict__ start,
+                                        int *__restrict__ slot0) {
+  if (threadIdx.x != 0) return;
+  int a = 0, b = 0;
+  for (int c = 0; c < P; ++c) {
+    start[c] = a;
+    slot0[c] = b;
+    a += cnt[c];
+    b += (cnt[c] + kFiltR - 1) / kFiltR * kFiltR;
+  }
+}
+
+// The reference stream in cell order: slot of sorted position i, its point
+// id (padding slots keep -1 and zero rows) and the id -> slot map.
+__global__ void knn_ref_scatter_kernel(const float *__restrict__ Xc, int n, int dp,
+                                       const int32_t *__restrict__ order, const int32_t *__restrict__ cell,
+                                       const int *__restrict__ start, const int *__restrict__ slot0,
+                                       float *__restrict__ Xr, int32_t *__restrict__ rid, int32_t *__restrict__ rinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = order[i], c = cell[id];
+  const int sl = slot0[c] + (i - start[c]);
+  rid[sl] = id;
+  rinv[id] = sl;
+  const float4 *src = reinterpret_cast<const float4 *>(Xc + (size_t)id * dp);
+  float4 *dst = reinterpret_cast<float4 *>(Xr + (size_t)sl * dp);
+  for (int c4 = 0; c4 < dp / 4; ++c4) dst[c4] = src[c4];
+}
+
+// Bounding sphere of each reference tile (one wave per tile), in the
+// caller's coordinates (FP64): centre = mean of the tile's points, radius =
+// the largest distance to it; -1 for a tile of padding only.
+__global__ __launch_bounds__(64) void knn_tile_sphere_kernel(const double *__restrict__ X, int dim,
+                                                             const int32_t *__restrict__ rid,
+                                                             double *__restrict__ tcen, double *__restrict__ trad) {
+  __shared__ double cen[kFiltMaxDp];
+  const int t = blockIdx.x, lane = threadIdx.x;
+  const int id = rid[(size_t)t * kFiltR + lane];
+  const bool ok = id >= 0;
+  const double cnt = wave_sum(ok ? 1.0 : 0.0);
+  if (cnt == 0.0) {
+    if (lane == 0) trad[t] = -1.0;
+    return;
+  }
+  for (int l = 0; l < dim; ++l) {
+    const double c = wave_sum(ok ? X[(size_t)id * dim + l] : 0.0) / cnt;
+    if (lane == 0) {
+      cen[l] = c;
+      tcen[(size_t)t * dim + l] = c;
+    }
+  }
+  __syncthreads();
+  double r2 = 0.0;
+  if (ok)
+    for (int l = 0; l < dim; ++l) {
+      const double v = X[(size_t)id * dim + l] - cen[l];
+      r2 = fma(v, v, r2);
+    }
+  const double r = wave_max(sqrt(r2));
+  if (lane == 0) trad[t] = r;
 }
 
 // One work-group per query block: the block's mean c_b (FP64) over its valid
@@ -517,8 +627,22 @@ __global__ __launch_bounds__(256) void knn_filter_kernel(const float *__restrict
 // to LDS; each query's candidates of the tile form a 64-bit mask built from
 // wave ballots (16 bits per 16-lane group), and the merge walks the mask in
 // reference order and sifts into the LDS heap.
+//
+// The references come as tiles of a cell order (see knn_cell_kernel), each
+// tile with a bounding sphere.  A block starts at the tiles around its own
+// queries' (its lists fill with near candidates), then visits the others,
+// 128 at a time, skipping every tile that cannot hold a candidate: with D
+// the distance between the block's and the tile's spheres, every query a and
+// reference b of them have a true distance of at least D, and the filter
+// value of the pair is within 2^-12 S^2 of its squared distance (S = block
+// radius + centre distance + tile radius + R + |c_b - mu|: more than eight
+// times the certificate's error bounds for the rounding of the centred
+// points and the FP32 dot products), so a tile with D^2 - 2^-12 S^2 above
+// every query's current worst key holds no candidate.  Skipped tiles thus
+// leave the lists as a full scan would: the k + kFiltSlack smallest keys.
 constexpr int kF2Q = 32;
 constexpr int kF2MaxKp = 128;
+constexpr int kFiltHomeBack = 16;  // tiles visited before a block's own
 
 // A reference tile shifted into a query block's frame: rt holds
 // fl32(x~ - delta) transposed, and rnp[t] the partial squared norm of the
@@ -546,9 +670,22 @@ __device__ __forceinline__ void tile_store_shift(float *rt, float *rnp, const fl
   rnp[threadIdx.x] = ss;
 }
 
+// The reference stream of the LDS-heap filter: tiles of kFiltR slots in
+// cell order (Xr rows, -1 ids for padding), their spheres, and the query
+// blocks' spheres (centre cb, radius from the queries' norms qa, |c_b - mu|
+// in dn; R the largest |x - mu|).
+struct RefTiles {
+  const float *Xr;
+  const int32_t *rid, *rinv;
+  int ntiles;
+  const double *tcen, *trad;
+  const double *cb, *qa, *dn;
+  const unsigned long long *rmax;
+  int dim;
+};
+
 template <int NST>
-__global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__restrict__ Xc,
-                                                             const float *__restrict__ Xq,
+__global__ __launch_bounds__(128) void knn_filter_lds_kernel(const RefTiles rf, const float *__restrict__ Xq,
                                                              const float *__restrict__ qn32,
                                                              const int32_t *__restrict__ qorder,
                                                              const float *__restrict__ delta, int n, int kp,
@@ -563,9 +700,13 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__rest
   __shared__ float rnp[128];
   __shared__ float qn[kF2Q];
   __shared__ int qid[kF2Q];
+  __shared__ unsigned long long skipm[2];
+  __shared__ float sthr;
+  __shared__ double srho;
 
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4;
   const int q0 = blockIdx.x * kF2Q;  // first slot of this query block
+  const int ntiles = rf.ntiles;
   float areg[NST];
   {
     const int sa = q0 + 16 * w + (lane & 15);
@@ -587,73 +728,133 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const float *__rest
     qn[tid] = q0 + tid < n ? qn32[q0 + tid] : 0.0f;
     qid[tid] = q0 + tid < n ? qorder[q0 + tid] : -1;
   }
+  if (w == 0) {  // the block's radius: its queries' largest |x - c_b| (|a| rounded up)
+    const int q = lane < kF2Q && q0 + lane < n ? qorder[q0 + lane] : -1;
+    const double r = wave_max(q >= 0 ? rf.qa[q] * (1.0 + 0x1p-20) : 0.0);
+    if (lane == 0) srho = r;
+  }
+  const int home = rf.rinv[qorder[q0]] / kFiltR;
+  const double R = __longlong_as_double((long long)*rf.rmax);
+  const double dnb = rf.dn[blockIdx.x];
+  const double *cbb = rf.cb + (size_t)blockIdx.x * rf.dim;
+  auto visit = [&](int sq) {  // the sq-th tile visited
+    int t = (home - kFiltHomeBack + sq) % ntiles;
+    return t < 0 ? t + ntiles : t;
+  };
   float4 buf[kFiltR * NST / 128];
   float rnv;
-  tile_load<128, NST>(Xc, Xc, n, 0, buf, rnv);
-  for (int r0 = 0; r0 < n; r0 += kFiltR) {
+  for (int g0 = 0; g0 < ntiles; g0 += 128) {
     __syncthreads();  // the previous tile's merge is done
-    tile_store_shift<NST>(rt, rnp, buf, dl);
-    __syncthreads();
-    if (r0 + kFiltR < n) tile_load<128, NST>(Xc, Xc, n, r0 + kFiltR, buf, rnv);
-    fv4 acc[4];
-    tile_dots<NST>(areg, rt, lane, acc);
-    // this lane's four query rows 16 w + 4 g + v: norms and current worst keys
-    float qv[4], thd[4];
-    int thi[4], qq[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int ql = 16 * w + 4 * g + v;
-      qv[v] = qn[ql];
-      thd[v] = hk[ql * kp];
-      thi[v] = hx[ql * kp];
-      qq[v] = qid[ql];
+    if (w == 0) {  // the worst key over the block's queries (infinite until every list is full)
+      const float th = lane < kF2Q && qid[lane] >= 0 ? hk[lane * kp] : -INFINITY;
+      const float m = (float)wave_max((double)th);
+      if (lane == 0) sthr = m;
     }
-    unsigned long long mq[4] = {0ull, 0ull, 0ull, 0ull};
+    __syncthreads();
+    bool skip = true;
+    if (g0 + tid < ntiles) {
+      const int t = visit(g0 + tid);
+      const double rt_ = rf.trad[t];
+      const double thr = (double)sthr;
+      if (rt_ < 0.0) {
+        skip = true;  // padding only
+      } else if (!(thr < INFINITY)) {
+        skip = false;
+      } else {
+        const double *tc = rf.tcen + (size_t)t * rf.dim;
+        double d2 = 0.0;
+        for (int l = 0; l < rf.dim; ++l) {
+          const double v = cbb[l] - tc[l];
+          d2 = fma(v, v, d2);
+        }
+        const double dcm = sqrt(d2), rho = srho;
+        const double dlb = dcm - rho - rt_;
+        const double S = rho + dcm + rt_ + R + dnb;
+        skip = dlb > 0.0 && dlb * dlb - 0x1p-12 * S * S > thr;
+      }
+    }
+    const unsigned long long sb = __ballot(skip);
+    if (lane == 0) skipm[w] = sb;
+    __syncthreads();
+    const unsigned long long todo[2] = {~skipm[0], ~skipm[1]};
+    auto next_tile = [&](int i) {  // the next visited index of this group after i, or -1
+      for (int k = i + 1; k < 128; ++k)
+        if ((todo[k >> 6] >> (k & 63)) & 1ull) return k;
+      return -1;
+    };
+    int i = next_tile(-1);
+    if (i >= 0) tile_load<128, NST>(rf.Xr, rf.Xr, ntiles * kFiltR, visit(g0 + i) * kFiltR, buf, rnv);
+    while (i >= 0) {
+      const int r0 = visit(g0 + i) * kFiltR;
+      const int inext = next_tile(i);
+      int rr[4];  // ids of this lane's reference columns 16 t + lane % 16
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int jl = 16 * t + (lane & 15), r = r0 + jl;
-      const float rv = rnp[jl] + rnp[jl + 64];
-      const bool rok = r < n;
+      for (int t = 0; t < 4; ++t) rr[t] = rf.rid[r0 + 16 * t + (lane & 15)];
+      __syncthreads();  // the previous tile's merge is done
+      tile_store_shift<NST>(rt, rnp, buf, dl);
+      __syncthreads();
+      if (inext >= 0) tile_load<128, NST>(rf.Xr, rf.Xr, ntiles * kFiltR, visit(g0 + inext) * kFiltR, buf, rnv);
+      fv4 acc[4];
+      tile_dots<NST>(areg, rt, lane, acc);
+      // this lane's four query rows 16 w + 4 g + v: norms and current worst keys
+      float qv[4], thd[4];
+      int thi[4], qq[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int ql = 16 * w + 4 * g + v;
-        const float dd = (qv[v] + rv) - 2.0f * acc[t][v];
-        dist[ql][jl] = dd;
-        const bool cand = qq[v] >= 0 && rok && r != qq[v] && key_lessf(dd, r, thd[v], thi[v]);
-        const unsigned long long b = __ballot(cand);
-        mq[v] |= ((b >> (16 * g)) & 0xFFFFull) << (16 * t);
+        qv[v] = qn[ql];
+        thd[v] = hk[ql * kp];
+        thi[v] = hx[ql * kp];
+        qq[v] = qid[ql];
       }
-    }
-    if ((lane & 15) == 0)
+      unsigned long long mq[4] = {0ull, 0ull, 0ull, 0ull};
 #pragma unroll
-      for (int v = 0; v < 4; ++v) mask[16 * w + 4 * g + v] = mq[v];
-    __syncthreads();
-    if (tid < kF2Q) {
-      float *H = hk + tid * kp;
-      int *HI = hx + tid * kp;
-      unsigned long long m = mask[tid];
-      while (m) {
-        const int j = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const float cd = dist[tid][j];
-        const int ci = r0 + j;
-        if (!key_lessf(cd, ci, H[0], HI[0])) continue;
-        int pos = 0;
-        for (;;) {
-          const int l = 2 * pos + 1, rr = l + 1;
-          int mm = -1;
-          float md = cd;
-          int mi = ci;
-          if (l < kp && key_lessf(md, mi, H[l], HI[l])) { mm = l; md = H[l]; mi = HI[l]; }
-          if (rr < kp && key_lessf(md, mi, H[rr], HI[rr])) { mm = rr; md = H[rr]; mi = HI[rr]; }
-          if (mm < 0) break;
-          H[pos] = md;
-          HI[pos] = mi;
-          pos = mm;
+      for (int t = 0; t < 4; ++t) {
+        const int jl = 16 * t + (lane & 15);
+        const float rv = rnp[jl] + rnp[jl + 64];
+        const int r = rr[t];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ql = 16 * w + 4 * g + v;
+          const float dd = (qv[v] + rv) - 2.0f * acc[t][v];
+          dist[ql][jl] = dd;
+          const bool cand = qq[v] >= 0 && r >= 0 && r != qq[v] && key_lessf(dd, r, thd[v], thi[v]);
+          const unsigned long long b = __ballot(cand);
+          mq[v] |= ((b >> (16 * g)) & 0xFFFFull) << (16 * t);
         }
-        H[pos] = cd;
-        HI[pos] = ci;
       }
+      if ((lane & 15) == 0)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) mask[16 * w + 4 * g + v] = mq[v];
+      __syncthreads();
+      if (tid < kF2Q) {
+        float *H = hk + tid * kp;
+        int *HI = hx + tid * kp;
+        unsigned long long m = mask[tid];
+        while (m) {
+          const int j = __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const float cd = dist[tid][j];
+          const int ci = rf.rid[r0 + j];
+          if (!key_lessf(cd, ci, H[0], HI[0])) continue;
+          int pos = 0;
+          for (;;) {
+            const int l = 2 * pos + 1, rr2 = l + 1;
+            int mm = -1;
+            float md = cd;
+            int mi = ci;
+            if (l < kp && key_lessf(md, mi, H[l], HI[l])) { mm = l; md = H[l]; mi = HI[l]; }
+            if (rr2 < kp && key_lessf(md, mi, H[rr2], HI[rr2])) { mm = rr2; md = H[rr2]; mi = HI[rr2]; }
+            if (mm < 0) break;
+            H[pos] = md;
+            HI[pos] = mi;
+            pos = mm;
+          }
+          H[pos] = cd;
+          HI[pos] = ci;
+        }
+      }
+      i = inext;
     }
   }
   __syncthreads();
@@ -673,6 +874,7 @@ struct BlockCentred {
   const float *qn32;       // their squared norms
   const int32_t *qorder;   // slot -> query id
   const float *delta;      // per block: fl32(c_b - mu), padded
+  RefTiles rf;             // the references in cell order, with tile spheres
 };
 
 template <int NST>
@@ -680,7 +882,7 @@ int launch_filter(hipStream_t s, const float *Xc, const float *n32, int n, int k
                   float *hd, int *hi) {
   if (kp <= kF2MaxKp) {
     const size_t lds = (sizeof(float) + sizeof(int)) * (size_t)kF2Q * kp;
-    knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(Xc, bc.Xq, bc.qn32, bc.qorder, bc.delta, n, kp,
+    knn_filter_lds_kernel<NST><<<div_up(n, kF2Q), 128, lds, s>>>(bc.rf, bc.Xq, bc.qn32, bc.qorder, bc.delta, n, kp,
                                                                  hd, hi);
   } else {
     if constexpr ((kFiltR * NST) % 256 == 0) {
@@ -1143,7 +1345,10 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   double *mu = nullptr, *nrm = nullptr, *cb = nullptr, *dn = nullptr, *qa = nullptr, *qdn = nullptr;
   float *Xc = nullptr, *n32 = nullptr, *hd = nullptr, *Xq = nullptr, *qn32 = nullptr, *delta = nullptr;
   int *hi = nullptr, *fb = nullptr;
-  int32_t *kidx = nullptr, *qorder = nullptr;
+  int32_t *kidx = nullptr, *qorder = nullptr, *cell = nullptr, *rid = nullptr, *rinv = nullptr;
+  int *ccnt = nullptr;  // per cell: count | first sorted position | first slot
+  float *Xr = nullptr;
+  double *tcen = nullptr, *trad = nullptr;
   uint32_t *key0 = nullptr, *key1 = nullptr;
   void *tmp = nullptr;
   unsigned long long *rmax = nullptr;
@@ -1174,18 +1379,43 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
     SGT_HIP(hipMallocAsync((void **)&Xq, sizeof(float) * (size_t)n * dp, s));
     SGT_HIP(hipMallocAsync((void **)&qn32, sizeof(float) * (size_t)n, s));
     SGT_HIP(hipMallocAsync((void **)&qa, sizeof(double) * (size_t)n, s));
-    // query order: Morton code of a 3-D projection of the centred points
-    knn_morton_key_kernel<<<div_up(n, 256), 256, 0, s>>>(Xc, n, dim, dp, rmax, key0, kidx);
+    // point order: cell (nearest of P pivots), then the Morton code of a 3-D
+    // projection of the centred points; the query blocks and the reference
+    // tiles both follow it, the tiles padded so each lies in one cell
+    const int P = std::min(kKnnMaxCells, std::max(1, n / 2048));
+    const int ntiles = div_up(n + (kFiltR - 1) * P, kFiltR);  // at least the padded slots
+    SGT_HIP(hipMallocAsync((void **)&cell, sizeof(int32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&ccnt, sizeof(int) * 3 * kKnnMaxCells, s));
+    SGT_HIP(hipMallocAsync((void **)&rid, sizeof(int32_t) * (size_t)ntiles * kFiltR, s));
+    SGT_HIP(hipMallocAsync((void **)&rinv, sizeof(int32_t) * (size_t)n, s));
+    SGT_HIP(hipMallocAsync((void **)&Xr, sizeof(float) * (size_t)ntiles * kFiltR * dp, s));
+    SGT_HIP(hipMallocAsync((void **)&tcen, sizeof(double) * (size_t)ntiles * dim, s));
+    SGT_HIP(hipMallocAsync((void **)&trad, sizeof(double) * (size_t)ntiles, s));
+    knn_cell_kernel<<<div_up(n, 256), 256, sizeof(float) * P * dp, s>>>(Xc, n, dp, P, cell);
+    SGT_LAUNCH_CHECK();
+    knn_morton_key_kernel<<<div_up(n, 256), 256, 0, s>>>(Xc, n, dim, dp, rmax, cell, key0, kidx);
     SGT_LAUNCH_CHECK();
     size_t tb = 0;
     SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key0, key1, kidx, qorder, n, 0, 30, s));
     SGT_HIP(hipMallocAsync(&tmp, tb, s));
     SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key0, key1, kidx, qorder, n, 0, 30, s));
+    SGT_HIP(hipMemsetAsync(ccnt, 0, sizeof(int) * kKnnMaxCells, s));
+    knn_cell_count_kernel<<<div_up(n, 256), 256, 0, s>>>(cell, n, ccnt);
+    SGT_LAUNCH_CHECK();
+    knn_cell_offsets_kernel<<<1, 64, 0, s>>>(ccnt, P, ccnt + kKnnMaxCells, ccnt + 2 * kKnnMaxCells);
+    SGT_LAUNCH_CHECK();
+    SGT_HIP(hipMemsetAsync(rid, 0xFF, sizeof(int32_t) * (size_t)ntiles * kFiltR, s));  // -1: padding
+    SGT_HIP(hipMemsetAsync(Xr, 0, sizeof(float) * (size_t)ntiles * kFiltR * dp, s));
+    knn_ref_scatter_kernel<<<div_up(n, 256), 256, 0, s>>>(Xc, n, dp, qorder, cell, ccnt + kKnnMaxCells,
+                                                          ccnt + 2 * kKnnMaxCells, Xr, rid, rinv);
+    SGT_LAUNCH_CHECK();
+    knn_tile_sphere_kernel<<<ntiles, 64, 0, s>>>(X, dim, rid, tcen, trad);
+    SGT_LAUNCH_CHECK();
     knn_block_centre_kernel<<<nb, 256, 0, s>>>(X, n, dim, dp, qorder, mu, cb, delta, dn, kF2Q);
     SGT_LAUNCH_CHECK();
     knn_query_prep_kernel<<<div_up(n, 256), 256, 0, s>>>(X, n, dim, dp, qorder, cb, dn, Xq, qn32, qa, qdn, kF2Q);
     SGT_LAUNCH_CHECK();
-    bc = BlockCentred{Xq, qn32, qorder, delta};
+    bc = BlockCentred{Xq, qn32, qorder, delta, RefTiles{Xr, rid, rinv, ntiles, tcen, trad, cb, qa, dn, rmax, dim}};
   } else {
     SGT_HIP(hipMemsetAsync(qdn, 0, sizeof(double) * (size_t)n, s));  // frame of every query: mu
   }
@@ -1206,7 +1436,8 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   int st = SGTSNE_OK;
   if (nfb > 0) st = knn_exact_rows(s, X, n, dim, k, fb + 1, nfb, I, D);
   if (fallback_rows) *fallback_rows = nfb;
-  void *bufs[] = {mu, nrm, Xc, n32, hd, hi, fb, rmax, qdn, key0, key1, kidx, qorder, cb, delta, dn, Xq, qn32, qa, tmp};
+  void *bufs[] = {mu, nrm, Xc, n32, hd, hi, fb, rmax, qdn, key0, key1, kidx, qorder, cb, delta, dn, Xq, qn32, qa, tmp,
+                  cell, ccnt, rid, rinv, Xr, tcen, trad};
   for (void *p : bufs)
     if (p) (void)hipFreeAsync(p, s);
   return st;

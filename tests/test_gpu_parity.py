@@ -95,6 +95,20 @@ def test_knn_methods_agree_large(sg):
     assert 0 <= fb < 200
 
 
+@pytest.mark.parametrize("data", ["gaussian", "powerlaw"])
+def test_knn_filtered_pruned_clusters(sg, data):
+    # clustered inputs at a size where the filter orders the points by 19
+    # cells and skips the tiles of far cells: every row still equals the
+    # exact FP64 kernel's (itself pinned to the oracle above)
+    from sgtsne_amd import synth
+
+    X = synth.gaussian_job(40000, seed=77)[0] if data == "gaussian" else synth.powerlaw_mixture(n=40000)[0]
+    Ie, De, _ = sg.knn(X, 90, sg.KNN_EXACT_FP64)
+    If, Df, fb = sg.knn(X, 90, sg.KNN_FILTERED)
+    assert np.array_equal(Ie, If) and np.array_equal(De, Df)
+    assert 0 <= fb < 400
+
+
 # ---------------------------------------------------------------- P construction
 def test_perplexity_golden(sg, mnist):
     f, g = mnist
