@@ -399,10 +399,16 @@ __global__ void knn_morton_key_kernel(const float *__restrict__ Xc, int n, int d
   idx[i] = i;
 }
 
-// Points per cell (the sorted order groups each cell's points together).
-__global__ void knn_cell_count_kernel(const int32_t *__restrict__ cell, int n, int *__restrict__ cnt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[cell[i]], 1);
+// Points per cell (the sorted order groups each cell's points together):
+// a work-group histogram in LDS, then one global addition per cell.
+__global__ __launch_bounds__(256) void knn_cell_count_kernel(const int32_t *__restrict__ cell, int n,
+                                                             int *__restrict__ cnt) {
+  __shared__ int h[kKnnMaxCells];
+  if (threadIdx.x < kKnnMaxCells) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) atomicAdd(&h[cell[i]], 1);
+  __syncthreads();
+  if (threadIdx.x < kKnnMaxCells && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
 // First sorted position of each cell, and its first reference slot: every
@@ -778,22 +784,27 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const RefTiles rf, 
     __syncthreads();
     const unsigned long long todo[2] = {~skipm[0], ~skipm[1]};
     auto next_tile = [&](int i) {  // the next visited index of this group after i, or -1
-      for (int k = i + 1; k < 128; ++k)
-        if ((todo[k >> 6] >> (k & 63)) & 1ull) return k;
-      return -1;
+      const int k = i + 1;
+      if (k >= 128) return -1;
+      if (k < 64) {
+        const unsigned long long m = todo[0] & (~0ull << k);
+        if (m) return __builtin_ctzll(m);
+      }
+      const unsigned long long m = todo[1] & (k <= 64 ? ~0ull : ~0ull << (k - 64));
+      return m ? 64 + __builtin_ctzll(m) : -1;
     };
     int i = next_tile(-1);
     if (i >= 0) tile_load<128, NST>(rf.Xr, rf.Xr, ntiles * kFiltR, visit(g0 + i) * kFiltR, buf, rnv);
     while (i >= 0) {
       const int r0 = visit(g0 + i) * kFiltR;
       const int inext = next_tile(i);
-      int rr[4];  // ids of this lane's reference columns 16 t + lane % 16
-#pragma unroll
-      for (int t = 0; t < 4; ++t) rr[t] = rf.rid[r0 + 16 * t + (lane & 15)];
       __syncthreads();  // the previous tile's merge is done
       tile_store_shift<NST>(rt, rnp, buf, dl);
       __syncthreads();
       if (inext >= 0) tile_load<128, NST>(rf.Xr, rf.Xr, ntiles * kFiltR, visit(g0 + inext) * kFiltR, buf, rnv);
+      int rr[4];  // ids of this lane's reference columns 16 t + lane % 16 (in flight over the dot products)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) rr[t] = rf.rid[r0 + 16 * t + (lane & 15)];
       fv4 acc[4];
       tile_dots<NST>(areg, rt, lane, acc);
       // this lane's four query rows 16 w + 4 g + v: norms and current worst keys
@@ -1400,7 +1411,7 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
     SGT_HIP(hipMallocAsync(&tmp, tb, s));
     SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key0, key1, kidx, qorder, n, 0, 30, s));
     SGT_HIP(hipMemsetAsync(ccnt, 0, sizeof(int) * kKnnMaxCells, s));
-    knn_cell_count_kernel<<<div_up(n, 256), 256, 0, s>>>(cell, n, ccnt);
+    knn_cell_count_kernel<<<std::min(div_up(n, 256), 1024), 256, 0, s>>>(cell, n, ccnt);
     SGT_LAUNCH_CHECK();
     knn_cell_offsets_kernel<<<1, 64, 0, s>>>(ccnt, P, ccnt + kKnnMaxCells, ccnt + 2 * kKnnMaxCells);
     SGT_LAUNCH_CHECK();
