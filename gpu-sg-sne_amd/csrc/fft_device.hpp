@@ -219,9 +219,103 @@ __device__ __forceinline__ void stockham_stage(const float2 *x, float2 *y, int L
   }
 }
 
+// Radix of stage s of a length-L transform in the order base_args uses
+// (radix 4 first, then 2, 3, 5); 0 past the last stage.
+constexpr int fixed_radix(int L, int s) {
+  int m = L, st = 0;
+  for (int r : {4, 2, 3, 5})
+    while (m % r == 0) {
+      if (st == s) return r;
+      m /= r;
+      ++st;
+    }
+  return 0;
+}
+constexpr int fixed_ns(int L, int s) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= fixed_radix(L, i);
+  return ns;
+}
+
+// stockham_stage for a length known at compile time: the same butterflies,
+// twiddles and order of operations (so the same bits), with the index
+// divisions by constants and the element offsets of a butterfly immediate.
+template <int L, int R, int Ns>
+__device__ __forceinline__ void stockham_stage_fixed(const float2 *x, float2 *y, int Lp, int nl,
+                                                     const float2 *__restrict__ tw, bool inv) {
+  constexpr int LR = L / R;
+  constexpr int stride = L / (Ns * R);
+  constexpr int G = LR >= kFftThreads ? 1 : kFftThreads / LR;
+  const float sg = inv ? 1.0f : -1.0f;
+  // the thread index through an empty asm statement: the per-thread index
+  // work below then stays inside its stage instead of being hoisted out of
+  // the callers' loops, where the stages of every length would all hold
+  // their registers at once
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+#pragma unroll 1
+  for (int s = tid; s < G * LR; s += kFftThreads) {
+    const int g = s / LR;
+    const int j = s - g * LR;
+    const int q = j / Ns;
+    const int k = j - q * Ns;
+    float2 w[R];
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+      w[t] = tw[t * k * stride];
+      if (inv) w[t].y = -w[t].y;
+    }
+    const int base = q * Ns * R + k;
+#pragma unroll 1
+    for (int l = g; l < nl; l += G) {
+      const float2 *src = x + l * Lp + j;
+      float2 *dst = y + l * Lp + base;
+      float2 v[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[t] = src[t * LR];
+#pragma unroll
+      for (int t = 1; t < R; ++t) v[t] = cmul(v[t], w[t]);
+      butterfly<R>(v, sg);
+#pragma unroll
+      for (int t = 0; t < R; ++t) dst[t * Ns] = v[t];
+    }
+  }
+}
+
+template <int L, int S = 0>
+__device__ __forceinline__ float2 *stockham_fixed(float2 *x, float2 *y, int Lp, int nl, bool inv,
+                                                  const float2 *tw) {
+  constexpr int R = fixed_radix(L, S);
+  if constexpr (R == 0) {
+    return x;
+  } else {
+    stockham_stage_fixed<L, R, fixed_ns(L, S)>(x, y, Lp, nl, tw, inv);
+    __syncthreads();
+    return stockham_fixed<L, S + 1>(y, x, Lp, nl, inv, tw);
+  }
+}
+
 // Stockham autosort on nl lines (natural order in and out) with the
-// twiddle table tw (LDS).  Returns the buffer holding the result.
-__device__ inline float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int nl, bool inv, const float2 *tw) {
+// twiddle table tw (LDS).  Returns the buffer holding the result.  The
+// lengths of grid sides 16-72 (the batched engine's 3-D grids and the
+// smaller 2-D ones) run as compile-time-length code, the others generically.
+__device__ __forceinline__ float2 *stockham(float2 *x, float2 *y, const PassArgs &a, int nl, bool inv,
+                                           const float2 *tw) {
+  switch (a.L) {
+    case 32: return stockham_fixed<32>(x, y, a.Lp, nl, inv, tw);
+    case 60: return stockham_fixed<60>(x, y, a.Lp, nl, inv, tw);
+    case 64: return stockham_fixed<64>(x, y, a.Lp, nl, inv, tw);
+    case 72: return stockham_fixed<72>(x, y, a.Lp, nl, inv, tw);
+    case 80: return stockham_fixed<80>(x, y, a.Lp, nl, inv, tw);
+    case 90: return stockham_fixed<90>(x, y, a.Lp, nl, inv, tw);
+    case 96: return stockham_fixed<96>(x, y, a.Lp, nl, inv, tw);
+    case 100: return stockham_fixed<100>(x, y, a.Lp, nl, inv, tw);
+    case 108: return stockham_fixed<108>(x, y, a.Lp, nl, inv, tw);
+    case 120: return stockham_fixed<120>(x, y, a.Lp, nl, inv, tw);
+    case 128: return stockham_fixed<128>(x, y, a.Lp, nl, inv, tw);
+    case 144: return stockham_fixed<144>(x, y, a.Lp, nl, inv, tw);
+    default: break;
+  }
   int Ns = 1;
   for (int s = 0; s < a.nst; ++s) {
     const int R = a.rad[s];
