@@ -116,6 +116,8 @@ struct PassArgs {
   int kdim;       // spectrum index form for MID_CONV
   int kpow;       // IN_KGEN: 0 -> (K2, K1) pair, 2 -> K2 only
   int in_c[4][2], out_c[4][2], kcplx[4];  // per-pair channels (-1: none) and spectrum kind
+  int npairs;     // channel pairs of the convolution
+  int pairloop;   // a 3-D multiply pass: each work-group runs every pair of its lines
   const void *in;
   void *out;
   const float2 *Kh;
@@ -441,74 +443,78 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
     const bool lv = act && line < a.nlines;
     const int outer = lv ? fdiv(line, a.ninner, invni) : 0;
     const int inner = line - outer * a.ninner;
+    // the 3-D multiply pass runs every channel pair of its lines, so the
+    // kernel spectrum of those lines is read once for all of them (in 2-D
+    // one pair per work-group keeps more work-groups for the few lines)
+    const bool loop = MID == MID_CONV && a.pairloop;
+    const int pa = loop ? 0 : p, pb = loop ? a.npairs : p + 1;
     float2 kreg[MID == MID_CONV ? kLineElems : 1];
-    if (act) {
-      float2 *dl = b0 + l * Lp;
-      if (IN == IN_W) {
-        const float2 *src = static_cast<const float2 *>(a.in) + (int64_t)p * a.pair_in + outer * a.i_os +
-                            inner * a.i_is + g * a.i_es;
-        const int64_t step = (int64_t)G * a.i_es;
-        const int ne = lv ? a.n_in : 0;
-        for (int e = g; e < L; e += G, src += step) dl[e] = e < ne ? *src : make_float2(0.f, 0.f);
-      } else {  // IN_KT: (K2, K1) half-spectra, mirrored along the pass axis
-        const float2 *src = static_cast<const float2 *>(a.in) + outer * a.i_os + inner * a.i_is;
-        const int es = (int)a.i_es;
-        const int ne = lv ? a.n_in : 0;
-        for (int e = g; e < L; e += G) dl[e] = e < ne ? src[mirror(e, L) * es] : make_float2(0.f, 0.f);
+    if (MID == MID_CONV && act) {
+      // kernel spectrum at (mirror(e), mirror(outer), mirror(inner)):
+      // index mirror(e) * ks + kb
+      int ks = 1, kb = 0;
+      if (a.kdim == 2) {
+        ks = a.H;
+        kb = mirror(inner, L);
+      } else if (a.kdim == 3) {
+        ks = a.H * a.H;
+        kb = mirror(outer, L) * a.H + mirror(inner, L);
       }
-      if (MID == MID_CONV) {
-        // kernel spectrum at (mirror(e), mirror(outer), mirror(inner)):
-        // index mirror(e) * ks + kb
-        int ks = 1, kb = 0;
-        if (a.kdim == 2) {
-          ks = a.H;
-          kb = mirror(inner, L);
-        } else if (a.kdim == 3) {
-          ks = a.H * a.H;
-          kb = mirror(outer, L) * a.H + mirror(inner, L);
-        }
-        const bool cplx = a.kcplx[p] != 0;
 #pragma unroll
-        for (int k = 0; k < kLineElems; ++k) {
-          const int e = g + k * G;
-          kreg[k] = make_float2(0.f, 0.f);
-          if (lv && e < L) {
-            const float2 kv = a.Kh[mirror(e, L) * ks + kb];
-            kreg[k] = make_float2(kv.x, cplx ? kv.y : 0.0f);
-          }
-        }
+      for (int k = 0; k < kLineElems; ++k) {
+        const int e = g + k * G;
+        kreg[k] = lv && e < L ? a.Kh[mirror(e, L) * ks + kb] : make_float2(0.f, 0.f);
       }
     }
-    __syncthreads();
-
-    float2 *r = stockham(b0, b1, a, nl, MID == MID_INV, twl);
-    if (MID == MID_CONV) {
-      // multiply by the kernel spectrum, then inverse transform
+    for (int pp = pa; pp < pb; ++pp) {
       if (act) {
-        float2 *rl = r + l * Lp;
-#pragma unroll
-        for (int k = 0; k < kLineElems; ++k) {
-          const int e = g + k * G;
-          if (e < L) {
-            const float kv = kreg[k].x, kvi = kreg[k].y;
-            const float2 v = rl[e];
-            rl[e] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
-          }
+        float2 *dl = b0 + l * Lp;
+        if (IN == IN_W) {
+          const float2 *src = static_cast<const float2 *>(a.in) + (int64_t)pp * a.pair_in + outer * a.i_os +
+                              inner * a.i_is + g * a.i_es;
+          const int64_t step = (int64_t)G * a.i_es;
+          const int ne = lv ? a.n_in : 0;
+          for (int e = g; e < L; e += G, src += step) dl[e] = e < ne ? *src : make_float2(0.f, 0.f);
+        } else {  // IN_KT: (K2, K1) half-spectra, mirrored along the pass axis
+          const float2 *src = static_cast<const float2 *>(a.in) + outer * a.i_os + inner * a.i_is;
+          const int es = (int)a.i_es;
+          const int ne = lv ? a.n_in : 0;
+          for (int e = g; e < L; e += G) dl[e] = e < ne ? src[mirror(e, L) * es] : make_float2(0.f, 0.f);
         }
       }
       __syncthreads();
-      float2 *other = (r == b0) ? b1 : b0;
-      r = stockham(r, other, a, nl, true, twl);
-    }
 
-    if (lv) {
-      const float2 *rl = r + l * Lp;
-      float2 *dst = static_cast<float2 *>(a.out) + (OUT == OUT_W ? (int64_t)p * a.pair_out : 0) + outer * a.o_os +
-                    inner * a.o_is + g * a.o_es;
-      const int64_t step = (int64_t)G * a.o_es;
-      for (int e = g; e < a.n_out; e += G, dst += step) *dst = rl[e];
+      float2 *r = stockham(b0, b1, a, nl, MID == MID_INV, twl);
+      if (MID == MID_CONV) {
+        // multiply by the kernel spectrum (real, or K2^ + i K1^ for the
+        // pair that carries the charge alone), then inverse transform
+        if (act) {
+          float2 *rl = r + l * Lp;
+          const bool cplx = a.kcplx[pp] != 0;
+#pragma unroll
+          for (int k = 0; k < kLineElems; ++k) {
+            const int e = g + k * G;
+            if (e < L) {
+              const float kv = kreg[k].x, kvi = cplx ? kreg[k].y : 0.0f;
+              const float2 v = rl[e];
+              rl[e] = make_float2(v.x * kv - v.y * kvi, v.x * kvi + v.y * kv);
+            }
+          }
+        }
+        __syncthreads();
+        float2 *other = (r == b0) ? b1 : b0;
+        r = stockham(r, other, a, nl, true, twl);
+      }
+
+      if (lv) {
+        const float2 *rl = r + l * Lp;
+        float2 *dst = static_cast<float2 *>(a.out) + (OUT == OUT_W ? (int64_t)pp * a.pair_out : 0) +
+                      outer * a.o_os + inner * a.o_is + g * a.o_es;
+        const int64_t step = (int64_t)G * a.o_es;
+        for (int e = g; e < a.n_out; e += G, dst += step) *dst = rl[e];
+      }
+      __syncthreads();  // LDS is reused by the next pair or the caller's next share
     }
-    __syncthreads();  // LDS is reused by the caller's next share
     return;
   } else {
   const int total = nl * L;
@@ -818,7 +824,9 @@ __host__ __device__ inline int conv_pass(const GridParams &g, bool zdirect, cons
   int kinds[kMaxPasses];
   pass_kinds(g.d, kinds);
   set_lines(a, pass_strided(kinds[i]));
-  return pairs;
+  a.npairs = make_plan(g.d, zdirect).pairs;
+  a.pairloop = kinds[i] == PK_W_W_CONV && g.d == 3;
+  return a.pairloop ? 1 : pairs;  // its work-groups run every pair
 }
 
 // Arguments of every pass of one convolution (in the order of pass_kinds),

@@ -775,7 +775,7 @@ int launch_bconv(hipStream_t s, int d, int zdirect, const void *args_v, const in
   const int np = pass_kinds(d, kinds);
   const int pairs = make_plan(d, zdirect != 0).pairs;
   for (int p = 0; p < np; ++p) {
-    const int pr = (kinds[p] == PK_KGEN || kinds[p] == PK_KT_T) ? 1 : pairs;
+    const int pr = (kinds[p] == PK_KGEN || kinds[p] == PK_KT_T || (kinds[p] == PK_W_W_CONV && d == 3)) ? 1 : pairs;
     switch (kinds[p]) {
       case PK_KGEN: SGT_TRY((bpass<IN_KGEN, OUT_KT, MID_FWD, true>(s, args, prefix, njobs, np, p, pr, blocks))); break;
       case PK_KT_T: SGT_TRY((bpass<IN_KT, OUT_KT, MID_FWD, false>(s, args, prefix, njobs, np, p, pr, blocks))); break;
