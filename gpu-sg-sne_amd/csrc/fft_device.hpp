@@ -335,42 +335,38 @@ __device__ __forceinline__ float2 *stockham(float2 *x, float2 *y, const PassArgs
 }
 
 // Visits every element (l, e) of a work-group's nl lines of length L once,
-// calling f(l, e, line, outer, inner) with line = line0 + l split as
-// outer * ninner + inner.  ECONTIG: adjacent threads take adjacent e (the
-// pass axis is contiguous in memory) and a thread walks its lines, stepping
-// (outer, inner) by addition; otherwise adjacent threads take adjacent lines
-// and a thread walks e.  The split is computed once per thread, not per element.
-template <bool ECONTIG, class F>
-__device__ __forceinline__ void for_elements(int nl, int L, int line0, int ninner, float invni, F &&f) {
-  if (ECONTIG) {
-    const int G = L >= kFftThreads ? 1 : kFftThreads / L;
-    const float invL = 1.0f / (float)L;
-    for (int s = threadIdx.x; s < G * L; s += kFftThreads) {
-      const int g = fdiv(s, L, invL);
-      const int e = s - g * L;
-      int line = line0 + g;
-      int outer = fdiv(line, ninner, invni);
-      int inner = line - outer * ninner;
-      for (int l = g; l < nl; l += G) {
-        f(l, e, line, outer, inner);
-        line += G;
-        inner += G;
-        while (inner >= ninner) {
-          inner -= ninner;
-          ++outer;
-        }
+// for a pass along the contiguous axis: adjacent threads take adjacent e and
+// a thread walks its lines.  Calls f(l, e, line, outer, inner, oi, oo) with
+// line = line0 + l split as outer * ninner + inner and the element offsets
+// oi = outer i_os + inner i_is + e i_es (oo likewise with the o_ strides),
+// all stepped by addition from one split per thread.
+template <class F>
+__device__ __forceinline__ void for_elements(const PassArgs &a, int nl, int L, int line0, float invni, F &&f) {
+  const int G = L >= kFftThreads ? 1 : kFftThreads / L;
+  const float invL = 1.0f / (float)L;
+  const int ninner = a.ninner;
+  const int64_t si = (int64_t)G * a.i_is, so = (int64_t)G * a.o_is;        // G lines on
+  const int64_t wi = a.i_os - (int64_t)ninner * a.i_is, wo = a.o_os - (int64_t)ninner * a.o_is;  // wrap
+  for (int s = threadIdx.x; s < G * L; s += kFftThreads) {
+    const int g = fdiv(s, L, invL);
+    const int e = s - g * L;
+    int line = line0 + g;
+    int outer = fdiv(line, ninner, invni);
+    int inner = line - outer * ninner;
+    int64_t oi = outer * a.i_os + inner * a.i_is + e * a.i_es;
+    int64_t oo = outer * a.o_os + inner * a.o_is + e * a.o_es;
+    for (int l = g; l < nl; l += G) {
+      f(l, e, line, outer, inner, oi, oo);
+      line += G;
+      inner += G;
+      oi += si;
+      oo += so;
+      while (inner >= ninner) {
+        inner -= ninner;
+        ++outer;
+        oi += wi;
+        oo += wo;
       }
-    }
-  } else {
-    const int G = kFftThreads / nl;
-    const float invnl = 1.0f / (float)nl;
-    for (int s = threadIdx.x; s < G * nl; s += kFftThreads) {
-      const int g = fdiv(s, nl, invnl);
-      const int l = s - g * nl;
-      const int line = line0 + l;
-      const int outer = fdiv(line, ninner, invni);
-      const int inner = line - outer * ninner;
-      for (int e = g; e < L; e += G) f(l, e, line, outer, inner);
     }
   }
 }
@@ -545,17 +541,15 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
   }
 
   // ---- load (zero padding beyond n_in) ----
-  for_elements<ECONTIG>(nl, L, line0, a.ninner, invni, [&](int l, int e, int line, int outer, int inner) {
+  for_elements(a, nl, L, line0, invni, [&](int l, int e, int line, int outer, int inner, int64_t off, int64_t) {
     float2 v = make_float2(0.f, 0.f);
     if (line < a.nlines && e < a.n_in) {
       if (IN == IN_VPAIR) {
         const float *V = static_cast<const float *>(a.in);
-        const int64_t off = outer * a.i_os + inner * a.i_is + e * a.i_es;
         v.x = V[(int64_t)a.in_c[p][0] * a.pair_in + off];
         if (a.in_c[p][1] >= 0) v.y = V[(int64_t)a.in_c[p][1] * a.pair_in + off];
       } else if (IN == IN_VFIX) {
         long long *V = static_cast<long long *>(const_cast<void *>(a.in));
-        const int64_t off = outer * a.i_os + inner * a.i_is + e * a.i_es;
         long long *p0 = V + (int64_t)a.in_c[p][0] * a.pair_in + off;
         v.x = (float)((double)*p0 * a.fix_inv);
         *p0 = 0;
@@ -566,7 +560,7 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
         }
       } else if (IN == IN_W) {
         const float2 *W = static_cast<const float2 *>(a.in);
-        v = W[(int64_t)p * a.pair_in + outer * a.i_os + inner * a.i_is + e * a.i_es];
+        v = W[(int64_t)p * a.pair_in + off];
       } else if (IN == IN_KGEN) {
         const int me = mirror(e, L);
         const float r2 = (float)me * me + (float)inner * inner + (float)outer * outer;
@@ -602,9 +596,8 @@ __device__ inline void pass_body(const PassArgs &a, int blk, int p, float2 *lds)
   }
 
   // ---- store ----
-  for_elements<ECONTIG>(nl, L, line0, a.ninner, invni, [&](int l, int e, int line, int outer, int inner) {
+  for_elements(a, nl, L, line0, invni, [&](int l, int e, int line, int, int, int64_t, int64_t off) {
     if (line >= a.nlines || e >= a.n_out) return;
-    const int64_t off = outer * a.o_os + inner * a.o_is + e * a.o_es;
     const float2 v = r[l * Lp + e];
     if (OUT == OUT_W) {
       float2 *W = static_cast<float2 *>(a.out);
