@@ -16,6 +16,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "grid_device.hpp"
+#include "spread_fix.hpp"
 #include "../../include/sgtsne.h"
 
 namespace sgt {
@@ -860,6 +861,32 @@ int finalize(hipStream_t s, int mode, int nblocks, int d, int n, int apply_mean,
   return SGTSNE_OK;
 }
 
+// Large grids (those no LDS copy or matrix-core tile holds): the points in
+// runs of `unit` consecutive ones (the engine's locality order), each run's
+// stencil window accumulated in LDS as 64-bit fixed point and added to the
+// fixed-point grid Vf (spread_fix.hpp), so the grid is exact and the same
+// whatever the order of the atomics.
+template <int D>
+__global__ __launch_bounds__(s2g_threads<D>()) void s2g_fix_kernel(GridParams g, float fsf, int n,
+                                                                   const float *__restrict__ Y,
+                                                                   unsigned long long *__restrict__ Vf, int unit,
+                                                                   int lds_slots) {
+  extern __shared__ unsigned long long win[];
+  __shared__ int sbox[6];
+  const int nunits = (n + unit - 1) / unit;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x)
+    spread_fix_unit<D>(g, fsf, Vf, u * unit, min(n, (u + 1) * unit), Y, win, lds_slots, sbox);
+}
+
+// V = Vf * 2^-e in FP32 (the FP64 product rounded once), Vf cleared.
+__global__ void fix_to_float_kernel(long long *__restrict__ Vf, float *__restrict__ V, int64_t count, double inv) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    V[i] = (float)((double)Vf[i] * inv);
+    Vf[i] = 0;
+  }
+}
+
 }  // namespace
 
 int launch_stream_copy(hipStream_t s, const float4 *in, float4 *out, int64_t count) {
@@ -943,6 +970,32 @@ int launch_s2g(hipStream_t s, const GridParams &g, int n, const float *Y, float 
   size_t lds = 0;
   int copies = 1;
   const int64_t cells = (int64_t)(g.d == 1 ? g.ng : g.d == 2 ? g.ng * g.ng : g.ng * g.ng * g.ng) * (g.d + 1);
+  if (n > 0 && g.d >= 2 && mode != SGTSNE_SPREAD_ATOMIC && !s2g_lds_fits(g.d, g.ng, &lds, &copies)) {
+    // fixed point in the scratch (2 floats per 64-bit cell): cleared, spread,
+    // converted into V and cleared again
+    if (needed) *needed = 2 * cells;
+    if (query) return SGTSNE_OK;
+    SGT_CHECK(2 * cells <= scratch_floats, SGTSNE_ERR_STATE, "spread scratch too small");
+    // no cell can exceed 2^61 in magnitude: |weight| <= 1 and the charges
+    // are within half the extent of the box centre
+    const double maxy = (double)(g.nGrid - 1) / (double)g.scale * (1.0 + 1e-6);
+    const double bound = 2.0 * (double)n * (0.5 * maxy + 1.0);
+    const int e = std::min(60, std::max(1, 61 - std::ilogb(bound)));
+    unsigned long long *Vf = reinterpret_cast<unsigned long long *>(scratch);
+    SGT_TRY(launch_fill(s, scratch, 2 * cells, 0.0f));
+    const int unit = g.d == 3 ? kFixUnit3 : 1024;
+    const int lds_b = 64512;  // two work-groups per CU
+    const int blocks = (int)std::min<int64_t>(div_up(n, unit), 2048);
+    if (g.d == 2)
+      s2g_fix_kernel<2><<<blocks, s2g_threads<2>(), lds_b, s>>>(g, std::ldexp(1.0f, e), n, Y, Vf, unit, lds_b / 8);
+    else
+      s2g_fix_kernel<3><<<blocks, s2g_threads<3>(), lds_b, s>>>(g, std::ldexp(1.0f, e), n, Y, Vf, unit, lds_b / 8);
+    SGT_LAUNCH_CHECK();
+    fix_to_float_kernel<<<(int)std::min<int64_t>(div_up(cells, 256), 4096), 256, 0, s>>>(
+        reinterpret_cast<long long *>(Vf), V, cells, std::ldexp(1.0, -e));
+    SGT_LAUNCH_CHECK();
+    return SGTSNE_OK;
+  }
   int blocks = 0, chunk = 0;
   if (n > 0 && s2g_lds_fits(g.d, g.ng, &lds, &copies)) {
     // about 64 points per wave, at most 1024 private grids and 64 MiB of them

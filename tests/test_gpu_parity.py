@@ -335,9 +335,11 @@ def test_terms_spread_modes_3d(sg, O, spread):
 @pytest.mark.parametrize("spread", ["AUTO", "ATOMIC", "MFMA"])
 def test_terms_spread_modes(sg, O, spread):
     # every spread path against the oracle on 2-D grids of side 16, 20-32
-    # (the matrix-core form: one and two 16-node tiles per axis) and larger
-    # (where the matrix-core request falls back); the start of a run crowds
-    # every point into a few cells, the worst case for atomics
+    # (the matrix-core form: one and two 16-node tiles per axis), larger
+    # (where the matrix-core request falls back to an LDS copy of the grid)
+    # and beyond an LDS copy (fixed-point windows, or atomics when asked);
+    # the start of a run crowds every point into a few cells, the worst case
+    # for atomics
     n = 6001
     X = _mixture(n, 20, 5, 77)
     I, D = O.knn_exact(X, 30)
@@ -347,7 +349,8 @@ def test_terms_spread_modes(sg, O, spread):
     cases = [(rng.standard_normal((n, 2)) * 1e-4).astype(np.float32),   # ng 16
              (rng.standard_normal((n, 2)) * 2.2).astype(np.float32),    # ng 20-32
              _clustered_embedding(n, 2, rng, spread=4.5),               # ng 20-32
-             _clustered_embedding(n, 2, rng)]                           # ng > 32
+             _clustered_embedding(n, 2, rng),                           # ng > 32
+             _clustered_embedding(n, 2, rng, spread=80.0)]              # too large for an LDS copy
     sides = []
     with sg.Context(sg.tsneparams(spread=getattr(sg, "SPREAD_" + spread))) as ctx:
         ctx.upload_P(S, symmetrize=False)
@@ -359,7 +362,7 @@ def test_terms_spread_modes(sg, O, spread):
             assert ng == ngo
             assert rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL, (spread, ng)
             sides.append(ng)
-    assert sides[0] == 16 and 16 < sides[1] <= 32 and 16 < sides[2] <= 32 and sides[3] > 32, sides
+    assert sides[0] == 16 and 16 < sides[1] <= 32 and 16 < sides[2] <= 32 and sides[3] > 32 and sides[4] > 74, sides
 
 
 @pytest.mark.parametrize("d", [1, 2, 3])
