@@ -297,6 +297,24 @@ __global__ __launch_bounds__(kBlock) void chunk_hist_kernel(int n, int rows_per_
 // Per row range, the nslots chunks with the most nonzeros (ties: lower chunk
 // first; chunks without nonzeros are not taken, their slots get -1).  One
 // work-group per range; the range's counts are consumed in place.
+// The same counts with one work-group per row range accumulating in LDS,
+// then writing every chunk's count (zeros included).
+__global__ __launch_bounds__(kBlock) void chunk_hist_lds_kernel(int n, int rows_per_block, int chs, int nchunks,
+                                                                const int32_t *__restrict__ rowptr,
+                                                                const int32_t *__restrict__ col,
+                                                                int32_t *__restrict__ counts) {
+  extern __shared__ int32_t hist[];
+  for (int i = threadIdx.x; i < nchunks; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const int r0 = blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int row = r0 + w; row < r1; row += nw)
+    for (int t = rowptr[row] + lane; t < rowptr[row + 1]; t += 64) atomicAdd(&hist[col[t] >> chs], 1);
+  __syncthreads();
+  int32_t *h = counts + (int64_t)blockIdx.x * nchunks;
+  for (int i = threadIdx.x; i < nchunks; i += blockDim.x) h[i] = hist[i];
+}
+
 __global__ __launch_bounds__(kBlock) void chunk_select_kernel(int nranges, int nchunks, int nslots,
                                                               int32_t *__restrict__ counts,
                                                               int32_t *__restrict__ slots) {
@@ -748,9 +766,15 @@ int launch_chunk_cache(hipStream_t s, int n, int rows_per_block, int chs, int ns
   if (n <= 0) return SGTSNE_OK;
   const int nranges = div_up(n, rows_per_block);
   const int nchunks = div_up(n, 1 << chs);
-  SGT_HIP(hipMemsetAsync(counts, 0, sizeof(int32_t) * (size_t)nranges * nchunks, s));
-  chunk_hist_kernel<<<(int)std::min<int64_t>(div_up((int64_t)n * 64, kBlock), 65536), kBlock, 0, s>>>(
-      n, rows_per_block, chs, nchunks, rowptr, col, counts);
+  if ((size_t)nchunks * sizeof(int32_t) <= 65536) {
+    // one work-group per row range, its histogram in LDS
+    chunk_hist_lds_kernel<<<nranges, kBlock, (size_t)nchunks * sizeof(int32_t), s>>>(n, rows_per_block, chs, nchunks,
+                                                                                    rowptr, col, counts);
+  } else {
+    SGT_HIP(hipMemsetAsync(counts, 0, sizeof(int32_t) * (size_t)nranges * nchunks, s));
+    chunk_hist_kernel<<<(int)std::min<int64_t>(div_up((int64_t)n * 64, kBlock), 65536), kBlock, 0, s>>>(
+        n, rows_per_block, chs, nchunks, rowptr, col, counts);
+  }
   SGT_LAUNCH_CHECK();
   chunk_select_kernel<<<nranges, kBlock, 0, s>>>(nranges, nchunks, nslots, counts, slots);
   SGT_LAUNCH_CHECK();

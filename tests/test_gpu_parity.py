@@ -365,6 +365,32 @@ def test_terms_spread_modes(sg, O, spread):
     assert sides[0] == 16 and 16 < sides[1] <= 32 and 16 < sides[2] <= 32 and sides[3] > 32 and sides[4] > 74, sides
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_large_grid_spread_is_deterministic(sg, O, d):
+    # grids no LDS copy holds go through fixed-point windows: the terms are
+    # the oracle's and the same bits on every evaluation, whatever order the
+    # work-groups' atomics land in
+    n = 8001
+    X = _mixture(n, 20, 5, 81 + d)
+    I, D = O.knn_exact(X, 30)
+    So = O.symmetrize(O.perplexity_equalize(I, D, 10.0))
+    S = sg.CSR(n, So.rowptr, So.col, So.val.astype(np.float32))
+    rng = np.random.default_rng(90 + d)
+    Y = _clustered_embedding(n, d, rng, spread={2: 80.0, 3: 25.0}[d])
+    h = {2: 0.7, 3: 1.2}[d]
+    with sg.Context(sg.tsneparams(d=d)) as ctx:
+        ctx.upload_P(S, symmetrize=False)
+        ctx.set_Y(Y)
+        runs = [ctx.terms() for _ in range(3)]
+        Yc = ctx.get_Y()
+    _, Fr, Z, ng = runs[0]
+    assert ng > (74 if d == 2 else 16), ng
+    for _, Fr2, Z2, _ in runs[1:]:
+        assert np.array_equal(Fr, Fr2) and Z == Z2
+    Fro, Zo, ngo = O.repulsive_interp(Yc.astype(np.float64), h, O.ZETA_DIRECT)
+    assert ng == ngo and rel_l2(Fr, Fro) < TOL and abs(Z - Zo) / Zo < TOL
+
+
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_terms_reference_acceptance_setup(sg, O, d):
     # the reference's repulsion self-test inputs (HY/test_modules.cpp:122-156):
