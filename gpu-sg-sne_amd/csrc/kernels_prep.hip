@@ -1393,7 +1393,7 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
     // point order: cell (nearest of P pivots), then the Morton code of a 3-D
     // projection of the centred points; the query blocks and the reference
     // tiles both follow it, the tiles padded so each lies in one cell
-    const int P = std::min(kKnnMaxCells, std::max(1, n / 2048));
+    const int P = std::min(kKnnMaxCells, std::max(1, n / 512));
     const int ntiles = div_up(n + (kFiltR - 1) * P, kFiltR);  // at least the padded slots
     SGT_HIP(hipMallocAsync((void **)&cell, sizeof(int32_t) * (size_t)n, s));
     SGT_HIP(hipMallocAsync((void **)&ccnt, sizeof(int) * 3 * kKnnMaxCells, s));

@@ -97,7 +97,7 @@ def test_knn_methods_agree_large(sg):
 
 @pytest.mark.parametrize("data", ["gaussian", "powerlaw"])
 def test_knn_filtered_pruned_clusters(sg, data):
-    # clustered inputs at a size where the filter orders the points by 19
+    # clustered inputs at a size where the filter orders the points by 64
     # cells and skips the tiles of far cells: every row still equals the
     # exact FP64 kernel's (itself pinned to the oracle above)
     from sgtsne_amd import synth
