@@ -59,8 +59,9 @@ extern "C" {
 #define SGTSNE_ZETA_ALGEBRAIC 1
 
 /* Y layouts: SoA [d][N] (CU path) or AoS [N][d] (HYB path) */
-#define SGTSNE_SPREAD_AUTO 0   /* matrix cores where they apply, else atomics   */
-#define SGTSNE_SPREAD_ATOMIC 1 /* LDS-privatised grids, or global atomics       */
+#define SGTSNE_SPREAD_AUTO 0   /* matrix cores where they apply, else LDS-privatised
+                                  grids, else fixed-point LDS windows (exact)    */
+#define SGTSNE_SPREAD_ATOMIC 1 /* LDS-privatised grids, or global float atomics */
 #define SGTSNE_SPREAD_MFMA 2   /* matrix cores (2-D side <= 32, 3-D side <= 16; else AUTO) */
 #define SGTSNE_LAYOUT_SOA 0
 #define SGTSNE_LAYOUT_AOS 1
