@@ -98,7 +98,7 @@ typedef struct sgtsne_params {
   int spread;         /* SGTSNE_SPREAD_*: how points are spread onto the grid
                          (AUTO: the matrix-core form for 2-D grids of side
                          <= 32 and 3-D grids of side <= 16, else
-                         LDS-privatised or atomic)                  (AUTO)  */
+                         LDS-privatised, else fixed-point windows)  (AUTO)  */
 } sgtsne_params;
 
 /* Fills the defaults listed above.  Replaces the in-class defaults of
