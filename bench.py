@@ -196,9 +196,11 @@ def batched_leg(sg, jobs, sample, k=90, u=30.0, graph=True):
     return out
 
 
-def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True, oracle_checks=2):
+def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True, oracle_checks=2, fronts=("async",)):
     """Poisson arrivals (N log-uniform in [1,000, 200,000], d in {2, 3}) through
-    the continuous-batching server, then one at a time through the plain method."""
+    the continuous-batching server, then one at a time through the plain
+    method.  The server runs once per front-end mode in `fronts`; the first
+    is the one reported as `served` (the others as `served_<mode>`)."""
     from sgtsne_amd import serving, synth
 
     arrivals = synth.serving_requests(requests, rate)
@@ -208,12 +210,15 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True, oracle_checks
         return data[seed]
 
     log(f"[bench] served: {requests} requests at {rate}/s, N total {sum(a[1] for a in arrivals)}")
-    with serving.Server(tick=tick, k=k, u=u) as srv:
-        for e in srv.engines.values():
-            e.set_graph(graph)
-        t0 = time.perf_counter()
-        got = srv.run(arrivals, make_X)
-        t_srv = time.perf_counter() - t0
+    runs = {}
+    for mode in fronts:
+        with serving.Server(tick=tick, k=k, u=u, front=mode) as srv:
+            for e in srv.engines.values():
+                e.set_graph(graph)
+            t0 = time.perf_counter()
+            runs[mode] = (srv.run(arrivals, make_X), time.perf_counter() - t0)
+        log(f"[bench] served ({mode} front ends): {serving.latency_stats(runs[mode][0])}")
+    got, t_srv = runs[fronts[0]]
     ok = sum(1 for r in got if r.Y is not None and np.all(np.isfinite(r.Y)))
     s_stats = serving.latency_stats(got)
     t0 = time.perf_counter()
@@ -235,12 +240,18 @@ def served_leg(sg, requests, rate, tick, k=90, u=30.0, graph=True, oracle_checks
         rs, rb = oracle_kl_ratios(P, synth.random_init(n, dd, seed=seed), dd, [got[idx].Y, base[idx].Y])
         vs_s.append(rs)
         vs_b.append(rb)
-    out = {"requests": requests, "rate_per_s": rate, "tick": tick, "completed_finite": ok,
+    out = {"requests": requests, "rate_per_s": rate, "tick": tick, "front_ends": fronts[0], "completed_finite": ok,
            "served": s_stats, "one_at_a_time": b_stats,
            "embeds_per_s_ratio": round(s_stats["embeds_per_s"] / b_stats["embeds_per_s"], 2),
            "kl_over_fp64_oracle": ({"served": _ratio_stats(vs_s), "one_at_a_time": _ratio_stats(vs_b),
                                     "sample": f"the {oracle_checks} smallest requests of each d"} if check else None),
            "wall_s": {"served": round(t_srv, 2), "one_at_a_time": round(t_base, 2)}}
+    for mode in fronts[1:]:
+        other = runs[mode][0]
+        out[f"served_{mode}"] = serving.latency_stats(other)
+        out[f"served_{mode}"]["same_Y_as_served"] = all(
+            (a.Y is None and b.Y is None) or (a.Y is not None and b.Y is not None and np.array_equal(a.Y, b.Y))
+            for a, b in zip(got, other))
     log(f"[bench] served: {out}")
     return out
 
@@ -569,6 +580,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-tick", type=int, default=10)
+    ap.add_argument("--serve-fronts", default="async",
+                    help="comma-separated front-end modes of the server (async, sync); the first is reported")
     ap.add_argument("--configs", default="d3,c2,c1",
                     help="extra configurations besides the headline: d3 (C3 at d=3 from the same P), "
                          "c2 (sparse784 formats), c1 (mixture10_50d)")
@@ -601,7 +614,8 @@ def main():
         log(f"[bench] plain copy: {copy}")
     g = not args.no_graph
     batched = batched_leg(sg, args.batch_jobs, args.batch_sample, graph=g) if "batched" in legs else None
-    served = served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g) if "served" in legs else None
+    served = (served_leg(sg, args.serve_requests, args.serve_rate, args.serve_tick, graph=g,
+                         fronts=tuple(args.serve_fronts.split(","))) if "served" in legs else None)
     knn = knn_leg(sg) if "knn" in legs else None
     c2 = c2_leg(sg, args) if "c2" in args.configs and "main" in legs else None
     c1 = c1_leg(sg, args) if "c1" in args.configs and "main" in legs else None

@@ -64,6 +64,25 @@ struct StageSpan {
 constexpr size_t kStageBytes = 16u << 20;  // pinned staging for host -> device updates
 constexpr int kMarks = 16;                 // tick marks kept
 
+// One admission from points (sgtsne_batch_add_points_async) whose front end
+// is queued in three parts on the front-end stream; sgtsne_batch_add_poll
+// queues the next part once the device has passed the previous one, so no
+// part waits on the host.  The parts end where the host needs a count from
+// the device: the kNN rows left uncertified, then the nonzeros of P + P^T.
+struct FrontEnd {
+  int stage = 0;  // 0 none, 1 kNN queued, 2 P queued up to its merge, 3 P_sym queued
+  int64_t ticket = 0;
+  int n = 0, dim = 0, k = 0, max_iter = 0, ng_cap = 0;
+  double u = 0.0;
+  const float *y_in = nullptr;  // the caller's, read at admission
+  float *Xf = nullptr, *vv = nullptr, *svv = nullptr;
+  double *Xd = nullptr, *Dd = nullptr;
+  int32_t *Id = nullptr, *rp = nullptr, *cc = nullptr, *srp = nullptr, *scc = nullptr;
+  int64_t m = 0;
+  KnnPending knn;
+  SymPending sym;
+};
+
 }  // namespace
 
 struct sgtsne_batch {
@@ -72,12 +91,16 @@ struct sgtsne_batch {
   hipStream_t stream = nullptr;  // iterations and every arena write
   hipStream_t prep = nullptr;    // front end of new jobs (kNN, perplexity, P + P^T)
   // reads of finished jobs: the front-end stream, idle whenever such a read
-  // runs (batch calls come from one thread and a front end returns after its
-  // work is done).  Two streams per batch, not three: a server's two engines
-  // then need four, one per hardware queue (GPU_MAX_HW_QUEUES = 4), and do not
-  // depend on which queues earlier streams of the process left busiest
+  // runs unless an asynchronous front end is in flight (the read then waits
+  // for its queued part).  Two streams per batch, not three: a server's two
+  // engines then need four, one per hardware queue (GPU_MAX_HW_QUEUES = 4),
+  // and do not depend on which queues earlier streams of the process left
+  // busiest
   hipStream_t copy = nullptr;
-  hipEvent_t ev_prep = nullptr;
+  hipEvent_t ev_prep = nullptr;  // after the queued part of the front end
+  FrontEnd fe;
+  int64_t fe_tickets = 0;
+  int *fe_host = nullptr;  // pinned: the counts the front end's parts read back
   // host -> device updates go through a pinned ring, so queuing them never
   // waits for the iterations queued ahead of them
   char *stage = nullptr;
@@ -774,8 +797,14 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   return SGTSNE_OK;
 }
 
+void fe_release(sgtsne_batch *b);
+
 void destroy_all(sgtsne_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->fe.stage != 0) {
+    (void)hipStreamSynchronize(b->prep);
+    fe_release(b);
+  }
   release_graph(b);
   release_deferred(b, true);
   for (hipEvent_t &e : b->exec_done)
@@ -832,6 +861,8 @@ void destroy_all(sgtsne_batch *b) {
       e = nullptr;
     }
   if (b->ev_prep) (void)hipEventDestroy(b->ev_prep);
+  if (b->fe_host) (void)hipHostFree(b->fe_host);
+  b->fe_host = nullptr;
   if (b->prep) (void)hipStreamDestroy(b->prep);
   b->copy = nullptr;  // the front-end stream, destroyed above
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -862,6 +893,66 @@ int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int lay
     }
   }
   return SGTSNE_OK;
+}
+
+// Releases a front end's device temporaries (stream-ordered on the
+// front-end stream, after whatever of it is queued) and clears it.
+void fe_release(sgtsne_batch *b) {
+  FrontEnd &f = b->fe;
+  const hipStream_t s = b->prep;
+  knn_release(s, &f.knn);
+  sym_release(s, &f.sym);
+  void *bufs[] = {f.Xf, f.Xd, f.Dd, f.Id, f.rp, f.cc, f.vv, f.srp, f.scc, f.svv};
+  for (void *q : bufs)
+    if (q) (void)hipFreeAsync(q, s);
+  f = FrontEnd{};
+}
+
+// Queues the front end's next part; the device has passed the previous one.
+// Returns 1 once the job is admitted (*job set), 0 when a further part is
+// queued, or an error status (the front end is then released).
+int fe_advance(sgtsne_batch *b, int *job) {
+  FrontEnd &f = b->fe;
+  const hipStream_t s = b->prep;
+  int st = SGTSNE_OK;
+  if (f.stage == 1) {  // kNN rows certified: the uncertified ones, P, and the merge of P and P^T
+    st = knn_end(s, &f.knn, nullptr);
+    if (!st) st = launch_perplexity(s, f.Dd, f.n, f.k, f.u, f.vv, nullptr);
+    if (!st) st = launch_knn_csr(s, f.n, f.k, f.Id, f.rp, f.cc);
+    if (!st && b->p.lambda != 1.0) st = launch_lambda_rescale(s, f.n, f.rp, f.vv, b->p.lambda, b->p.drop_leaf);
+    if (!st) st = sym_begin(s, f.n, (int64_t)f.n * f.k, f.rp, f.cc, f.vv, b->fe_host + 1, &f.sym);
+  } else if (f.stage == 2) {  // merged count known: P_sym's CSR
+    st = sym_end(s, &f.sym, &f.srp, &f.scc, &f.svv, &f.m, true);
+    // released in stream order, after the kernels just queued that read them
+    void *tmp[] = {f.Xf, f.Xd, f.Dd, f.Id, f.rp, f.cc, f.vv};
+    for (void *q : tmp)
+      if (q) (void)hipFreeAsync(q, s);
+    f.Xf = f.vv = nullptr;
+    f.Xd = f.Dd = nullptr;
+    f.Id = f.rp = f.cc = nullptr;
+  } else {  // P_sym built: admission
+    std::lock_guard<std::mutex> lock(b->mu);
+    release_deferred(b);
+    st = admit(b, f.n, f.m, f.srp, f.scc, f.svv, f.y_in, f.max_iter, f.ng_cap, job);
+    if (st) {
+      fe_release(b);
+      return st;
+    }
+    // the admitted P is read by copies queued on the iteration stream
+    st = defer_free(b, f.srp, f.scc, f.svv);
+    f = FrontEnd{};
+    return st ? st : 1;
+  }
+  if (!st && hipEventRecord(b->ev_prep, s) != hipSuccess) {
+    set_error("front end: event record failed");
+    st = SGTSNE_ERR_HIP;
+  }
+  if (st) {
+    fe_release(b);
+    return st;
+  }
+  ++f.stage;
+  return 0;
 }
 
 }  // namespace
@@ -912,6 +1003,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
   for (hipEvent_t &e : b->marks) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
   chk(hipHostMalloc((void **)&b->stage, kStageBytes, hipHostMallocDefault), "pinned staging");
+  chk(hipHostMalloc((void **)&b->fe_host, 256, hipHostMallocDefault), "pinned counts");
   const size_t P = (size_t)max_points;
   if (!st) st = balloc((void **)&b->rowptr, sizeof(int32_t) * (P + 1));
   if (!st) st = balloc((void **)&b->col, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1));
@@ -1012,54 +1104,81 @@ int sgtsne_batch_add(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rowptr,
   return st;
 }
 
-int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int k, double u, const float *y_in,
-                            int max_iter, int ng_cap, int *job) {
+int sgtsne_batch_add_points_async(sgtsne_batch *b, const float *X, int n, int dim, int k, double u,
+                                  const float *y_in, int max_iter, int ng_cap, int64_t *ticket) {
   SGT_TRY(check_batch(b));
   SGT_CHECK(X && n > k && k >= 1 && dim >= 1 && u > 0, SGTSNE_ERR_ARG, "bad kNN arguments");
-  if (max_iter <= 0) max_iter = b->p.max_iter;
+  SGT_CHECK(ticket != nullptr, SGTSNE_ERR_ARG, "ticket is NULL");
+  SGT_CHECK(b->fe.stage == 0, SGTSNE_ERR_STATE, "front end %lld is in flight: poll it to its admission first",
+            (long long)b->fe.ticket);
+  FrontEnd &f = b->fe;
+  f = FrontEnd{};
+  f.n = n;
+  f.dim = dim;
+  f.k = k;
+  f.u = u;
+  f.y_in = y_in;
+  f.max_iter = max_iter > 0 ? max_iter : b->p.max_iter;
+  f.ng_cap = ng_cap;
   const hipStream_t s = b->prep;
   const size_t nx = (size_t)n * dim;
-  float *Xf = nullptr;
-  double *Xd = nullptr, *Dd = nullptr;
-  int32_t *Id = nullptr, *rp = nullptr, *cc = nullptr, *srp = nullptr, *scc = nullptr;
-  float *vv = nullptr, *svv = nullptr;
-  int64_t m = 0;
-  // temporaries come from the stream-ordered pool (see sgtsne_batch_add)
-  SGT_HIP(hipMallocAsync((void **)&Xf, nx * sizeof(float), s));
-  SGT_HIP(hipMallocAsync((void **)&Xd, nx * sizeof(double), s));
-  SGT_HIP(hipMallocAsync((void **)&Dd, (size_t)n * (k + 1) * sizeof(double), s));
-  SGT_HIP(hipMallocAsync((void **)&Id, (size_t)n * (k + 1) * sizeof(int32_t), s));
-  SGT_HIP(hipMallocAsync((void **)&rp, ((size_t)n + 1) * sizeof(int32_t), s));
-  SGT_HIP(hipMallocAsync((void **)&cc, (size_t)n * k * sizeof(int32_t), s));
-  SGT_HIP(hipMallocAsync((void **)&vv, (size_t)n * k * sizeof(float), s));
   int st = SGTSNE_OK;
-  if (hipMemcpyAsync(Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess) {
-    set_error("copy of X failed");
-    st = SGTSNE_ERR_HIP;
-  }
-  if (!st) st = launch_to_double(s, Xf, Xd, (int64_t)nx);
-  if (!st) st = launch_knn_exact(s, Xd, n, dim, k, Id, Dd);
-  if (!st) st = launch_perplexity(s, Dd, n, k, u, vv, nullptr);
-  if (!st) st = launch_knn_csr(s, n, k, Id, rp, cc);
-  if (!st && b->p.lambda != 1.0) st = launch_lambda_rescale(s, n, rp, vv, b->p.lambda, b->p.drop_leaf);
-  if (!st) st = gpu_symmetrize(s, n, (int64_t)n * k, rp, cc, vv, &srp, &scc, &svv, &m, true);
-  if (!st && hipStreamSynchronize(s) != hipSuccess) {
-    set_error("front end failed");
-    st = SGTSNE_ERR_HIP;
-  }
-  void *tmp[] = {Xf, Xd, Dd, Id, rp, cc, vv};
-  for (void *q : tmp) (void)hipFreeAsync(q, s);
-  std::lock_guard<std::mutex> lock(b->mu);
-  release_deferred(b);
-  if (!st) st = admit(b, n, m, srp, scc, svv, y_in, max_iter, ng_cap, job);
-  // the admitted P is read by copies queued on the iteration stream
+  auto chk = [&](hipError_t e, const char *what) {
+    if (st == SGTSNE_OK && e != hipSuccess) {
+      set_error("front end: %s failed: %s", what, hipGetErrorString(e));
+      st = e == hipErrorOutOfMemory ? SGTSNE_ERR_NOMEM : SGTSNE_ERR_HIP;
+    }
+  };
+  // temporaries come from the stream-ordered pool (see sgtsne_batch_add)
+  chk(hipMallocAsync((void **)&f.Xf, nx * sizeof(float), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.Xd, nx * sizeof(double), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.Dd, (size_t)n * (k + 1) * sizeof(double), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.Id, (size_t)n * (k + 1) * sizeof(int32_t), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.rp, ((size_t)n + 1) * sizeof(int32_t), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.cc, (size_t)n * k * sizeof(int32_t), s), "hipMallocAsync");
+  chk(hipMallocAsync((void **)&f.vv, (size_t)n * k * sizeof(float), s), "hipMallocAsync");
+  chk(hipMemcpyAsync(f.Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s), "copy of X");
+  if (!st) st = launch_to_double(s, f.Xf, f.Xd, (int64_t)nx);
+  if (!st) st = knn_begin(s, f.Xd, n, dim, k, f.Id, f.Dd, SGTSNE_KNN_AUTO, b->fe_host, &f.knn);
+  chk(hipEventRecord(b->ev_prep, s), "event record");
   if (st) {
-    if (srp) (void)hipFreeAsync(srp, s);
-    if (scc) (void)hipFreeAsync(scc, s);
-    if (svv) (void)hipFreeAsync(svv, s);
-  } else {
-    SGT_TRY(defer_free(b, srp, scc, svv));
+    fe_release(b);
+    return st;
   }
+  f.stage = 1;
+  f.ticket = ++b->fe_tickets;
+  *ticket = f.ticket;
+  return SGTSNE_OK;
+}
+
+int sgtsne_batch_add_poll(sgtsne_batch *b, int64_t ticket, int wait, int *job) {
+  SGT_TRY(check_batch(b));
+  SGT_CHECK(job != nullptr, SGTSNE_ERR_ARG, "job is NULL");
+  *job = -1;
+  SGT_CHECK(b->fe.stage != 0 && ticket == b->fe.ticket, SGTSNE_ERR_ARG, "no front end %lld in flight",
+            (long long)ticket);
+  for (;;) {
+    const hipError_t e = wait ? hipEventSynchronize(b->ev_prep) : hipEventQuery(b->ev_prep);
+    if (e == hipErrorNotReady) return SGTSNE_OK;
+    if (e != hipSuccess) {
+      set_error("front end failed: %s", hipGetErrorString(e));
+      fe_release(b);
+      return SGTSNE_ERR_HIP;
+    }
+    const int r = fe_advance(b, job);
+    if (r == 1) return SGTSNE_OK;
+    if (r != 0) return r;
+    if (!wait) return SGTSNE_OK;  // the next part is queued; a later poll takes it on
+  }
+}
+
+int sgtsne_batch_add_points(sgtsne_batch *b, const float *X, int n, int dim, int k, double u, const float *y_in,
+                            int max_iter, int ng_cap, int *job) {
+  int64_t ticket = 0;
+  SGT_TRY(sgtsne_batch_add_points_async(b, X, n, dim, k, u, y_in, max_iter, ng_cap, &ticket));
+  int slot = -1;
+  const int st = sgtsne_batch_add_poll(b, ticket, 1, &slot);
+  if (job) *job = slot;
   return st;
 }
 

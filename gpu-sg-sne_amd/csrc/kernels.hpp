@@ -114,6 +114,26 @@ void retain_pool_memory();
 int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D);
 int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
                int *fallback_rows);
+// launch_knn in two halves for callers that must not wait in between:
+// knn_begin queues everything up to the certificate and a copy of the count
+// of uncertified rows into *count_host (pinned memory for a copy that does
+// not block); knn_end, called once the stream has passed that copy, queues
+// the exact recomputation of those rows and releases the temporaries.
+struct KnnPending {
+  bool open = false;  // knn_end still has work to queue
+  const double *X = nullptr;
+  int n = 0, dim = 0, k = 0;
+  int32_t *I = nullptr;
+  double *D = nullptr;
+  int *fb = nullptr;  // device: [0] count, then the rows
+  const int *count_host = nullptr;
+  void *bufs[32] = {};
+  int nbufs = 0;
+};
+int knn_begin(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
+              int *count_host, KnnPending *kp);
+int knn_end(hipStream_t s, KnnPending *kp, int *fallback_rows);
+void knn_release(hipStream_t s, KnnPending *kp);  // frees the temporaries without finishing
 int launch_perplexity(hipStream_t s, const double *D, int n, int k, double u, float *val,
                       double *beta_out);
 int launch_lambda_rescale(hipStream_t s, int n, const int32_t *rowptr, float *val, double lambda,
@@ -123,6 +143,24 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d,
                    const int32_t *col_d, const float *val_d, int32_t **rowptr_out,
                    int32_t **col_out, float **val_out, int64_t *nnz_out,
                    bool stream_ordered = false);
+// gpu_symmetrize in two halves (see KnnPending): sym_begin queues the merge of
+// P and P^T and a copy of the merged nonzero count into *runs_host; sym_end,
+// once the stream has passed that copy, queues the normalisation and the CSR
+// build into stream-ordered outputs, without waiting for them.
+struct SymPending {
+  bool open = false;
+  int n = 0;
+  int64_t m2 = 0;
+  unsigned long long *k0 = nullptr, *k1 = nullptr;
+  double *v0 = nullptr, *v1 = nullptr, *total = nullptr;
+  int *nruns = nullptr;
+  const int *runs_host = nullptr;
+};
+int sym_begin(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
+              const float *val_d, int *runs_host, SymPending *sp);
+int sym_end(hipStream_t s, SymPending *sp, int32_t **rowptr_out, int32_t **col_out, float **val_out,
+            int64_t *nnz_out, bool stream_ordered);
+void sym_release(hipStream_t s, SymPending *sp);
 int launch_build_ell(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,
                      const float *val, int32_t *ecol, float *eval, int32_t *tail_cnt);
 int launch_build_tail(hipStream_t s, int n, int width, const int32_t *rowptr, const int32_t *col,

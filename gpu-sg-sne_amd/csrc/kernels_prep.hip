@@ -1339,14 +1339,14 @@ int knn_exact_rows(hipStream_t s, const double *X, int n, int dim, int k, const 
 
 }  // namespace
 
-int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
-               int *fallback_rows) {
+int knn_begin(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
+              int *count_host, KnnPending *pend) {
+  *pend = KnnPending{};
   SGT_CHECK(n > k && k >= 1 && dim >= 1, SGTSNE_ERR_ARG, "kNN needs n > k >= 1 (n=%d k=%d)", n, k);
   SGT_CHECK(k <= kKnnR * 64, SGTSNE_ERR_ARG, "k too large");
-  if (fallback_rows) *fallback_rows = -1;
   const bool filter_ok = k + kFiltSlack <= kRerank && dim <= kFiltMaxDp && n >= kFiltQ;
   if (method == SGTSNE_KNN_EXACT_FP64 || (method == SGTSNE_KNN_AUTO && !filter_ok))
-    return knn_exact_rows(s, X, n, dim, k, nullptr, n, I, D);
+    return knn_exact_rows(s, X, n, dim, k, nullptr, n, I, D);  // nothing left for knn_end
   SGT_CHECK(filter_ok, SGTSNE_ERR_ARG, "the filtered kNN needs k <= %d, dim <= %d and n >= %d",
             kRerank - kFiltSlack, kFiltMaxDp, kFiltQ);
   const int kp = std::min(k + kFiltSlack, n - 1);
@@ -1441,17 +1441,54 @@ int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I
   knn_rerank_kernel<<<n, kRerank, 0, s>>>(X, n, dim, k, kp, hd, hi, blocked ? qa : nrm, qdn, rmax, cE, ce, I, D,
                                           fb + 1, fb);
   SGT_LAUNCH_CHECK();
-  int nfb = 0;
-  SGT_HIP(hipMemcpyAsync(&nfb, fb, sizeof(int), hipMemcpyDeviceToHost, s));
-  SGT_HIP(hipStreamSynchronize(s));
-  int st = SGTSNE_OK;
-  if (nfb > 0) st = knn_exact_rows(s, X, n, dim, k, fb + 1, nfb, I, D);
-  if (fallback_rows) *fallback_rows = nfb;
+  SGT_HIP(hipMemcpyAsync(count_host, fb, sizeof(int), hipMemcpyDeviceToHost, s));
   void *bufs[] = {mu, nrm, Xc, n32, hd, hi, fb, rmax, qdn, key0, key1, kidx, qorder, cb, delta, dn, Xq, qn32, qa, tmp,
                   cell, ccnt, rid, rinv, Xr, tcen, trad};
+  static_assert(sizeof(bufs) / sizeof(bufs[0]) <= sizeof(pend->bufs) / sizeof(pend->bufs[0]), "KnnPending::bufs");
+  pend->open = true;
+  pend->X = X;
+  pend->n = n;
+  pend->dim = dim;
+  pend->k = k;
+  pend->I = I;
+  pend->D = D;
+  pend->fb = fb;
+  pend->count_host = count_host;
   for (void *p : bufs)
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) pend->bufs[pend->nbufs++] = p;
+  return SGTSNE_OK;
+}
+
+void knn_release(hipStream_t s, KnnPending *kp) {
+  for (int i = 0; i < kp->nbufs; ++i) (void)hipFreeAsync(kp->bufs[i], s);
+  *kp = KnnPending{};
+}
+
+int knn_end(hipStream_t s, KnnPending *kp, int *fallback_rows) {
+  if (!kp->open) {  // exact rows throughout
+    if (fallback_rows) *fallback_rows = -1;
+    return SGTSNE_OK;
+  }
+  const int nfb = *kp->count_host;
+  int st = SGTSNE_OK;
+  if (nfb > 0) st = knn_exact_rows(s, kp->X, kp->n, kp->dim, kp->k, kp->fb + 1, nfb, kp->I, kp->D);
+  if (fallback_rows) *fallback_rows = nfb;
+  knn_release(s, kp);
   return st;
+}
+
+int launch_knn(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D, int method,
+               int *fallback_rows) {
+  if (fallback_rows) *fallback_rows = -1;
+  int nfb = 0;
+  KnnPending kp;
+  SGT_TRY(knn_begin(s, X, n, dim, k, I, D, method, &nfb, &kp));
+  if (kp.open && hipStreamSynchronize(s) != hipSuccess) {
+    knn_release(s, &kp);
+    set_error("kNN failed");
+    return SGTSNE_ERR_HIP;
+  }
+  return knn_end(s, &kp, fallback_rows);
 }
 
 int launch_knn_exact(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I, double *D) {
@@ -1507,46 +1544,63 @@ int device_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int n)
   return SGTSNE_OK;
 }
 
-int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
-                   const float *val_d, int32_t **rowptr_out, int32_t **col_out, float **val_out,
-                   int64_t *nnz_out, bool stream_ordered) {
+int sym_begin(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
+              const float *val_d, int *runs_host, SymPending *sp) {
+  *sp = SymPending{};
   const int64_t m2 = 2 * nnz;
   SGT_CHECK(m2 < (int64_t)INT_MAX, SGTSNE_ERR_ARG, "2*nnz exceeds the int32 range");
-  unsigned long long *k0 = nullptr, *k1 = nullptr, *uk = nullptr;
-  double *v0 = nullptr, *v1 = nullptr, *uv = nullptr, *total = nullptr;
-  int *nruns = nullptr;
-  SGT_HIP(hipMallocAsync((void **)&k0, sizeof(unsigned long long) * m2, s));
-  SGT_HIP(hipMallocAsync((void **)&k1, sizeof(unsigned long long) * m2, s));
-  SGT_HIP(hipMallocAsync((void **)&v0, sizeof(double) * m2, s));
-  SGT_HIP(hipMallocAsync((void **)&v1, sizeof(double) * m2, s));
-  SGT_HIP(hipMallocAsync((void **)&nruns, sizeof(int), s));
-  SGT_HIP(hipMallocAsync((void **)&total, sizeof(double), s));
-  sym_keys_kernel<<<div_up(n, 256), 256, 0, s>>>(n, rowptr_d, col_d, val_d, nnz, k0, v0);
+  sp->n = n;
+  sp->m2 = m2;
+  sp->runs_host = runs_host;
+  SGT_HIP(hipMallocAsync((void **)&sp->k0, sizeof(unsigned long long) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&sp->k1, sizeof(unsigned long long) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&sp->v0, sizeof(double) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&sp->v1, sizeof(double) * m2, s));
+  SGT_HIP(hipMallocAsync((void **)&sp->nruns, sizeof(int), s));
+  SGT_HIP(hipMallocAsync((void **)&sp->total, sizeof(double), s));
+  sp->open = true;
+  sym_keys_kernel<<<div_up(n, 256), 256, 0, s>>>(n, rowptr_d, col_d, val_d, nnz, sp->k0, sp->v0);
   SGT_LAUNCH_CHECK();
   int nbits = 1;
   while ((1ll << nbits) < (long long)n) ++nbits;
   const int end_bit = 32 + nbits;
   size_t tb = 0;
-  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)m2, 0, end_bit, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, sp->k0, sp->k1, sp->v0, sp->v1, (int)m2, 0, end_bit, s));
   void *tmp = nullptr;
   SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
-  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)m2, 0, end_bit, s));
+  SGT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, sp->k0, sp->k1, sp->v0, sp->v1, (int)m2, 0, end_bit, s));
   SGT_HIP(hipFreeAsync(tmp, s));
   // unique keys into k0 / v0 (reuse), summing duplicates (P_ij + P_ji)
-  uk = k0;
-  uv = v0;
   tb = 0;
-  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, tb, k1, uk, v1, uv, nruns, hipcub::Sum(), (int)m2, s));
+  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, tb, sp->k1, sp->k0, sp->v1, sp->v0, sp->nruns, hipcub::Sum(),
+                                            (int)m2, s));
   SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
-  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(tmp, tb, k1, uk, v1, uv, nruns, hipcub::Sum(), (int)m2, s));
+  SGT_HIP(hipcub::DeviceReduce::ReduceByKey(tmp, tb, sp->k1, sp->k0, sp->v1, sp->v0, sp->nruns, hipcub::Sum(),
+                                            (int)m2, s));
   SGT_HIP(hipFreeAsync(tmp, s));
-  int runs = 0;
-  SGT_HIP(hipMemcpyAsync(&runs, nruns, sizeof(int), hipMemcpyDeviceToHost, s));
-  SGT_HIP(hipStreamSynchronize(s));
-  tb = 0;
-  SGT_HIP(hipcub::DeviceReduce::Sum(nullptr, tb, uv, total, runs, s));
+  SGT_HIP(hipMemcpyAsync(runs_host, sp->nruns, sizeof(int), hipMemcpyDeviceToHost, s));
+  return SGTSNE_OK;
+}
+
+void sym_release(hipStream_t s, SymPending *sp) {
+  void *bufs[] = {sp->k0, sp->k1, sp->v0, sp->v1, sp->nruns, sp->total};
+  for (void *p : bufs)
+    if (p) (void)hipFreeAsync(p, s);
+  *sp = SymPending{};
+}
+
+int sym_end(hipStream_t s, SymPending *sp, int32_t **rowptr_out, int32_t **col_out, float **val_out,
+            int64_t *nnz_out, bool stream_ordered) {
+  SGT_CHECK(sp->open, SGTSNE_ERR_STATE, "no symmetrisation pending");
+  const int n = sp->n;
+  const int runs = *sp->runs_host;
+  const unsigned long long *uk = sp->k0;
+  const double *uv = sp->v0;
+  size_t tb = 0;
+  void *tmp = nullptr;
+  SGT_HIP(hipcub::DeviceReduce::Sum(nullptr, tb, uv, sp->total, runs, s));
   SGT_HIP(hipMallocAsync(&tmp, tb > 0 ? tb : 1, s));
-  SGT_HIP(hipcub::DeviceReduce::Sum(tmp, tb, uv, total, runs, s));
+  SGT_HIP(hipcub::DeviceReduce::Sum(tmp, tb, uv, sp->total, runs, s));
   SGT_HIP(hipFreeAsync(tmp, s));
   int32_t *rp = nullptr, *cc = nullptr;
   float *vv = nullptr;
@@ -1562,22 +1616,28 @@ int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, c
     SGT_HIP(hipMalloc((void **)&vv, sizeof(float) * (size_t)(runs > 0 ? runs : 1)));
   }
   if (runs > 0) {
-    sym_finish_kernel<<<div_up(runs, 256), 256, 0, s>>>(runs, uk, uv, total, cc, vv);
+    sym_finish_kernel<<<div_up(runs, 256), 256, 0, s>>>(runs, uk, uv, sp->total, cc, vv);
     SGT_LAUNCH_CHECK();
   }
   sym_rowptr_kernel<<<div_up((int64_t)n + 1, 256), 256, 0, s>>>(n, runs, uk, rp);
   SGT_LAUNCH_CHECK();
-  SGT_HIP(hipFreeAsync(k0, s));
-  SGT_HIP(hipFreeAsync(k1, s));
-  SGT_HIP(hipFreeAsync(v0, s));
-  SGT_HIP(hipFreeAsync(v1, s));
-  SGT_HIP(hipFreeAsync(nruns, s));
-  SGT_HIP(hipFreeAsync(total, s));
-  SGT_HIP(hipStreamSynchronize(s));
+  sym_release(s, sp);
   *rowptr_out = rp;
   *col_out = cc;
   *val_out = vv;
   *nnz_out = runs;
+  return SGTSNE_OK;
+}
+
+int gpu_symmetrize(hipStream_t s, int n, int64_t nnz, const int32_t *rowptr_d, const int32_t *col_d,
+                   const float *val_d, int32_t **rowptr_out, int32_t **col_out, float **val_out,
+                   int64_t *nnz_out, bool stream_ordered) {
+  int runs = 0;
+  SymPending sp;
+  SGT_TRY(sym_begin(s, n, nnz, rowptr_d, col_d, val_d, &runs, &sp));
+  SGT_HIP(hipStreamSynchronize(s));
+  SGT_TRY(sym_end(s, &sp, rowptr_out, col_out, val_out, nnz_out, stream_ordered));
+  SGT_HIP(hipStreamSynchronize(s));
   return SGTSNE_OK;
 }
 

@@ -45,7 +45,7 @@ EXPORTED = [
     "sgtsne_ctx_set_timing", "sgtsne_ctx_errors", "sgtsne_ctx_info", "sgtsne_ctx_destroy",
     "sgtsne_ctx_tuning", "sgtsne_ctx_set_attractive", "sgtsne_ctx_reorders",
     "sgtsne_batch_create", "sgtsne_batch_destroy", "sgtsne_batch_add", "sgtsne_batch_add_points",
-    "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_mark", "sgtsne_batch_wait",
+    "sgtsne_batch_add_points_async", "sgtsne_batch_add_poll", "sgtsne_batch_step", "sgtsne_batch_sync", "sgtsne_batch_mark", "sgtsne_batch_wait",
     "sgtsne_batch_passed", "sgtsne_batch_job_left",
     "sgtsne_batch_job_status", "sgtsne_batch_get_Y",
     "sgtsne_batch_terms", "sgtsne_batch_remove", "sgtsne_batch_info", "sgtsne_batch_tuning",
@@ -131,6 +131,9 @@ def lib():
                                        C.POINTER(C.c_int)]),
         "sgtsne_batch_add_points": (C.c_int, [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_double, vp, C.c_int,
                                               C.c_int, C.POINTER(C.c_int)]),
+        "sgtsne_batch_add_points_async": (C.c_int, [vp, f32p, C.c_int, C.c_int, C.c_int, C.c_double, vp, C.c_int,
+                                                    C.c_int, C.POINTER(C.c_int64)]),
+        "sgtsne_batch_add_poll": (C.c_int, [vp, C.c_int64, C.c_int, C.POINTER(C.c_int)]),
         "sgtsne_batch_step": (C.c_int, [vp, C.c_int]),
         "sgtsne_batch_sync": (C.c_int, [vp]),
         "sgtsne_batch_mark": (C.c_int, [vp, C.POINTER(C.c_int64)]),
@@ -470,6 +473,7 @@ class Batch:
         _check(lib().sgtsne_batch_create(C.byref(self._h), C.byref(params), int(max_jobs), int(max_points),
                                          int(max_nnz), int(grid_budget_bytes)))
         self.n = {}
+        self._fronts = {}  # ticket -> (X, y_in, n): arrays the library reads until the admission
 
     def close(self):
         if self._h:
@@ -517,6 +521,33 @@ class Batch:
         st = lib().sgtsne_batch_add_points(self._h, X, n, dim, int(k), float(u), yp, int(max_iter), int(ng_cap),
                                            C.byref(job))
         return self._admit(st, job, n)
+
+    def add_points_async(self, X, k=90, u=30.0, y_in=None, max_iter=0, ng_cap=0):
+        """Queues the first part of a job's front end (kNN) and returns a
+        ticket at once; poll_points(ticket) takes it to the admission.  One
+        front end per batch at a time."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, dim = X.shape
+        keep, yp = self._y_arg(y_in, n)
+        t = C.c_int64(-1)
+        _check(lib().sgtsne_batch_add_points_async(self._h, X, n, dim, int(k), float(u), yp, int(max_iter),
+                                                   int(ng_cap), C.byref(t)))
+        self._fronts[t.value] = (X, keep, n)
+        return t.value
+
+    def poll_points(self, ticket, wait=False):
+        """The job once the front end of `ticket` has been admitted, else None
+        (the device has not passed its queued part yet: the next part is
+        queued by a later poll).  Raises Full when the job does not fit."""
+        job = C.c_int(-1)
+        st = lib().sgtsne_batch_add_poll(self._h, int(ticket), int(bool(wait)), C.byref(job))
+        if st != 0:
+            self._fronts.pop(ticket, None)  # a failed front end is released (an unknown ticket: nothing)
+            return self._admit(st, job, 0)  # raises
+        if job.value >= 0:
+            _, _, n = self._fronts.pop(ticket)
+            return self._admit(st, job, n)
+        return None
 
     def step(self, iters=1):
         _check(lib().sgtsne_batch_step(self._h, int(iters)))

@@ -7,17 +7,24 @@ grids of every admitted job stay resident on the device.  Its loop:
   1. queue the next `tick` iterations of every engine (asynchronous) that has
      fewer than `depth` ticks in flight (checked without blocking), so the
      device always has work while the host admits and retires;
-  2. while the GPU runs them, admit one queued request whose HBM footprint
-     fits, 3-D requests first (their iterations are the longest, so starting
-     them early shortens the makespan), then in arrival order: the front end
-     (kNN, perplexity, P + P^T) runs on the engine's second stream,
-     overlapping the queued iterations, and the loop returns to step 1
-     before the next admission;
+  2. admissions: with front="async" (the default) each engine has at most
+     one front end (kNN, perplexity, P + P^T on the engine's second stream)
+     in flight; the loop polls it without blocking, which queues its next
+     part once the device has passed the previous one and admits the job
+     after the last, then starts the next front end on every engine without
+     one: the first queued request for that engine whose HBM footprint fits,
+     3-D requests first (their iterations are the longest, so starting them
+     early shortens the makespan), then in arrival order.  The host never
+     waits for a kNN, so step 1 keeps both engines fed meanwhile.  With
+     front="sync" one request per loop pass is admitted by a front end that
+     holds the host until its P is built;
   3. retire the jobs whose last iteration the device has passed (checked
      without blocking, so a job still queued holds up neither engine): its Y
-     is read, then its slot is freed.  With nothing to do (every engine at
-     `depth` queued ticks, nothing admissible, nothing finished) the host
-     sleeps `poll_s` before looking again.
+     is read, then its slot is freed.  Reads share the front-end stream, so
+     an engine with a front end in flight retires on a later pass.  With
+     nothing to do (every engine at `depth` queued ticks, no front end
+     advanced, nothing admissible, nothing finished) the host sleeps
+     `poll_s` before looking again.
 
 A job's result does not depend on what else shares its engine (the engine
 guarantees bit-identical trajectories alone and in any mix), so serving
@@ -72,8 +79,10 @@ class Server:
     `share`; each engine admits jobs while their summed footprint fits."""
 
     def __init__(self, hbm_bytes=48 << 30, max_jobs=256, tick=10, k=90, u=30.0, max_iter=1000,
-                 share=(0.5, 0.5), seed=0, depth=3, poll_s=2e-4, engine_factory=None):
-        self.k, self.u, self.max_iter, self.tick = k, u, max_iter, tick
+                 share=(0.5, 0.5), seed=0, depth=3, poll_s=2e-4, engine_factory=None, front="async"):
+        if front not in ("async", "sync"):
+            raise ValueError(f"front must be 'async' or 'sync', not {front!r}")
+        self.k, self.u, self.max_iter, self.tick, self.front = k, u, max_iter, tick, front
         # ticks in flight per engine: an int for both, or {d: depth}
         self.depth = depth if isinstance(depth, dict) else {2: depth, 3: depth}
         if engine_factory is None and (tick > MAX_TICK or max(self.depth.values()) > MAX_DEPTH):
@@ -100,6 +109,7 @@ class Server:
         self.queue = collections.deque()
         self.active = {2: {}, 3: {}}
         self.inflight = {2: collections.deque(), 3: collections.deque()}  # marks of queued ticks
+        self.fronts = {2: None, 3: None}  # (request, footprint, ticket) of each engine's front end in flight
         self.finished = []
 
     def close(self):
@@ -160,6 +170,60 @@ class Server:
         self.queue = collections.deque(sorted(keep, key=lambda q: (q.arrival, q.rid)))
         return admitted
 
+    def _start_fronts(self, now_fn):
+        """Starts a front end on every engine without one in flight: its
+        first queued request that fits (3-D engine first, then arrival
+        order), whose footprint is reserved at once.  Returns how many."""
+        started = 0
+        keep = collections.deque()
+        for r in sorted(self.queue, key=lambda q: (-q.d, q.arrival, q.rid)):
+            if self.fronts[r.d] is not None:
+                keep.append(r)
+                continue
+            fp = footprint_bytes(r.X.shape[0], r.d, self.k)
+            if fp > self.budget[r.d]:
+                r.done, r.Y = now_fn(), None  # can never fit: rejected
+                self.finished.append(r)
+                continue
+            if self.used[r.d] + fp > self.budget[r.d]:
+                keep.append(r)
+                continue
+            y0 = random_init(r.X.shape[0], r.d, seed=r.seed)  # the start depends on the request alone
+            t = self.engines[r.d].add_points_async(r.X, k=self.k, u=self.u, y_in=y0, max_iter=self.max_iter)
+            self.used[r.d] += fp
+            self.fronts[r.d] = (r, fp, t)
+            started += 1
+        self.queue = collections.deque(sorted(keep, key=lambda q: (q.arrival, q.rid)))
+        return started
+
+    def _poll_fronts(self, now_fn):
+        """Advances every front end in flight without blocking; registers
+        the jobs admitted.  Returns whether any front end moved on."""
+        moved = False
+        for d, f in self.fronts.items():
+            if f is None:
+                continue
+            r, fp, t = f
+            try:
+                job = self.engines[d].poll_points(t)
+            except Full:
+                self.fronts[d] = None
+                self.used[d] -= fp
+                if not self.active[d]:
+                    r.done, r.Y = now_fn(), None  # does not fit even an empty engine: rejected
+                    self.finished.append(r)
+                else:
+                    self.queue.append(r)  # retried once jobs retire
+                moved = True
+                continue
+            if job is None:
+                continue
+            self.fronts[d] = None
+            self.used[d] -= fp  # _register counts it again
+            self._register(r, fp, job, now_fn)
+            moved = True
+        return moved
+
     def _advance(self):
         """Queues a tick on every engine whose jobs have iterations left;
         returns whether any tick was queued."""
@@ -187,7 +251,10 @@ class Server:
         """Retires the jobs whose last iteration the device has passed (a
         non-blocking check, so a job still queued holds up neither engine);
         returns how many."""
-        due = [(d, job) for d, act in self.active.items() for job, (r, _) in act.items() if r.t_left <= 0]
+        # an engine with a front end in flight reads on a later pass: the
+        # reads would queue behind the front end's part on the same stream
+        due = [(d, job) for d, act in self.active.items() if self.fronts[d] is None
+               for job, (r, _) in act.items() if r.t_left <= 0]
         ready = [(d, job) for d, job in due if self.engines[d].passed(self.active[d][job][0].end_mark)]
         for d, job in ready:
             r, fp = self.active[d].pop(job)
@@ -201,7 +268,8 @@ class Server:
         return len(ready)
 
     def busy(self):
-        return bool(self.queue) or any(self.active[d] for d in self.active)
+        return (bool(self.queue) or any(self.active[d] for d in self.active)
+                or any(f is not None for f in self.fronts.values()))
 
     def run(self, arrivals, make_X, clock=None):
         """Serves requests arriving at the given times.  arrivals: list of
@@ -219,9 +287,14 @@ class Server:
             if not self.busy():
                 time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
                 continue
-            stepped = self._advance()            # queued on the GPU; returns at once
-            admitted = self._admit(now, limit=1)  # a front end overlaps the queued iterations
-            retired = self._retire(now)
+            stepped = self._advance()  # queued on the GPU; returns at once
+            if self.front == "async":
+                admitted = self._poll_fronts(now)  # never waits for the device
+                retired = self._retire(now)
+                admitted = self._start_fronts(now) or admitted
+            else:
+                admitted = self._admit(now, limit=1)  # a front end overlaps the queued iterations
+                retired = self._retire(now)
             if not (stepped or admitted or retired):
                 time.sleep(self.poll_s)  # every engine has `depth` ticks queued
         return sorted(self.finished, key=lambda r: r.rid)

@@ -363,6 +363,24 @@ int sgtsne_batch_add(sgtsne_batch *batch, int n, int64_t nnz, const int32_t *row
 int sgtsne_batch_add_points(sgtsne_batch *batch, const float *X, int n, int dim, int k, double u, const float *y_in,
                             int max_iter, int ng_cap, int *job);
 
+/* sgtsne_batch_add_points without holding the host: _async queues the first
+ * part of the front end (the kNN up to its certificate) on the batch's
+ * front-end stream and returns a ticket at once; each sgtsne_batch_add_poll
+ * (wait = 0) returns at once with *job = -1 until the device has passed the
+ * queued part, then queues the next one (P and the merge of P and P^T, then
+ * P_sym's CSR), and after the last one admits the job and sets *job.  wait = 1
+ * runs the remaining parts to the admission.  Same kernels in the same order
+ * as sgtsne_batch_add_points, so the same job bits.  One front end per batch
+ * at a time (SGTSNE_ERR_STATE otherwise); X and y_in must stay valid and
+ * unchanged until the poll that admits the job or returns an error.  While a
+ * front end is in flight, reads of finished jobs (sgtsne_batch_get_Y,
+ * sgtsne_batch_job_status) wait for its queued part, as they share its
+ * stream.  A server uses this to keep queuing iterations while a job's kNN
+ * runs.  No counterpart in the reference. */
+int sgtsne_batch_add_points_async(sgtsne_batch *batch, const float *X, int n, int dim, int k, double u,
+                                  const float *y_in, int max_iter, int ng_cap, int64_t *ticket);
+int sgtsne_batch_add_poll(sgtsne_batch *batch, int64_t ticket, int wait, int *job);
+
 /* Queues iters batch iterations (asynchronous; returns before they finish).
  * A newly admitted job spends its first batch iteration centring Y and
  * sizing its grid, then advances one iteration per batch iteration until

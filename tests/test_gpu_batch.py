@@ -288,6 +288,51 @@ def test_batch_job_left(sg, O):
         assert st == sg.JOB_DONE and it == 40
 
 
+def test_batch_async_front_end_same_bits(sg, O):
+    # the front end in parts (add_points_async + non-blocking polls) while
+    # iterations of another job are queued between the polls, on one host
+    # thread: the admitted job runs the same bits as when admitted by
+    # add_points alone; one front end per batch at a time
+    from sgtsne_amd import synth
+
+    X, _ = synth.gaussian_job(1800, seed=61)
+    y0 = synth.random_init(1800, 2, seed=61)
+    with _batch(sg, max_jobs=4) as B:
+        a = B.add_points(X, k=30, u=10.0, y_in=y0, max_iter=40)
+        B.step(41)
+        Y_alone = B.get_Y(a)
+    S, _ = _job(sg, O, 2500, 62)
+    with _batch(sg, max_jobs=4) as B:
+        other = B.add(S, symmetrize=False, max_iter=400)
+        B.step(20)
+        t = B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)
+        with pytest.raises(sg.SgtsneError):
+            B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)  # one in flight already
+        with pytest.raises(sg.SgtsneError):
+            B.poll_points(t + 1)  # no such ticket
+        steps, polls, a = 0, 0, None
+        while a is None and polls < 5000:
+            a = B.poll_points(t)
+            polls += 1
+            if a is None and steps < 300:
+                B.step(1)
+                steps += 1
+        assert a is not None, polls
+        assert polls >= 3  # each of the three parts is taken on by a poll of its own
+        left = B.job_left(a)
+        assert left == 41
+        B.step(left)
+        st, it = B.job_status(a)
+        assert st == sg.JOB_DONE and it == 40
+        assert np.array_equal(B.get_Y(a), Y_alone)
+        assert B.job_left(other) == max(0, 401 - 20 - steps - left)
+        # the batch takes a new front end once the last one is admitted
+        t2 = B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)
+        b = B.poll_points(t2, wait=True)
+        B.step(41)
+        assert np.array_equal(B.get_Y(b), Y_alone)
+
+
 _THREADED_FRONT_END_FAULT = (
     "a front end on a second host thread overlapping steps on the first (the served loop with a worker-thread "
     "admission) faulted the GPU once; no cause was found in the code, so batches are called from one thread "

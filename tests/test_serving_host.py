@@ -19,8 +19,10 @@ from sgtsne_amd import serving
 class FakeEngine:
     """Batch's serving surface over a modelled queue of batch iterations."""
 
-    def __init__(self, d, log, speed, front_end_s=0.0):
+    def __init__(self, d, log, speed, front_end_s=0.0, front_polls=3):
         self.d, self.log, self.speed, self.front_end_s = d, log, speed, front_end_s
+        self.front_polls = front_polls  # polls until an asynchronous front end admits its job
+        self.front = None
         self.lock = threading.Lock()  # the library's batch mutex
         self.host_iter = 0     # batch iterations queued
         self.dev_iter = 0      # batch iterations the device has run
@@ -41,6 +43,21 @@ class FakeEngine:
             self.jobs[job] = (self.host_iter, max_iter, X.shape[0])
             self.log.append(("admit", self.d, X.shape[0]))
             return job
+
+    def add_points_async(self, X, k, u, y_in, max_iter):
+        assert self.front is None, "two front ends in flight on one engine"
+        self.front = [X, max_iter, self.front_polls, len(self.log)]
+        self.log.append(("start", self.d, X.shape[0]))
+        return len(self.log)
+
+    def poll_points(self, ticket, wait=False):
+        f = self.front
+        assert f is not None and ticket == f[3] + 1
+        f[2] -= 1
+        if f[2] > 0 and not wait:
+            return None
+        self.front = None
+        return self.add_points(f[0], None, None, None, f[1])
 
     def step(self, iters):
         with self.lock:
@@ -67,6 +84,7 @@ class FakeEngine:
             return max(0, max_iter + 1 - (self.host_iter - start))
 
     def job_status(self, job):
+        assert self.front is None, "a read queued behind a front end in flight"
         with self.lock:
             start, max_iter, _ = self.jobs[job]
             done = self.dev_iter - start
@@ -85,11 +103,11 @@ class FakeEngine:
         pass
 
 
-def _server(log, speed=7, front_end_s=0.0, **kw):
+def _server(log, speed=7, front_end_s=0.0, front_polls=3, **kw):
     engines = {}
 
     def factory(params, max_jobs, pts, nnz, grid_budget):
-        e = FakeEngine(params.d, log, speed, front_end_s)
+        e = FakeEngine(params.d, log, speed, front_end_s, front_polls)
         engines[params.d] = e
         return e
 
@@ -110,6 +128,50 @@ def test_server_finishes_every_request_after_its_iterations():
         assert e.max_queued <= 3 * 10
         assert not e.jobs
     assert all(r.admitted >= r.arrival and r.done >= r.admitted for r in got)
+
+
+@pytest.mark.parametrize("front", ["async", "sync"])
+def test_server_both_front_end_modes_finish(front):
+    log = []
+    srv, eng = _server(log, front=front, front_end_s=0.0005)
+    arrivals = [(0.0003 * i, 400 + 53 * i, 2 + (i % 2), i) for i in range(10)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 10 and all(r.Y is not None for r in got)
+    assert sum(1 for x in log if x[0] == "admit") == 10
+    assert sum(1 for x in log if x[0] == "start") == (10 if front == "async" else 0)
+
+
+def test_server_async_front_ends_keep_engines_fed():
+    # while one engine's front end is in flight, the loop keeps queuing ticks
+    # for the jobs already admitted (each poll returns at once), holds at most
+    # one front end per engine and starts the 3-D one first
+    log = []
+    srv, eng = _server(log, speed=2, front_polls=40, max_iter=200)
+    steps = []
+    for e in eng.values():
+        orig = e.step
+
+        def step(iters, e=e, orig=orig):
+            steps.append((e.d, len(log)))
+            orig(iters)
+
+        e.step = step
+    arrivals = [(0.0, 600, 2, 0), (0.0, 700, 3, 1), (0.0, 500, 2, 2), (0.0, 800, 3, 3)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 4 and all(r.Y is not None for r in got)
+    starts = [i for i, x in enumerate(log) if x[0] == "start"]
+    assert log[starts[0]][1] == 3  # the 3-D engine's front end goes first
+    # ticks queued on the 2-D engine while the second 2-D front end was in flight
+    s2 = [i for i in starts if log[i][1] == 2][1]
+    a2 = log.index(("admit", 2, log[s2][2]))
+    assert any(d == 2 and s2 < at <= a2 for d, at in steps)
+    # at most one front end per engine: each start is followed by its
+    # engine's admission before that engine's next start
+    for d in (2, 3):
+        seq = [x[0] for x in log if x[0] in ("start", "admit") and x[1] == d]
+        assert seq == ["start", "admit"] * 2
 
 
 def test_server_admits_3d_first_one_per_pass_and_rejects_oversize():
