@@ -69,8 +69,12 @@ constexpr int kMarks = 16;                 // tick marks kept
 // queues the next part once the device has passed the previous one, so no
 // part waits on the host.  The parts end where the host needs a count from
 // the device: the kNN rows left uncertified, then the nonzeros of P + P^T.
+// Up to kFrontEnds of them are in flight at once: their parts queue behind
+// one another on the stream, so the device runs one front end's part while
+// the host takes up another's.
+constexpr int kFrontEnds = 8;
 struct FrontEnd {
-  int stage = 0;  // 0 none, 1 kNN queued, 2 P queued up to its merge, 3 P_sym queued
+  int stage = 0;  // 0 free, 1 kNN queued, 2 P queued up to its merge, 3 P_sym queued
   int64_t ticket = 0;
   int n = 0, dim = 0, k = 0, max_iter = 0, ng_cap = 0;
   double u = 0.0;
@@ -81,6 +85,8 @@ struct FrontEnd {
   int64_t m = 0;
   KnnPending knn;
   SymPending sym;
+  hipEvent_t ev = nullptr;  // after the queued part (kept across uses of the slot)
+  int *counts = nullptr;    // pinned: [0] uncertified kNN rows, [1] nonzeros of P + P^T
 };
 
 }  // namespace
@@ -97,10 +103,9 @@ struct sgtsne_batch {
   // and do not depend on which queues earlier streams of the process left
   // busiest
   hipStream_t copy = nullptr;
-  hipEvent_t ev_prep = nullptr;  // after the queued part of the front end
-  FrontEnd fe;
+  FrontEnd fe[kFrontEnds];
   int64_t fe_tickets = 0;
-  int *fe_host = nullptr;  // pinned: the counts the front end's parts read back
+  int *fe_host = nullptr;  // pinned: two counts per front-end slot
   // host -> device updates go through a pinned ring, so queuing them never
   // waits for the iterations queued ahead of them
   char *stage = nullptr;
@@ -797,13 +802,15 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   return SGTSNE_OK;
 }
 
-void fe_release(sgtsne_batch *b);
+void fe_release(sgtsne_batch *b, FrontEnd &f);
 
 void destroy_all(sgtsne_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->fe.stage != 0) {
-    (void)hipStreamSynchronize(b->prep);
-    fe_release(b);
+  if (b->prep) (void)hipStreamSynchronize(b->prep);
+  for (FrontEnd &f : b->fe) {
+    if (f.stage != 0) fe_release(b, f);
+    if (f.ev) (void)hipEventDestroy(f.ev);
+    f.ev = nullptr;
   }
   release_graph(b);
   release_deferred(b, true);
@@ -860,7 +867,6 @@ void destroy_all(sgtsne_batch *b) {
       (void)hipEventDestroy(e);
       e = nullptr;
     }
-  if (b->ev_prep) (void)hipEventDestroy(b->ev_prep);
   if (b->fe_host) (void)hipHostFree(b->fe_host);
   b->fe_host = nullptr;
   if (b->prep) (void)hipStreamDestroy(b->prep);
@@ -895,24 +901,31 @@ int fetch_points(sgtsne_batch *b, int job, const float *src, float *out, int lay
   return SGTSNE_OK;
 }
 
+// Frees the slot for the next front end (its event and pinned counts stay).
+void fe_clear(FrontEnd &f) {
+  hipEvent_t ev = f.ev;
+  int *counts = f.counts;
+  f = FrontEnd{};
+  f.ev = ev;
+  f.counts = counts;
+}
+
 // Releases a front end's device temporaries (stream-ordered on the
 // front-end stream, after whatever of it is queued) and clears it.
-void fe_release(sgtsne_batch *b) {
-  FrontEnd &f = b->fe;
+void fe_release(sgtsne_batch *b, FrontEnd &f) {
   const hipStream_t s = b->prep;
   knn_release(s, &f.knn);
   sym_release(s, &f.sym);
   void *bufs[] = {f.Xf, f.Xd, f.Dd, f.Id, f.rp, f.cc, f.vv, f.srp, f.scc, f.svv};
   for (void *q : bufs)
     if (q) (void)hipFreeAsync(q, s);
-  f = FrontEnd{};
+  fe_clear(f);
 }
 
 // Queues the front end's next part; the device has passed the previous one.
 // Returns 1 once the job is admitted (*job set), 0 when a further part is
 // queued, or an error status (the front end is then released).
-int fe_advance(sgtsne_batch *b, int *job) {
-  FrontEnd &f = b->fe;
+int fe_advance(sgtsne_batch *b, FrontEnd &f, int *job) {
   const hipStream_t s = b->prep;
   int st = SGTSNE_OK;
   if (f.stage == 1) {  // kNN rows certified: the uncertified ones, P, and the merge of P and P^T
@@ -920,7 +933,7 @@ int fe_advance(sgtsne_batch *b, int *job) {
     if (!st) st = launch_perplexity(s, f.Dd, f.n, f.k, f.u, f.vv, nullptr);
     if (!st) st = launch_knn_csr(s, f.n, f.k, f.Id, f.rp, f.cc);
     if (!st && b->p.lambda != 1.0) st = launch_lambda_rescale(s, f.n, f.rp, f.vv, b->p.lambda, b->p.drop_leaf);
-    if (!st) st = sym_begin(s, f.n, (int64_t)f.n * f.k, f.rp, f.cc, f.vv, b->fe_host + 1, &f.sym);
+    if (!st) st = sym_begin(s, f.n, (int64_t)f.n * f.k, f.rp, f.cc, f.vv, f.counts + 1, &f.sym);
   } else if (f.stage == 2) {  // merged count known: P_sym's CSR
     st = sym_end(s, &f.sym, &f.srp, &f.scc, &f.svv, &f.m, true);
     // released in stream order, after the kernels just queued that read them
@@ -935,20 +948,20 @@ int fe_advance(sgtsne_batch *b, int *job) {
     release_deferred(b);
     st = admit(b, f.n, f.m, f.srp, f.scc, f.svv, f.y_in, f.max_iter, f.ng_cap, job);
     if (st) {
-      fe_release(b);
+      fe_release(b, f);
       return st;
     }
     // the admitted P is read by copies queued on the iteration stream
     st = defer_free(b, f.srp, f.scc, f.svv);
-    f = FrontEnd{};
+    fe_clear(f);
     return st ? st : 1;
   }
-  if (!st && hipEventRecord(b->ev_prep, s) != hipSuccess) {
+  if (!st && hipEventRecord(f.ev, s) != hipSuccess) {
     set_error("front end: event record failed");
     st = SGTSNE_ERR_HIP;
   }
   if (st) {
-    fe_release(b);
+    fe_release(b, f);
     return st;
   }
   ++f.stage;
@@ -1000,10 +1013,13 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   chk(hipStreamCreateWithFlags(&b->prep, hipStreamNonBlocking), "stream");
   b->copy = b->prep;
-  chk(hipEventCreateWithFlags(&b->ev_prep, hipEventDisableTiming), "event");
   for (hipEvent_t &e : b->marks) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
   chk(hipHostMalloc((void **)&b->stage, kStageBytes, hipHostMallocDefault), "pinned staging");
-  chk(hipHostMalloc((void **)&b->fe_host, 256, hipHostMallocDefault), "pinned counts");
+  chk(hipHostMalloc((void **)&b->fe_host, sizeof(int) * 2 * kFrontEnds, hipHostMallocDefault), "pinned counts");
+  for (int i = 0; i < kFrontEnds; ++i) {
+    chk(hipEventCreateWithFlags(&b->fe[i].ev, hipEventDisableTiming), "event");
+    b->fe[i].counts = b->fe_host ? b->fe_host + 2 * i : nullptr;
+  }
   const size_t P = (size_t)max_points;
   if (!st) st = balloc((void **)&b->rowptr, sizeof(int32_t) * (P + 1));
   if (!st) st = balloc((void **)&b->col, sizeof(int32_t) * std::max<int64_t>(max_nnz, 1));
@@ -1109,10 +1125,16 @@ int sgtsne_batch_add_points_async(sgtsne_batch *b, const float *X, int n, int di
   SGT_TRY(check_batch(b));
   SGT_CHECK(X && n > k && k >= 1 && dim >= 1 && u > 0, SGTSNE_ERR_ARG, "bad kNN arguments");
   SGT_CHECK(ticket != nullptr, SGTSNE_ERR_ARG, "ticket is NULL");
-  SGT_CHECK(b->fe.stage == 0, SGTSNE_ERR_STATE, "front end %lld is in flight: poll it to its admission first",
-            (long long)b->fe.ticket);
-  FrontEnd &f = b->fe;
-  f = FrontEnd{};
+  FrontEnd *slot = nullptr;
+  for (FrontEnd &c : b->fe)
+    if (c.stage == 0) {
+      slot = &c;
+      break;
+    }
+  SGT_CHECK(slot != nullptr, SGTSNE_ERR_STATE, "%d front ends are in flight: poll one to its admission first",
+            kFrontEnds);
+  FrontEnd &f = *slot;
+  fe_clear(f);
   f.n = n;
   f.dim = dim;
   f.k = k;
@@ -1139,10 +1161,10 @@ int sgtsne_batch_add_points_async(sgtsne_batch *b, const float *X, int n, int di
   chk(hipMallocAsync((void **)&f.vv, (size_t)n * k * sizeof(float), s), "hipMallocAsync");
   chk(hipMemcpyAsync(f.Xf, X, nx * sizeof(float), hipMemcpyHostToDevice, s), "copy of X");
   if (!st) st = launch_to_double(s, f.Xf, f.Xd, (int64_t)nx);
-  if (!st) st = knn_begin(s, f.Xd, n, dim, k, f.Id, f.Dd, SGTSNE_KNN_AUTO, b->fe_host, &f.knn);
-  chk(hipEventRecord(b->ev_prep, s), "event record");
+  if (!st) st = knn_begin(s, f.Xd, n, dim, k, f.Id, f.Dd, SGTSNE_KNN_AUTO, f.counts, &f.knn);
+  chk(hipEventRecord(f.ev, s), "event record");
   if (st) {
-    fe_release(b);
+    fe_release(b, f);
     return st;
   }
   f.stage = 1;
@@ -1155,17 +1177,20 @@ int sgtsne_batch_add_poll(sgtsne_batch *b, int64_t ticket, int wait, int *job) {
   SGT_TRY(check_batch(b));
   SGT_CHECK(job != nullptr, SGTSNE_ERR_ARG, "job is NULL");
   *job = -1;
-  SGT_CHECK(b->fe.stage != 0 && ticket == b->fe.ticket, SGTSNE_ERR_ARG, "no front end %lld in flight",
-            (long long)ticket);
+  FrontEnd *slot = nullptr;
+  for (FrontEnd &c : b->fe)
+    if (c.stage != 0 && c.ticket == ticket) slot = &c;
+  SGT_CHECK(slot != nullptr, SGTSNE_ERR_ARG, "no front end %lld in flight", (long long)ticket);
+  FrontEnd &f = *slot;
   for (;;) {
-    const hipError_t e = wait ? hipEventSynchronize(b->ev_prep) : hipEventQuery(b->ev_prep);
+    const hipError_t e = wait ? hipEventSynchronize(f.ev) : hipEventQuery(f.ev);
     if (e == hipErrorNotReady) return SGTSNE_OK;
     if (e != hipSuccess) {
       set_error("front end failed: %s", hipGetErrorString(e));
-      fe_release(b);
+      fe_release(b, f);
       return SGTSNE_ERR_HIP;
     }
-    const int r = fe_advance(b, job);
+    const int r = fe_advance(b, f, job);
     if (r == 1) return SGTSNE_OK;
     if (r != 0) return r;
     if (!wait) return SGTSNE_OK;  // the next part is queued; a later poll takes it on

@@ -524,8 +524,8 @@ class Batch:
 
     def add_points_async(self, X, k=90, u=30.0, y_in=None, max_iter=0, ng_cap=0):
         """Queues the first part of a job's front end (kNN) and returns a
-        ticket at once; poll_points(ticket) takes it to the admission.  One
-        front end per batch at a time."""
+        ticket at once; poll_points(ticket) takes it to the admission.  Up to
+        8 front ends per batch in flight."""
         X = np.ascontiguousarray(X, dtype=np.float32)
         n, dim = X.shape
         keep, yp = self._y_arg(y_in, n)

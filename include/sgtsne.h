@@ -370,13 +370,14 @@ int sgtsne_batch_add_points(sgtsne_batch *batch, const float *X, int n, int dim,
  * queued part, then queues the next one (P and the merge of P and P^T, then
  * P_sym's CSR), and after the last one admits the job and sets *job.  wait = 1
  * runs the remaining parts to the admission.  Same kernels in the same order
- * as sgtsne_batch_add_points, so the same job bits.  One front end per batch
- * at a time (SGTSNE_ERR_STATE otherwise); X and y_in must stay valid and
- * unchanged until the poll that admits the job or returns an error.  While a
- * front end is in flight, reads of finished jobs (sgtsne_batch_get_Y,
- * sgtsne_batch_job_status) wait for its queued part, as they share its
- * stream.  A server uses this to keep queuing iterations while a job's kNN
- * runs.  No counterpart in the reference. */
+ * as sgtsne_batch_add_points, so the same job bits.  Up to 8 front ends per
+ * batch in flight (SGTSNE_ERR_STATE beyond): their parts queue one behind
+ * another on the front-end stream, so the device runs one while the host takes
+ * up the next.  X and y_in must stay valid and unchanged until the poll that
+ * admits the job or returns an error.  While a front end is in flight, reads
+ * of finished jobs (sgtsne_batch_get_Y, sgtsne_batch_job_status) wait for its
+ * queued part, as they share its stream.  A server uses this to keep queuing
+ * iterations while a job's kNN runs.  No counterpart in the reference. */
 int sgtsne_batch_add_points_async(sgtsne_batch *batch, const float *X, int n, int dim, int k, double u,
                                   const float *y_in, int max_iter, int ng_cap, int64_t *ticket);
 int sgtsne_batch_add_poll(sgtsne_batch *batch, int64_t ticket, int wait, int *job);

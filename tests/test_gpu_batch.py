@@ -5,6 +5,8 @@ Bars: each job's per-iteration terms within 1e-4 relative of the FP64 oracle
 it runs alone or among other jobs (admitted earlier or later, removed on the
 way), with or without graph replay; final KL of a 1000-iteration job within
 1% of the FP64 loop."""
+import time
+
 import numpy as np
 import pytest
 
@@ -292,7 +294,7 @@ def test_batch_async_front_end_same_bits(sg, O):
     # the front end in parts (add_points_async + non-blocking polls) while
     # iterations of another job are queued between the polls, on one host
     # thread: the admitted job runs the same bits as when admitted by
-    # add_points alone; one front end per batch at a time
+    # add_points alone; several front ends in flight at once do too
     from sgtsne_amd import synth
 
     X, _ = synth.gaussian_job(1800, seed=61)
@@ -302,12 +304,10 @@ def test_batch_async_front_end_same_bits(sg, O):
         B.step(41)
         Y_alone = B.get_Y(a)
     S, _ = _job(sg, O, 2500, 62)
-    with _batch(sg, max_jobs=4) as B:
+    with _batch(sg, max_jobs=12) as B:
         other = B.add(S, symmetrize=False, max_iter=400)
         B.step(20)
         t = B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)
-        with pytest.raises(sg.SgtsneError):
-            B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)  # one in flight already
         with pytest.raises(sg.SgtsneError):
             B.poll_points(t + 1)  # no such ticket
         steps, polls, a = 0, 0, None
@@ -326,11 +326,24 @@ def test_batch_async_front_end_same_bits(sg, O):
         assert st == sg.JOB_DONE and it == 40
         assert np.array_equal(B.get_Y(a), Y_alone)
         assert B.job_left(other) == max(0, 401 - 20 - steps - left)
-        # the batch takes a new front end once the last one is admitted
-        t2 = B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)
-        b = B.poll_points(t2, wait=True)
+        # eight front ends in flight (a ninth is refused), polled in turn:
+        # every job admitted runs the same bits as the one admitted alone
+        ts = [B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40) for _ in range(8)]
+        with pytest.raises(sg.SgtsneError):
+            B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40)
+        jobs = {}
+        t_end = time.perf_counter() + 60.0
+        while len(jobs) < len(ts) and time.perf_counter() < t_end:
+            for t in ts:
+                if t not in jobs:
+                    j = B.poll_points(t)
+                    if j is not None:
+                        jobs[t] = j
+        assert len(jobs) == len(ts)
+        assert B.poll_points(B.add_points_async(X, k=30, u=10.0, y_in=y0, max_iter=40), wait=True) is not None
         B.step(41)
-        assert np.array_equal(B.get_Y(b), Y_alone)
+        for j in jobs.values():
+            assert np.array_equal(B.get_Y(j), Y_alone)
 
 
 _THREADED_FRONT_END_FAULT = (
