@@ -688,6 +688,8 @@ struct RefTiles {
   const double *cb, *qa, *dn;
   const unsigned long long *rmax;
   int dim;
+  const int32_t *cell;  // point -> cell
+  const int *slot0;     // cell -> its first slot (a tile boundary)
 };
 
 template <int NST>
@@ -739,14 +741,15 @@ __global__ __launch_bounds__(128) void knn_filter_lds_kernel(const RefTiles rf, 
     const double r = wave_max(q >= 0 ? rf.qa[q] * (1.0 + 0x1p-20) : 0.0);
     if (lane == 0) srho = r;
   }
+  // the first tile visited: kFiltHomeBack tiles before the block's own, but
+  // not before its cell's first tile, so the lists fill from the block's
+  // own cell and its threshold prunes the other cells' tiles early
   const int home = rf.rinv[qorder[q0]] / kFiltR;
+  const int start = max(home - kFiltHomeBack, rf.slot0[rf.cell[qorder[q0]]] / kFiltR);
   const double R = __longlong_as_double((long long)*rf.rmax);
   const double dnb = rf.dn[blockIdx.x];
   const double *cbb = rf.cb + (size_t)blockIdx.x * rf.dim;
-  auto visit = [&](int sq) {  // the sq-th tile visited
-    int t = (home - kFiltHomeBack + sq) % ntiles;
-    return t < 0 ? t + ntiles : t;
-  };
+  auto visit = [&](int sq) { return (start + sq) % ntiles; };  // the sq-th tile visited
   float4 buf[kFiltR * NST / 128];
   float rnv;
   for (int g0 = 0; g0 < ntiles; g0 += 128) {
@@ -1426,7 +1429,8 @@ int knn_begin(hipStream_t s, const double *X, int n, int dim, int k, int32_t *I,
     SGT_LAUNCH_CHECK();
     knn_query_prep_kernel<<<div_up(n, 256), 256, 0, s>>>(X, n, dim, dp, qorder, cb, dn, Xq, qn32, qa, qdn, kF2Q);
     SGT_LAUNCH_CHECK();
-    bc = BlockCentred{Xq, qn32, qorder, delta, RefTiles{Xr, rid, rinv, ntiles, tcen, trad, cb, qa, dn, rmax, dim}};
+    bc = BlockCentred{Xq, qn32, qorder, delta,
+                      RefTiles{Xr, rid, rinv, ntiles, tcen, trad, cb, qa, dn, rmax, dim, cell, ccnt + 2 * kKnnMaxCells}};
   } else {
     SGT_HIP(hipMemsetAsync(qdn, 0, sizeof(double) * (size_t)n, s));  // frame of every query: mu
   }
