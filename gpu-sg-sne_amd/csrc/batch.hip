@@ -138,6 +138,7 @@ struct sgtsne_batch {
   // job table and work items
   BatchJob *jobs_d = nullptr;
   std::vector<BatchJob> jobs_h;  // static part mirrored on the host
+  std::vector<BatchJob> jobs_up; // static part as last uploaded (the device's copy, in stream order)
   std::vector<HostJob> hj;
   std::vector<int> arena;        // live slots in ascending off
   BatchItem *items_d = nullptr;
@@ -339,8 +340,18 @@ void release_deferred(sgtsne_batch *b, bool all = false) {
 }
 
 // Writes the static part of slot j's record (everything before the dynamic
-// fields) so that a running job's iteration state is left alone.
-int write_static(sgtsne_batch *b, int j) { return upload(b, b->jobs_d + j, &b->jobs_h[j], offsetof(BatchJob, iter)); }
+// fields) so that a running job's iteration state is left alone; skipped
+// when the device already holds it (an admission at the tail of the arena
+// changes only the new job's record, so admitting J jobs makes J uploads,
+// not J^2 / 2).
+constexpr size_t kJobStatic = offsetof(BatchJob, iter);
+static_assert(kJobStatic == 10 * sizeof(int) + 6 * sizeof(float) + sizeof(double) + 6 * sizeof(void *),
+              "the static part of a job record has no padding bytes (it is compared with memcmp)");
+int write_static(sgtsne_batch *b, int j) {
+  if (std::memcmp(&b->jobs_h[j], &b->jobs_up[j], kJobStatic) == 0) return SGTSNE_OK;
+  std::memcpy(&b->jobs_up[j], &b->jobs_h[j], kJobStatic);
+  return upload(b, b->jobs_d + j, &b->jobs_h[j], kJobStatic);
+}
 
 // Host-side check of the tables a step reads, before they are uploaded: the
 // running jobs' point and nonzero ranges lie inside the arena and do not
@@ -789,6 +800,7 @@ int admit(sgtsne_batch *b, int n, int64_t nnz, const int32_t *rp_d, const int32_
   J.iter = 0;
   J.status = BJ_INIT;
   SGT_TRY(upload(b, b->jobs_d + slot, &J, sizeof(BatchJob)));
+  std::memcpy(&b->jobs_up[slot], &J, kJobStatic);
   if (!h.done_ev) SGT_HIP(hipEventCreateWithFlags(&h.done_ev, hipEventDisableTiming));
 
   h.live = true;
@@ -1000,6 +1012,7 @@ int sgtsne_batch_create(sgtsne_batch **out, const sgtsne_params *params, int max
   b->grid_budget = grid_budget_bytes > 0 ? (size_t)grid_budget_bytes : ((size_t)64 << 30);
   b->hj.resize(max_jobs);
   b->jobs_h.resize(max_jobs);
+  b->jobs_up.resize(max_jobs);  // zero, as the device table is cleared below
   b->items_cap = (int)(max_points / kBatchChunk + max_jobs + 1);
   const int np = batch_npasses(b->d);
   int st = SGTSNE_OK;
@@ -1348,6 +1361,7 @@ int sgtsne_batch_remove(sgtsne_batch *b, int job) {
   BatchJob empty{};
   empty.status = BJ_EMPTY;
   SGT_TRY(upload(b, b->jobs_d + job, &empty, sizeof(BatchJob)));
+  std::memcpy(&b->jobs_up[job], &empty, kJobStatic);
   release_deferred(b);
   SGT_TRY(defer_free(b, h.gmem));
   if (h.done_ev) (void)hipEventDestroy(h.done_ev);
