@@ -58,6 +58,12 @@ def test_argument_errors_without_device(sg):
     assert b"d must be" in lib.sgtsne_last_error()
     p = sg.tsneparams(format=1)
     assert lib.sgtsne_ctx_create(C.byref(h), C.byref(p)) == -1
+    # the asynchronous front end of a batch: a NULL batch is an argument error
+    t = C.c_int64(-1)
+    job = C.c_int(7)
+    assert lib.sgtsne_batch_add_points_async(None, X, 3, 2, 1, 10.0, None, 0, 0, C.byref(t)) == -1
+    assert t.value == -1
+    assert lib.sgtsne_batch_add_poll(None, 1, 0, C.byref(job)) == -1
 
 
 def test_product_never_uses_the_oracle():
