@@ -174,6 +174,44 @@ def test_server_async_front_ends_keep_engines_fed():
         assert seq == ["start", "admit"] * 2
 
 
+def test_server_async_front_end_full_is_retried_or_rejected():
+    # an admission that finds the engine full (sgtsne_batch_add_poll returns
+    # SGTSNE_ERR_FULL after the front end) puts the request back in the queue
+    # while the engine has jobs, and rejects it when the engine is empty
+    log = []
+    srv, eng = _server(log, max_iter=30, front_polls=2)
+    e2 = eng[2]
+    orig = e2.poll_points
+    state = {"full_once": True}
+
+    def poll(ticket, wait=False):
+        if state["full_once"] and e2.jobs:
+            state["full_once"] = False
+            e2.front = None
+            raise serving.Full("arena full")
+        return orig(ticket, wait)
+
+    e2.poll_points = poll
+    arrivals = [(0.0, 500, 2, 0), (0.0, 600, 2, 1)]
+    with srv:
+        got = srv.run(arrivals, lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 2 and all(r.Y is not None for r in got)
+    assert not state["full_once"]  # the second request was refused once, then admitted
+    assert sum(1 for x in log if x == ("start", 2, 600)) == 2
+
+    log = []
+    srv, eng = _server(log, max_iter=30)
+
+    def always_full(ticket, wait=False):
+        eng[3].front = None
+        raise serving.Full("never fits")
+
+    eng[3].poll_points = always_full
+    with srv:
+        got = srv.run([(0.0, 700, 3, 5)], lambda n, s: np.zeros((n, 4), np.float32))
+    assert len(got) == 1 and got[0].Y is None  # an empty engine that cannot take it: rejected
+
+
 def test_server_admits_3d_first_one_per_pass_and_rejects_oversize():
     log = []
     srv, eng = _server(log, hbm_bytes=1 << 30)
